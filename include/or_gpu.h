@@ -1,0 +1,117 @@
+/*
+ * or_gpu.h -- C ABI of the MI355X explicit element-force / time-integration path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (OR_OK == 0); or_last_error() returns the message of the last failure of
+ * the calling thread.  Host arrays use the reference's Fortran layouts:
+ *   X(3,NUMNOD), V(3,NUMNOD), D(3,NUMNOD)   node-major triples (nodal_arrays.F90:109-177)
+ *   SIG(NEL,6)                              component-major (elbufdef_mod.F90 g_bufel_%SIG)
+ *   IXS(11,NUMELS) rows 2..9                1-based node ids (element_mod.F90:1174)
+ *   FSKY(8,LSKY), IADS(8,*), ADSKY(NUMNOD+1) Parith/ON skyline (scumu3p.F, asspar4.F:164-180)
+ * The resident-model API takes 0-based indices (documented per call).
+ *
+ * Reference interfaces replaced (file:line in the OpenRadioss tree):
+ *   or_model_*            the resident GPU handle of shell_gpu_mod.F90:279-715 /
+ *                         shell_gpu_driver.h:35-210 (create/allocate/upload/step/download/destroy),
+ *                         extended to hex elements and to the nodal update.
+ *   or_model_run          one RESOL cycle of the path per iteration: FORINT -> SFORC3
+ *                         (resol.F:4225, forint.F:1159), ASSPAR4 (resol.F:4858), the DT2
+ *                         pipeline (resol.F:6125-6129, 6352, 6494-6497), ACCELE (resol.F:6921),
+ *                         BCS10 (resol.F:7322), FIXVEL (resol.F:7610), VELOCITY (resol.F:8947),
+ *                         DEPLA (resol.F:9042), time advance (resol.F:8599-8613).
+ *   or_sforc3_group       SFORC3 for one element group (sforc3.F:131-166) on the Isolid=1,
+ *                         law-2, Lagrangian branch; writes FSKY rows like SCUMU3P (scumu3p.F:37).
+ *   or_model_energy       internal / kinetic energy sums of ECRIT (ecrit.F:222-275, 344).
+ */
+#ifndef OR_GPU_H
+#define OR_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_OK 0
+#define OR_ERR_ARG 1      /* invalid argument (sizes, null pointers, unsupported option) */
+#define OR_ERR_HIP 2      /* HIP runtime failure */
+#define OR_ERR_NODEV 3    /* no usable GPU */
+#define OR_ERR_STATE 4    /* call out of order (e.g. run before upload) */
+#define OR_ERR_MSTOP 5    /* an element reported a non-positive Jacobian (MSTOP=1) */
+
+#define OR_PREC_FP64 0    /* plain method: everything in FP64 */
+#define OR_PREC_SPLIT 1   /* FP32 element arithmetic/state, FP64 nodes and reductions */
+
+/* Law 2 (Johnson-Cook) + /PROP/SOLID Isolid=1 + run control, resolved as the
+ * Starter stores them (hm_read_mat02_jc.F90, hm_read_prop14.F:355-369). */
+typedef struct {
+  double rho0, bulk, shear;
+  double ca, cb, cn, epmx, sigmx, cc, epdr, fisokin, asrate, m_exp;
+  int32_t icc, vp, israte, iform;
+  double qa, qb, qh, vns1, vns2;
+  double dtfac; /* DTFAC1(1) */
+  double dtmx;  /* DTMX */
+} or_hex_params;
+
+/* Cycle scalars (resol.F: TT, DT1, DT2, DT12, DT2OLD, NCYCLE, MSTOP). */
+typedef struct {
+  double tt, dt1, dt2, dt12, dt2old;
+  int64_t ncycle;
+  int32_t mstop;
+} or_time_state;
+
+typedef struct or_model or_model;
+
+/* ---- library ---- */
+int or_version(void);
+const char *or_last_error(void);
+int or_device_count(int *count);
+
+/* ---- resident model (upload once, step N cycles, state stays in HBM) ---- */
+/* nn nodes, ne_hex 8-node hexes, precision OR_PREC_*. */
+int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int precision);
+int or_model_destroy(or_model *m);
+int or_model_set_hex_params(or_model *m, const or_hex_params *p);
+/* x,v: X(3,nn), V(3,nn); ms: (nn); bcs: (nn) BCS code (4=x,2=y,1=z), may be NULL;
+ * vimp: V(3,nn) imposed velocities used where impcode bit (4=x,2=y,1=z) is set, may be NULL. */
+int or_model_upload_nodes(or_model *m, const double *x, const double *v, const double *ms,
+                          const int32_t *bcs, const int32_t *impcode, const double *vimp);
+/* conn: (ne,8) 0-based node ids; state arrays: sig SIG(NEL,6) component-major over all hexes,
+ * eint, rho, vol0, pla, epsd, offg, qvis (ne each). */
+int or_model_upload_hex(or_model *m, const int32_t *conn, const double *sig, const double *eint,
+                        const double *rho, const double *vol0, const double *pla, const double *epsd,
+                        const double *offg, const double *qvis);
+/* adsky: (nn+1) 0-based CSR pointers; iads: (ne,8) 0-based row of each element node. */
+int or_model_upload_skyline(or_model *m, const int64_t *adsky, const int64_t *iads);
+int or_model_set_time(or_model *m, const or_time_state *ts);
+int or_model_get_time(or_model *m, or_time_state *ts);
+/* Run ncycles cycles asynchronously on the model's stream. */
+int or_model_run(or_model *m, int64_t ncycles);
+int or_model_sync(or_model *m);
+int or_model_download_nodes(or_model *m, double *x, double *v, double *d, double *stifn);
+int or_model_download_hex(or_model *m, double *sig, double *eint, double *rho, double *pla,
+                          double *epsd, double *offg, double *qvis);
+/* out[0] = sum EINT*VOL0 (internal), out[1] = 0.5*sum MS*|V|^2 (kinetic);
+ * fixed-order deterministic reductions in FP64. */
+int or_model_energy(or_model *m, double out[2]);
+/* Average device time per launch (ms) of the element and node kernels over the
+ * cycles run since the last reset, measured with HIP events on the model stream. */
+int or_model_kernel_times(or_model *m, int reset, double *elem_ms, double *node_ms, int64_t *cycles);
+int or_model_set_timing(or_model *m, int enable);
+/* Bytes of device memory held by the model. */
+int or_model_footprint(const or_model *m, int64_t *bytes);
+
+/* ---- group-level drop-in for SFORC3 (one group of nel <= 128 hexes) ----
+ * Host arrays in the reference layouts: x X(3,numnod), v V(3,numnod), ixs IXS(11,*)
+ * with the group starting at column nft+1, GBUF arrays of the group (sig SIG(nel,6)),
+ * fsky FSKY(8,lsky), iads IADS(8,*) with 1-based rows (column nft+i for element i).
+ * *dt2t is min-folded with the group's element time steps like MQVISCB.  dt1 = DT1. */
+int or_sforc3_group(int device, const or_hex_params *p, int64_t numnod, int32_t nel, int64_t nft,
+                    const double *x, const double *v, const int32_t *ixs, double *sig, double *eint,
+                    double *rho, const double *vol, double *pla, double *epsd, double *off,
+                    double *qvis, int64_t lsky, double *fsky, const int32_t *iads, double dt1,
+                    double *dt2t, int32_t *mstop);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
