@@ -253,3 +253,330 @@ def cantilever_bar(nx: int = 20, ny: int = 20, nz: int = 80, h: float = 1.0, v0:
     v[:, 2] = v0
     bcs = np.where(x[:, 2] == 0.0, 7, 0).astype(np.int32)
     return build_hex_model(x, conn, mat, v0=v, bcs=bcs)
+
+
+# ---------------------------------------------------------------------------
+# Q4 shells and mixed hex + Q4 models (6-DOF nodes)
+# ---------------------------------------------------------------------------
+
+# node code bits (BCS translations / rotations, imposed velocity), see bcs1.F LCOD
+BC_TX, BC_TY, BC_TZ = 4, 2, 1
+BC_RX, BC_RY, BC_RZ = 32, 16, 8
+IMP_X, IMP_Y, IMP_Z = 256, 128, 64
+BC_ALL = BC_TX | BC_TY | BC_TZ | BC_RX | BC_RY | BC_RZ
+
+
+@dataclasses.dataclass
+class ShellProp:
+    """/PROP/SHELL (IGTYP=1) as the Starter resolves it (hm_read_prop01.F:156-245).
+
+    Ishell=1 (IHBE=1), Ismstr=2, NPT points with the COQINI Newton-Cotes-type
+    tables, hourglass coefficients Hm/Hf/Hr (default 0.01), shear factor 5/6.
+    """
+
+    thick: float
+    npt: int = 5
+    ipla: int = 1
+    ithk: int = 1
+    ishear: int = 0
+    hm: float = 0.01
+    hf: float = 0.01
+    hr: float = 0.01
+    dtmin: float = 0.0
+
+
+def shell_params(mat: Law2, prop: ShellProp, dtfac: float) -> dict:
+    """Resolved parameters of the Q4 path (PM words, uparam, GEO words)."""
+    p = mat.params()
+    young, nu, rho = mat.young, p["nu"], mat.rho
+    ssp = float(np.sqrt(young / max(rho, 1.0e-20)))
+    a11 = young / (1.0 - nu * nu)
+    return dict(rho=rho, young=young, nu=nu, shear=p["shear"], a11=a11, a12=a11 * nu, ssp=ssp,
+                ca=p["ca"], cb=p["cb"], cn=p["cn"], epmx=p["epmx"], ymax=p["sigmx"], cc=p["cc"],
+                epdr=p["epdr"], asrate=p["asrate"], m_exp=p["m_exp"], icc=p["icc"], vp=p["vp"],
+                israte=p["israte"], iform=0, npt=prop.npt, ipla=prop.ipla, ithk=prop.ithk, ishear=prop.ishear,
+                hm=prop.hm, hf=prop.hf, hr=prop.hr, fsh=5.0 / 6.0, dtfac=dtfac, dtmin=prop.dtmin)
+
+
+@dataclasses.dataclass
+class HexPart:
+    conn: np.ndarray     # (ne,8) int32 0-based
+    vol0: np.ndarray     # (ne,)
+    iads: np.ndarray     # (ne,8) int64 skyline rows
+    mat: Law2
+    prop: SolidProp
+    dtfac: float = 0.9
+
+    @property
+    def ne(self) -> int:
+        return int(self.conn.shape[0])
+
+    def params(self, dtmx: float) -> dict:
+        p = self.mat.params()
+        p.update(qa=self.prop.qa, qb=self.prop.qb, qh=self.prop.qh, vns1=self.prop.vns1, vns2=self.prop.vns2,
+                 dtfac=self.dtfac, dtmx=dtmx)
+        return p
+
+    def initial_state(self) -> dict:
+        ne = self.ne
+        return dict(sig=np.zeros((6, ne)), eint=np.zeros(ne), rho=np.full(ne, self.mat.rho), pla=np.zeros(ne),
+                    epsd=np.zeros(ne), offg=np.ones(ne), qvis=np.zeros(ne))
+
+
+@dataclasses.dataclass
+class Q4Part:
+    conn: np.ndarray     # (ne,4) int32 0-based (node 4 == node 3 for a triangle)
+    thke: np.ndarray     # (ne,) initial thickness
+    iadc: np.ndarray     # (ne,4) int64 skyline rows
+    mat: Law2
+    prop: ShellProp
+    dtfac: float = 0.9
+
+    @property
+    def ne(self) -> int:
+        return int(self.conn.shape[0])
+
+    def params(self) -> dict:
+        return shell_params(self.mat, self.prop, self.dtfac)
+
+    def initial_state(self) -> dict:
+        """GBUF/LBUF initial state: THK = THKE, OFF = 1, everything else zero."""
+        ne, npt = self.ne, self.prop.npt
+        return dict(offg=np.ones(ne), thk=self.thke.astype(np.float64).copy(), forc=np.zeros((5, ne)),
+                    mom=np.zeros((3, ne)), gstr=np.zeros((8, ne)), hourg=np.zeros((5, ne)),
+                    eint=np.zeros((2, ne)), epsd=np.zeros(ne), sig=np.zeros((npt, 5, ne)),
+                    pla=np.zeros((npt, ne)), epsd_ip=np.zeros((npt, ne)))
+
+
+@dataclasses.dataclass
+class Model:
+    """A hex and/or Q4 model in the Engine's post-Starter form (0-based indices).
+
+    Skyline rows of a node are ordered as FILLCNE orders them: solids before
+    shells, inside a family by ascending element id (domdec2.F:2046, 2166-2317).
+    """
+
+    x: np.ndarray
+    v: np.ndarray
+    vr: np.ndarray
+    ms: np.ndarray
+    inr: np.ndarray          # nodal rotational inertia IN (zero for hex-only nodes)
+    code: np.ndarray         # (nn,) int32 BC / imposed-velocity bits
+    vimp: np.ndarray         # (nn,3) imposed velocities
+    adsky: np.ndarray        # (nn+1,) int64
+    hexp: HexPart | None = None
+    q4: Q4Part | None = None
+    dtmx: float = EP20
+
+    @property
+    def nn(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def lsky(self) -> int:
+        return int(self.adsky[-1])
+
+    @property
+    def ne_hex(self) -> int:
+        return 0 if self.hexp is None else self.hexp.ne
+
+    @property
+    def ne_q4(self) -> int:
+        return 0 if self.q4 is None else self.q4.ne
+
+    @property
+    def rotational(self) -> bool:
+        return self.q4 is not None
+
+
+def mixed_skyline(nn: int, hex_conn=None, q4_conn=None):
+    """FILLCNE-ordered skyline over hex then Q4 contributions; returns (iads, iadc, adsky)."""
+    nodes, ranks, sizes = [], [], []
+    base = 0
+    for conn in (hex_conn, q4_conn):
+        if conn is None or conn.shape[0] == 0:
+            sizes.append(None)
+            continue
+        ne, nk = conn.shape
+        nodes.append(conn.reshape(-1).astype(np.int64))
+        ranks.append(np.repeat(np.arange(base, base + ne, dtype=np.int64), nk))
+        sizes.append((ne, nk))
+        base += ne
+    node = np.concatenate(nodes)
+    rank = np.concatenate(ranks)
+    order = np.lexsort((rank, node))
+    counts = np.bincount(node, minlength=nn).astype(np.int64)
+    adsky = np.zeros(nn + 1, dtype=np.int64)
+    np.cumsum(counts, out=adsky[1:])
+    rows = np.empty(node.shape[0], dtype=np.int64)
+    rows[order] = np.arange(node.shape[0], dtype=np.int64)
+    out, off = [], 0
+    for sz in sizes:
+        if sz is None:
+            out.append(None)
+            continue
+        ne, nk = sz
+        out.append(rows[off:off + ne * nk].reshape(ne, nk))
+        off += ne * nk
+    return out[0], out[1], adsky
+
+
+def q4_area(xyz: np.ndarray) -> np.ndarray:
+    """Area of Q4 elements (ne,4,3): half the norm of the diagonal cross product."""
+    d13 = xyz[:, 2] - xyz[:, 0]
+    d24 = xyz[:, 3] - xyz[:, 1]
+    return 0.5 * np.linalg.norm(np.cross(d13, d24), axis=1)
+
+
+def build_model(x, hex_conn=None, hex_mat: Law2 | None = None, hex_prop: SolidProp | None = None,
+                q4_conn=None, q4_mat: Law2 | None = None, q4_prop: ShellProp | None = None, thke=None,
+                v0=None, vr0=None, code=None, vimp=None, dtfac_hex: float = 0.9, dtfac_q4: float = 0.9,
+                dtmx: float = EP20) -> Model:
+    """Starter-equivalent builder: volumes, SMASS3 / CINMAS lumped masses and
+    inertias, FILLCNE skyline, BC codes."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nn = x.shape[0]
+    hex_conn = None if hex_conn is None else np.ascontiguousarray(hex_conn, dtype=np.int32)
+    q4_conn = None if q4_conn is None else np.ascontiguousarray(q4_conn, dtype=np.int32)
+    iads, iadc, adsky = mixed_skyline(nn, hex_conn, q4_conn)
+    ms = np.zeros(nn)
+    inr = np.zeros(nn)
+    hexp = q4p = None
+    # nodal masses are summed in skyline order (rows per node, FILLCNE order)
+    rows_m = np.zeros(int(adsky[-1]))
+    rows_i = np.zeros(int(adsky[-1]))
+    if hex_conn is not None and hex_conn.shape[0]:
+        vol0 = hex_volume(x[hex_conn])
+        rho = hex_mat.params()["rho0"]
+        mass = (1.0 * rho) * vol0 * (1.0 / 8.0)          # smass3.F: RHO*VOLU*ONE_OVER_8
+        rows_m[iads.reshape(-1)] = np.repeat(mass, 8)
+        hexp = HexPart(conn=hex_conn, vol0=vol0, iads=iads, mat=hex_mat, prop=hex_prop or SolidProp(),
+                       dtfac=dtfac_hex)
+    if q4_conn is not None and q4_conn.shape[0]:
+        ne = q4_conn.shape[0]
+        thke = np.full(ne, q4_prop.thick) if thke is None else np.asarray(thke, dtype=np.float64)
+        area = q4_area(x[q4_conn])
+        ems = q4_mat.rho * thke * area * 0.25              # cinmas.F:851
+        xi = ems * (area / 9.0 + thke * thke * (1.0 / 12.0))  # cinmas.F:1383 (FAC=9, no offset)
+        rows_m[iadc.reshape(-1)] = np.repeat(ems, 4)
+        rows_i[iadc.reshape(-1)] = np.repeat(xi, 4)
+        q4p = Q4Part(conn=q4_conn, thke=thke, iadc=iadc, mat=q4_mat, prop=q4_prop, dtfac=dtfac_q4)
+    counts = np.diff(adsky)
+    for j in range(int(counts.max()) if nn else 0):
+        has = counts > j
+        ms[has] = ms[has] + rows_m[adsky[:-1][has] + j]
+        inr[has] = inr[has] + rows_i[adsky[:-1][has] + j]
+    v = np.zeros((nn, 3)) if v0 is None else np.array(v0, dtype=np.float64).reshape(nn, 3)
+    vr = np.zeros((nn, 3)) if vr0 is None else np.array(vr0, dtype=np.float64).reshape(nn, 3)
+    code = np.zeros(nn, np.int32) if code is None else np.asarray(code, dtype=np.int32)
+    vimp = np.zeros((nn, 3)) if vimp is None else np.asarray(vimp, dtype=np.float64).reshape(nn, 3)
+    return Model(x=x, v=v, vr=vr, ms=ms, inr=inr, code=code, vimp=vimp, adsky=adsky, hexp=hexp, q4=q4p,
+                 dtmx=dtmx)
+
+
+def hex_model_to_model(m: HexModel) -> Model:
+    """A HexModel as a general Model (same skyline, masses and BCs)."""
+    nn = m.nn
+    code = np.asarray(m.bcs, np.int32).copy()
+    vimp = np.zeros((nn, 3))
+    bit = np.array([IMP_X, IMP_Y, IMP_Z], np.int32)
+    for n, dof, val in zip(m.imp_node, m.imp_dof, m.imp_val):
+        code[n] |= bit[dof]
+        vimp[n, dof] = val
+    hexp = HexPart(conn=m.conn, vol0=m.vol0, iads=m.iads, mat=m.mat, prop=m.prop, dtfac=m.dtfac)
+    return Model(x=m.x, v=m.v, vr=np.zeros((nn, 3)), ms=m.ms, inr=np.zeros(nn), code=code, vimp=vimp,
+                 adsky=m.adsky, hexp=hexp, q4=None, dtmx=m.dtmx)
+
+
+def q4_grid(nu: int, nv: int, origin, du, dv):
+    """Structured nu x nv quad grid spanned by vectors du, dv (per cell); node (i,j) -> i + (nu+1)*j."""
+    i, j = np.meshgrid(np.arange(nu + 1), np.arange(nv + 1), indexing="xy")
+    x = (np.asarray(origin, float)[None, :] + i.reshape(-1, 1) * np.asarray(du, float)[None, :]
+         + j.reshape(-1, 1) * np.asarray(dv, float)[None, :])
+    ei, ej = np.meshgrid(np.arange(nu), np.arange(nv), indexing="xy")
+    ei, ej = ei.reshape(-1), ej.reshape(-1)
+    nid = lambda a, b: a + (nu + 1) * b  # noqa: E731
+    conn = np.stack([nid(ei, ej), nid(ei + 1, ej), nid(ei + 1, ej + 1), nid(ei, ej + 1)], axis=1).astype(np.int32)
+    return x, conn
+
+
+ALUMINIUM = Law2(rho=2.7e-9, young=70000.0, nu=0.33, a=200.0, b=150.0, n=0.3)
+
+
+def square_tube(width: float = 60.0, length: float = 300.0, h: float = 1.0, thick: float = 1.5,
+                v_top: float = -1.5e4, mat: Law2 = ALUMINIUM, prop: ShellProp | None = None,
+                dtfac: float = 0.9) -> Model:
+    """Thin-walled square tube along z: base (z=0) fully fixed, top (z=length)
+    given the constant axial velocity v_top (mm/s), Q4 shells of edge h."""
+    nside = int(round(width / h))
+    nper = 4 * nside
+    nlen = int(round(length / h))
+    # perimeter points, counter-clockwise seen from +z, starting at (-w/2,-w/2)
+    t = np.arange(nper) * h
+    px = np.empty(nper)
+    py = np.empty(nper)
+    w2 = 0.5 * width
+    for k in range(nper):
+        s = t[k]
+        side, r = divmod(s, width)
+        side = int(side)
+        if side == 0:
+            px[k], py[k] = -w2 + r, -w2
+        elif side == 1:
+            px[k], py[k] = w2, -w2 + r
+        elif side == 2:
+            px[k], py[k] = w2 - r, w2
+        else:
+            px[k], py[k] = -w2, w2 - r
+    zz = np.arange(nlen + 1) * h
+    x = np.zeros(((nlen + 1) * nper, 3))
+    for j in range(nlen + 1):
+        x[j * nper:(j + 1) * nper, 0] = px
+        x[j * nper:(j + 1) * nper, 1] = py
+        x[j * nper:(j + 1) * nper, 2] = zz[j]
+    ei, ej = np.meshgrid(np.arange(nper), np.arange(nlen), indexing="xy")
+    ei, ej = ei.reshape(-1), ej.reshape(-1)
+    n0 = ej * nper + ei
+    n1 = ej * nper + (ei + 1) % nper
+    conn = np.stack([n0, n1, n1 + nper, n0 + nper], axis=1).astype(np.int32)
+    nn = x.shape[0]
+    code = np.zeros(nn, np.int32)
+    vimp = np.zeros((nn, 3))
+    code[x[:, 2] == 0.0] = BC_ALL
+    top = x[:, 2] == zz[-1]
+    code[top] |= IMP_Z
+    vimp[top, 2] = v_top
+    prop = prop or ShellProp(thick=thick)
+    return build_model(x, q4_conn=conn, q4_mat=mat, q4_prop=prop, code=code, vimp=vimp, dtfac_q4=dtfac)
+
+
+def flat_plate(nx: int, ny: int, h: float = 1.0, thick: float = 2.0, z: float = 0.0, origin=(0.0, 0.0),
+               mat: Law2 = STEEL, prop: ShellProp | None = None):
+    """(x, conn) of a flat nx x ny Q4 plate in the plane z."""
+    x, conn = q4_grid(nx, ny, (origin[0], origin[1], z), (h, 0.0, 0.0), (0.0, h, 0.0))
+    return x, conn
+
+
+def merge_nodes(xa: np.ndarray, xb: np.ndarray, conn_b: np.ndarray, tol: float = 1e-9):
+    """Append the nodes of mesh b to mesh a, sharing nodes that coincide (within tol).
+
+    Returns (x, conn_b renumbered into x)."""
+    key = {tuple(np.round(p / tol).astype(np.int64)): i for i, p in enumerate(xa)}
+    remap = np.array([key.get(tuple(np.round(p / tol).astype(np.int64)), -1) for p in xb], dtype=np.int64)
+    keep = remap < 0
+    newid = remap.copy()
+    newid[keep] = xa.shape[0] + np.arange(int(keep.sum()))
+    return np.vstack([xa, xb[keep]]), newid[conn_b].astype(np.int32)
+
+
+def tie_q4_on_hex(xh, ch, xp, cp, rng, v0: float = -5.0e3, noise: float = 300.0, thick: float = 2.0) -> Model:
+    """Hex block with a Q4 plate sharing the block's coincident nodes; base (z=0) fixed;
+    initial velocity v0 along z plus Gaussian noise."""
+    x, cq = merge_nodes(xh, xp, cp)
+    nn = x.shape[0]
+    v = np.zeros((nn, 3))
+    v[:, 2] = v0
+    v += noise * rng.standard_normal((nn, 3))
+    code = np.where(x[:, 2] == 0.0, BC_ALL, 0).astype(np.int32)
+    return build_model(x, hex_conn=ch, hex_mat=STEEL, q4_conn=cq, q4_mat=STEEL, q4_prop=ShellProp(thick=thick),
+                       v0=v, code=code)

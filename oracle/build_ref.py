@@ -50,7 +50,24 @@ LEAVES = [
     "engine/source/assembly/accele.F",
     "engine/source/assembly/velocity.F",
     "engine/source/assembly/displacement.F",
+    "engine/source/elements/shell/coque/ccoor3.F",
+    "engine/source/elements/shell/coque/cnvec3.F",
+    "engine/source/elements/shell/coque/cortdir3.F",
+    "engine/source/elements/shell/coque/cderi3.F",
+    "engine/source/elements/shell/coque/ccoef3.F",
+    "engine/source/elements/shell/coque/cdlen3.F",
+    "engine/source/elements/shell/coque/cdefo3.F",
+    "engine/source/elements/shell/coque/ccurv3.F",
+    "engine/source/elements/shell/coque/cstra3.F",
+    "engine/source/elements/shell/coque/chvis3.F",
+    "engine/source/elements/shell/coque/cdt3.F",
+    "engine/source/elements/shell/coque/cfint3.F",
+    "engine/source/elements/shell/coque/cupdt3.F",
+    "engine/source/elements/shell/coqini.F",
+    "engine/source/materials/mat/mat002/sigeps02c.F",
+    "engine/source/materials/mat/mat002/m2cplr.F",
 ]
+DRIVERS = ["ref_driver.F", "ref_shell.F"]
 
 _MOD = re.compile(r"^\s*module\s+([a-z0-9_]+)\s*(!.*)?$", re.I)
 
@@ -103,19 +120,22 @@ def main():
     ok = all(compile_one(os.path.join(REF, leaf)) for leaf in LEAVES)
     if not ok:
         return 1
-    drv = os.path.join(HERE, "ref_driver.F")
-    drv_obj = os.path.join(OBJ, "ref_driver.o")
-    for _ in range(40):
-        r = subprocess.run([FC, *FLAGS, *INC, "-c", drv, "-o", drv_obj], capture_output=True, text=True)
-        if r.returncode == 0:
-            break
-        m = re.search(r"module '([a-z0-9_]+)'", r.stderr, re.I)
-        dep = idx.get(m.group(1).lower()) if m else None
-        if not (dep and compile_one(dep)):
-            print(r.stderr[-3000:])
-            return 1
+    drv_objs = []
+    for name in DRIVERS:
+        drv = os.path.join(HERE, name)
+        drv_obj = os.path.join(OBJ, name.replace(".F", ".o"))
+        for _ in range(40):
+            r = subprocess.run([FC, *FLAGS, *INC, "-c", drv, "-o", drv_obj], capture_output=True, text=True)
+            if r.returncode == 0:
+                break
+            m = re.search(r"module '([a-z0-9_]+)'", r.stderr, re.I)
+            dep = idx.get(m.group(1).lower()) if m else None
+            if not (dep and compile_one(dep)):
+                print(r.stderr[-3000:])
+                return 1
+        drv_objs.append(drv_obj)
     lib = os.path.join(OUT, "libref.so")
-    r = subprocess.run([FC, "-shared", "-o", lib, drv_obj, *done.values()], capture_output=True, text=True)
+    r = subprocess.run([FC, "-shared", "-o", lib, *drv_objs, *done.values()], capture_output=True, text=True)
     if r.returncode != 0:
         print(r.stderr[-3000:])
         return 1

@@ -86,6 +86,75 @@ void or_energy(int64_t nn, int64_t ne, const double *eint, const double *vol0,
 /* Exposed leaves for unit checks against the compiled reference. */
 double or_const(int which);
 
+/* ---------------------------------------------------------------- Q4 shell */
+/* Law 2 as SIGEPS02C reads it, the PM words CCOEF3 reads and /PROP/SHELL
+ * (hm_read_mat02_jc.F90:236-290, hm_read_prop01.F:156-245). */
+typedef struct {
+  double rho;      /* PM(1) */
+  double young;    /* PM(20), mat_param%young */
+  double nu;       /* PM(21) */
+  double shear;    /* PM(22) */
+  double ssp;      /* PM(27) = SQRT(E/RHO) */
+  double ca, cb, cn, epmx, ymax, cc, epdr, m_exp; /* uparam(1..7,10) */
+  double asrate;   /* PM(9): 2*pi*fcut (strain-rate filter) */
+  int    icc, vp, israte, iform;
+  int    npt;      /* through-thickness points, 2..10 */
+  int    ipla;     /* Iplas: 0 radial return, 1 Newton (3 iterations), 2 normal projection */
+  int    ithk;     /* Ithick: 1 = thickness change enters the element */
+  int    ishear;   /* GEO(37) */
+  double hm, hf, hr; /* GEO(13..15) hourglass coefficients */
+  double fsh;      /* GEO(38) shear factor (5/6) */
+  double dtfac;    /* DTFAC1(3) */
+  double dtmin;    /* DTMIN1(3) */
+} or_q4_params;
+
+/* GBUF / LBUF state of ne Q4 elements, component-major like ELBUF (SIG(NEL,5) ...). */
+typedef struct {
+  int64_t ne;
+  double *offg, *thk;      /* GBUF%OFF, GBUF%THK */
+  const double *thke;      /* initial thickness (THKE) */
+  double *forc;            /* GBUF%FOR  [5][ne] */
+  double *mom;             /* GBUF%MOM  [3][ne] */
+  double *gstr;            /* GBUF%STRA [8][ne] */
+  double *hourg;           /* GBUF%HOURG [5][ne] */
+  double *eint;            /* GBUF%EINT [2][ne] (membrane, bending) */
+  double *epsd;            /* GBUF%EPSD [ne] */
+  double *sig;             /* LBUF%SIG per point [npt][5][ne] */
+  double *pla;             /* LBUF%PLA [npt][ne] */
+  double *epsd_ip;         /* LBUF%EPSD [npt][ne] */
+} or_q4_state;
+
+void or_q4_forces(int64_t e0, int64_t e1, const int32_t *conn /* [ne*4] */, const double *x, const double *v,
+                  const double *vr, or_q4_state *s, const int64_t *iadc /* [ne*4] */, double *fsky,
+                  const or_q4_params *p, double dt1, double *dt2t);
+double or_shell_const(int which);
+double or_shell_table(int which, int ipt, int npt);
+
+/* A mixed hex + Q4 model with 6-DOF nodes.  code[n] bits: BCS translations
+ * 4=x 2=y 1=z, BCS rotations 32=x 16=y 8=z, imposed velocity 256=x 128=y 64=z. */
+typedef struct {
+  int64_t nn;
+  double *x, *v, *a, *d, *vr, *ar, *stifn, *stifr;
+  const double *ms, *in;   /* in may be NULL (no rotational DOF) */
+  const int32_t *code;
+  const double *vimp;      /* [3*nn] imposed velocities */
+  const int64_t *adsky;
+  double *fsky;            /* FSKY(8,LSKY) */
+  double dtmx;
+  int64_t ne_hex;
+  const int32_t *hconn; const int64_t *hiads;
+  double *hsig, *heint, *hrho, *hpla, *hepsd, *hoffg, *hqvis;
+  const double *hvol0;
+  const or_hex_params *hp;
+  int64_t ne_q4;
+  const int32_t *qconn; const int64_t *qiadc;
+  or_q4_state q;
+  const or_q4_params *qp;
+} or_model_c;
+
+void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads);
+void or_energy_model(const or_model_c *m, double out[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,14 +13,16 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
-_SRC = os.path.join(_HERE, "or_oracle.c")
+_SRCS = [os.path.join(_HERE, "or_oracle.c"), os.path.join(_HERE, "or_shell.c")]
+_HDR = os.path.join(_HERE, "or_oracle.h")
 
 
 def build(force: bool = False) -> str:
     """Compile the oracle with contraction off (matching the reference release flags)."""
-    if force or not os.path.exists(_LIB) or os.path.getmtime(_LIB) < os.path.getmtime(_SRC):
+    stale = not os.path.exists(_LIB) or any(os.path.getmtime(_LIB) < os.path.getmtime(f) for f in _SRCS + [_HDR])
+    if force or stale:
         subprocess.check_call(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-                               "-fopenmp", "-Wall", _SRC, "-o", _LIB, "-lm"])
+                               "-fopenmp", "-Wall", *_SRCS, "-o", _LIB, "-lm"])
     return _LIB
 
 
@@ -46,6 +48,10 @@ def lib():
         _lib = C.CDLL(_LIB)
         _lib.or_const.restype = C.c_double
         _lib.or_const.argtypes = [C.c_int]
+        _lib.or_shell_const.restype = C.c_double
+        _lib.or_shell_const.argtypes = [C.c_int]
+        _lib.or_shell_table.restype = C.c_double
+        _lib.or_shell_table.argtypes = [C.c_int, C.c_int, C.c_int]
     return _lib
 
 
@@ -122,4 +128,125 @@ class HexOracle:
         out = np.zeros(2)
         lib().or_energy(C.c_int64(self.m.nn), C.c_int64(self.m.ne), _p(self.eint), _p(self.vol0), _p(self.ms),
                         _p(self.v), _p(out))
+        return out
+
+
+# ---------------------------------------------------------------------------
+# General (hex + Q4, 6-DOF) model
+# ---------------------------------------------------------------------------
+_D = C.POINTER(C.c_double)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+
+
+class Q4Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("rho", "young", "nu", "shear", "ssp", "ca", "cb", "cn", "epmx", "ymax",
+                                          "cc", "epdr", "m_exp", "asrate")] + \
+               [(n, C.c_int) for n in ("icc", "vp", "israte", "iform", "npt", "ipla", "ithk", "ishear")] + \
+               [(n, C.c_double) for n in ("hm", "hf", "hr", "fsh", "dtfac", "dtmin")]
+
+
+class Q4State(C.Structure):
+    _fields_ = [("ne", C.c_int64), ("offg", _D), ("thk", _D), ("thke", _D), ("forc", _D), ("mom", _D),
+                ("gstr", _D), ("hourg", _D), ("eint", _D), ("epsd", _D), ("sig", _D), ("pla", _D),
+                ("epsd_ip", _D)]
+
+
+class ModelC(C.Structure):
+    _fields_ = [("nn", C.c_int64)] + [(n, _D) for n in ("x", "v", "a", "d", "vr", "ar", "stifn", "stifr", "ms",
+                                                         "inr")] + \
+               [("code", _I32), ("vimp", _D), ("adsky", _I64), ("fsky", _D), ("dtmx", C.c_double),
+                ("ne_hex", C.c_int64), ("hconn", _I32), ("hiads", _I64)] + \
+               [(n, _D) for n in ("hsig", "heint", "hrho", "hpla", "hepsd", "hoffg", "hqvis", "hvol0")] + \
+               [("hp", C.POINTER(HexParams)), ("ne_q4", C.c_int64), ("qconn", _I32), ("qiadc", _I64),
+                ("q", Q4State), ("qp", C.POINTER(Q4Params))]
+
+
+def q4_params(part) -> Q4Params:
+    p = part.params()
+    qp = Q4Params()
+    for name, _ in Q4Params._fields_:
+        setattr(qp, name, p[name])
+    return qp
+
+
+def _pd(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+class ModelOracle:
+    """Runs a general openradioss_amd.model.Model on the CPU (hex and Q4 chains)."""
+
+    def __init__(self, model, nthreads: int = 1):
+        m = self.m = model
+        nn = m.nn
+        self.x = m.x.copy()
+        self.v = m.v.copy()
+        self.vr = m.vr.copy()
+        self.a = np.zeros((nn, 3))
+        self.ar = np.zeros((nn, 3))
+        self.d = np.zeros((nn, 3))
+        self.stifn = np.zeros(nn)
+        self.stifr = np.zeros(nn)
+        self.ms = np.ascontiguousarray(m.ms, np.float64)
+        self.inr = np.ascontiguousarray(m.inr, np.float64)
+        self.code = np.ascontiguousarray(m.code, np.int32)
+        self.vimp = np.ascontiguousarray(m.vimp, np.float64)
+        self.adsky = np.ascontiguousarray(m.adsky, np.int64)
+        self.fsky = np.zeros((m.lsky, 8))
+        self.nthreads = nthreads
+        c = self.c = ModelC()
+        c.nn = nn
+        for n in ("x", "v", "a", "d", "vr", "ar", "stifn", "stifr", "ms", "vimp", "fsky"):
+            setattr(c, n, _pd(getattr(self, n)))
+        c.inr = _pd(self.inr) if m.rotational else None
+        c.code = self.code.ctypes.data_as(_I32)
+        c.adsky = self.adsky.ctypes.data_as(_I64)
+        c.dtmx = m.dtmx
+        c.ne_hex = m.ne_hex
+        if m.hexp is not None:
+            h = m.hexp
+            st = h.initial_state()
+            self.hsig = np.ascontiguousarray(st["sig"])
+            self.heint, self.hrho, self.hpla = st["eint"], st["rho"], st["pla"]
+            self.hepsd, self.hoffg, self.hqvis = st["epsd"], st["offg"], st["qvis"]
+            self.hvol0 = np.ascontiguousarray(h.vol0)
+            self.hconn = np.ascontiguousarray(h.conn, np.int32)
+            self.hiads = np.ascontiguousarray(h.iads, np.int64)
+            hp = HexParams()
+            p = h.params(m.dtmx)
+            for name, _ in HexParams._fields_:
+                setattr(hp, name, p[name])
+            self.hp = hp
+            c.hconn = self.hconn.ctypes.data_as(_I32)
+            c.hiads = self.hiads.ctypes.data_as(_I64)
+            for n in ("hsig", "heint", "hrho", "hpla", "hepsd", "hoffg", "hqvis", "hvol0"):
+                setattr(c, n, _pd(getattr(self, n)))
+            c.hp = C.pointer(self.hp)
+        c.ne_q4 = m.ne_q4
+        if m.q4 is not None:
+            q = m.q4
+            st = q.initial_state()
+            self.q = {k: np.ascontiguousarray(v) for k, v in st.items()}
+            self.qthke = np.ascontiguousarray(q.thke, np.float64)
+            self.qconn = np.ascontiguousarray(q.conn, np.int32)
+            self.qiadc = np.ascontiguousarray(q.iadc, np.int64)
+            self.qp = q4_params(q)
+            c.qconn = self.qconn.ctypes.data_as(_I32)
+            c.qiadc = self.qiadc.ctypes.data_as(_I64)
+            c.q.ne = q.ne
+            c.q.thke = _pd(self.qthke)
+            for k in ("offg", "thk", "forc", "mom", "gstr", "hourg", "eint", "epsd", "sig", "pla", "epsd_ip"):
+                setattr(c.q, k, _pd(self.q[k]))
+            c.qp = C.pointer(self.qp)
+        self.ts = initial_time_state()
+
+    def cycle(self, n: int = 1):
+        L = lib()
+        for _ in range(n):
+            L.or_cycle_model(C.byref(self.c), C.byref(self.ts), C.c_int(self.nthreads))
+
+    def energy(self):
+        out = np.zeros(3)
+        lib().or_energy_model(C.byref(self.c), _pd(out))
         return out

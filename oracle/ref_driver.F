@@ -42,13 +42,13 @@ C
       TYPE(FANI_CELL_) :: FANI
       INTEGER I, K, NC
       INTEGER NGL(MVSIZ), MAT(MVSIZ), NGEO(MVSIZ), IPARTS(MVSIZ),
-     .   IPM(NPROPMI,1), IGEO(NPROPGI,1), IPARG1(200), IXS(11,1),
+     .   IPARG1(200), IXS(11,1),
      .   NELTST, ITYPTST, NC1(MVSIZ), NC2(MVSIZ), NC3(MVSIZ),
      .   NC4(MVSIZ), NC5(MVSIZ), NC6(MVSIZ), NC7(MVSIZ), NC8(MVSIZ)
       DOUBLE PRECISION XD(MVSIZ,8), YD(MVSIZ,8), ZD(MVSIZ,8),
      .   VOLDP(MVSIZ), VOL0DP(MVSIZ), SAV(MVSIZ,21)
       my_real
-     .   PM(NPROPM,1), GEO(NPROPG,1), OFF(MVSIZ), VOLN(MVSIZ),
+     .   OFF(MVSIZ), VOLN(MVSIZ),
      .   XS(MVSIZ,8), YS(MVSIZ,8), ZS(MVSIZ,8),
      .   VX(MVSIZ,8), VY(MVSIZ,8), VZ(MVSIZ,8),
      .   PX(MVSIZ,8), PY(MVSIZ,8), PZ(MVSIZ,8), PXH(MVSIZ,12),
@@ -67,10 +67,21 @@ C
      .   TEMPEL(MVSIZ), SIGB(MVSIZ,6), AL_IMP(MVSIZ),
      .   SIGNOR(MVSIZ,6), CONDE(MVSIZ), DTEL(MVSIZ), DMG(MVSIZ),
      .   DPDM(MVSIZ), FHEAT(MVSIZ), F(MVSIZ,24), EANI(MVSIZ),
-     .   PARTSAV(NPSAV,1), DFE(MVSIZ,3), PBAK(MVSIZ), SVIS(MVSIZ,6),
+     .   DFE(MVSIZ,3), PBAK(MVSIZ), SVIS(MVSIZ,6),
      .   AR(3,1), FR_WAVE(1), FR_WAV(MVSIZ), MXYZ(MVSIZ,24),
      .   THEM(MVSIZ,8), FTHESKY(1), CONDNSKY(1), EHOUR0
+      my_real, ALLOCATABLE :: PM(:,:), GEO(:,:), PARTSAV(:,:)
+      INTEGER, ALLOCATABLE :: IPM(:,:), IGEO(:,:)
 C
+C-----array sizes of the restart (rdcomm.F:508-526, 762-764): any value above the
+C     largest index the called routines use
+      NPROPM = 500
+      NPROPG = 1000
+      NPROPMI = 400
+      NPROPGI = 1000
+      NPSAV = 29
+      ALLOCATE(PM(NPROPM,1), GEO(NPROPG,1), PARTSAV(NPSAV,1))
+      ALLOCATE(IPM(NPROPMI,1), IGEO(NPROPGI,1))
 C-----switches of the restated branch (common blocks)
       DT1 = DTIN1
       TT = ONE
@@ -292,6 +303,7 @@ C-----MMAIN law-2 glue (mmain.F90:668-745)
      .   THEM, FTHESKY, CONDNSKY, CONDE, NEL, 0, 0, 0, 0, 0)
       MSTOP_OUT = MSTOP
       DEALLOCATE(MATP%UPARAM, MATP%IPARAM)
+      DEALLOCATE(PM, GEO, PARTSAV, IPM, IGEO)
       RETURN
       END
 C

@@ -29,6 +29,8 @@ def lib():
         _lib.ref_hex_forces.restype = None
         _lib.ref_accele.restype = None
         _lib.ref_veldep.restype = None
+        for f in ("ref_q4_forces", "ref_accele6", "ref_veldep6", "ref_shell_consts"):
+            getattr(_lib, f).restype = None
     return _lib
 
 
@@ -126,5 +128,141 @@ class RefHex:
                 a[n_, dof] = (val - self.v[n_, dof]) / self.dt12
             L.ref_veldep(C.c_int(m.nn), _p(a), _p(self.v), _p(self.x), _p(self.d), C.c_double(self.dt2),
                          C.c_double(self.dt12))
+            self.tt = self.tt + self.dt2
+            self.ncycle += 1
+
+
+def q4_param_arrays(p: dict):
+    prm = np.zeros(24)
+    vals = [p["rho"], p["young"], p["nu"], p["shear"], p["a11"], p["a12"], p["ssp"], p["ca"], p["cb"], p["cn"],
+            p["epmx"], p["ymax"], p["cc"], p["epdr"], p["asrate"], p["m_exp"], p["hm"], p["hf"], p["hr"], p["fsh"],
+            p["dtfac"], p["dtmin"]]
+    prm[:len(vals)] = vals
+    iprm = np.array([p["icc"], p["vp"], p["israte"], p["iform"], p["npt"], p["ipla"], p["ithk"], p["ishear"]],
+                    dtype=np.int32)
+    return prm, iprm
+
+
+def shell_consts():
+    out = np.zeros(4)
+    lib().ref_shell_consts(_p(out))
+    return out
+
+
+class RefModel:
+    """Reference-routine run of a general Model (hex groups through SFORC3 leaves,
+    Q4 groups through the CFORC3 chain, nodal update through ACCELE / VELOCITY / DEPLA
+    with rotational DOF).  Same arrays as oracle.ModelOracle."""
+
+    GROUP = 128
+
+    def __init__(self, model):
+        m = self.m = model
+        nn = m.nn
+        self.x, self.v, self.vr = m.x.copy(), m.v.copy(), m.vr.copy()
+        self.d = np.zeros((nn, 3))
+        self.fsky = np.zeros((m.lsky, 8))
+        if m.hexp is not None:
+            st = m.hexp.initial_state()
+            self.hsig = np.ascontiguousarray(st["sig"])
+            self.heint, self.hrho, self.hpla = st["eint"], st["rho"], st["pla"]
+            self.hepsd, self.hoffg, self.hqvis = st["epsd"], st["offg"], st["qvis"]
+            self.hvol0 = m.hexp.vol0.copy()
+            self.hconn1 = np.ascontiguousarray(m.hexp.conn + 1, dtype=np.int32)
+            self.hiads1 = np.ascontiguousarray(m.hexp.iads + 1, dtype=np.int32)
+            self.hprm, self.hiprm = param_arrays(m.hexp.params(m.dtmx))
+        if m.q4 is not None:
+            self.q = {k: np.ascontiguousarray(v) for k, v in m.q4.initial_state().items()}
+            self.qthke = m.q4.thke.astype(np.float64).copy()
+            self.qconn1 = np.ascontiguousarray(m.q4.conn + 1, dtype=np.int32)
+            self.qiadc1 = np.ascontiguousarray(m.q4.iadc + 1, dtype=np.int32)
+            self.qprm, self.qiprm = q4_param_arrays(m.q4.params())
+        self.tt = self.dt1 = self.dt2 = self.dt12 = 0.0
+        self.dt2old = 1.0e6 / (1.0 + 1.0 / 10.0)
+        self.ncycle = 0
+        self.mstop = 0
+
+    def _hex_forces(self, dt1, dt2t):
+        L = lib()
+        ne = self.m.ne_hex
+        for e0 in range(0, ne, self.GROUP):
+            e1 = min(ne, e0 + self.GROUP)
+            sig = np.ascontiguousarray(self.hsig[:, e0:e1])
+            arrs = (self.heint, self.hrho, self.hvol0, self.hpla, self.hepsd, self.hoffg, self.hqvis)
+            grp = [np.ascontiguousarray(a[e0:e1]) for a in arrs]
+            ms, eh = C.c_int(0), C.c_double(0.0)
+            L.ref_hex_forces(C.c_int(e1 - e0), C.c_int(self.m.nn), _p(self.x), _p(self.v),
+                             _p(np.ascontiguousarray(self.hconn1[e0:e1])), _p(sig), *[_p(a) for a in grp],
+                             C.c_int(self.m.lsky), _p(self.fsky), _p(np.ascontiguousarray(self.hiads1[e0:e1])),
+                             _p(self.hprm), _p(self.hiprm), C.c_double(dt1), C.byref(dt2t), C.byref(ms),
+                             C.byref(eh))
+            self.hsig[:, e0:e1] = sig
+            for dst, src in zip(arrs, grp):
+                dst[e0:e1] = src
+            self.mstop |= ms.value
+
+    def _q4_forces(self, dt1, dt2t):
+        L = lib()
+        ne = self.m.ne_q4
+        q = self.q
+        for e0 in range(0, ne, self.GROUP):
+            e1 = min(ne, e0 + self.GROUP)
+            g = {}
+            for k, a in q.items():
+                g[k] = np.ascontiguousarray(a[..., e0:e1])
+            # Fortran layouts: FOR(NEL,5) etc. are component-major = (5, nel) C arrays;
+            # SIGL(NEL,5,NPT) = (npt, 5, nel) C array
+            L.ref_q4_forces(C.c_int(e1 - e0), C.c_int(self.m.nn), _p(self.x), _p(self.v), _p(self.vr),
+                            _p(np.ascontiguousarray(self.qconn1[e0:e1])), _p(g["offg"]), _p(g["thk"]),
+                            _p(np.ascontiguousarray(self.qthke[e0:e1])), _p(g["forc"]), _p(g["mom"]),
+                            _p(g["gstr"]), _p(g["hourg"]), _p(g["eint"]), _p(g["epsd"]), _p(g["sig"]),
+                            _p(g["pla"]), _p(g["epsd_ip"]), C.c_int(self.m.lsky), _p(self.fsky),
+                            _p(np.ascontiguousarray(self.qiadc1[e0:e1])), _p(self.qprm), _p(self.qiprm),
+                            C.c_double(dt1), C.byref(dt2t))
+            for k in q:
+                q[k][..., e0:e1] = g[k]
+
+    def cycle(self, n: int = 1):
+        m = self.m
+        L = lib()
+        for _ in range(n):
+            self.dt1 = self.dt2
+            self.dt2 = 1.0e6
+            dt2t = C.c_double(1.0e6)
+            if m.hexp is not None:
+                self._hex_forces(self.dt1, dt2t)
+            if m.q4 is not None:
+                self._q4_forces(self.dt1, dt2t)
+            # ASSPAR4 (asspar4.F:164-180): rows of each node in skyline order from zero
+            a = np.zeros((m.nn, 3))
+            ar = np.zeros((m.nn, 3))
+            counts = np.diff(m.adsky)
+            for j in range(int(counts.max())):
+                has = counts > j
+                rows = self.fsky[m.adsky[:-1][has] + j]
+                a[has] = a[has] + rows[:, 0:3]
+                ar[has] = ar[has] + rows[:, 3:6]
+            if dt2t.value < self.dt2:
+                self.dt2 = dt2t.value
+            self.dt2 = min(self.dt2, float(np.float32(1.1)) * self.dt2old, m.dtmx)
+            self.dt2old = self.dt2
+            self.dt12 = 0.5 * (self.dt1 + self.dt2)
+            ms = np.ascontiguousarray(m.ms)
+            inr = np.ascontiguousarray(m.inr)
+            L.ref_accele6(C.c_int(m.nn), _p(a), _p(ar), _p(self.v), _p(ms), _p(inr))
+            code = m.code
+            for bit, c in ((4, 0), (2, 1), (1, 2)):
+                sel = (code & bit) != 0
+                a[sel, c] = 0.0
+                self.v[sel, c] = 0.0
+            for bit, c in ((32, 0), (16, 1), (8, 2)):
+                sel = (code & bit) != 0
+                ar[sel, c] = 0.0
+                self.vr[sel, c] = 0.0
+            for bit, c in ((256, 0), (128, 1), (64, 2)):
+                sel = (code & bit) != 0
+                a[sel, c] = (m.vimp[sel, c] - self.v[sel, c]) / self.dt12
+            L.ref_veldep6(C.c_int(m.nn), _p(a), _p(ar), _p(self.v), _p(self.vr), _p(self.x), _p(self.d),
+                          C.c_double(self.dt2), C.c_double(self.dt12))
             self.tt = self.tt + self.dt2
             self.ncycle += 1

@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Write golden vectors of the hex path from the reference's own compiled routines.
+"""Write golden vectors of the hex and Q4 paths from the reference's own compiled routines.
 
 Run in the build container (needs oracle/_ref/libref.so, built by
 oracle/build_ref.py from /root/reference).  Each fixture holds the inputs that
@@ -18,12 +18,16 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import refchain as R  # noqa: E402
 sys.path.insert(0, os.path.dirname(HERE))
-from golden_cases import CASES, build_case  # noqa: E402
+from golden_cases import CASES, Q4_CASES, build_case, build_q4_case  # noqa: E402
+
 
 def main():
     if not R.available():
         raise SystemExit("oracle/_ref/libref.so missing: run oracle/build_ref.py first")
+    only = set(sys.argv[1:])
     for name, c in CASES.items():
+        if only and name not in only:
+            continue
         m = build_case(c)
         r = R.RefHex(m)
         r.cycle(c["cycles"])
@@ -33,6 +37,21 @@ def main():
                             time=np.array([r.tt, r.dt1, r.dt2, r.dt12, r.dt2old]),
                             ncycle=np.array([r.ncycle]))
         print(name, m.ne, "elements", c["cycles"], "cycles, tt", r.tt)
+    for name, c in Q4_CASES.items():
+        if only and name not in only:
+            continue
+        m = build_q4_case(c)
+        r = R.RefModel(m)
+        r.cycle(c["cycles"])
+        assert r.mstop == 0
+        out = dict(x=r.x, v=r.v, vr=r.vr, d=r.d, time=np.array([r.tt, r.dt1, r.dt2, r.dt12, r.dt2old]),
+                   ncycle=np.array([r.ncycle]))
+        if m.q4 is not None:
+            out.update({"q_" + k: v for k, v in r.q.items()})
+        if m.hexp is not None:
+            out.update(h_sig=r.hsig, h_eint=r.heint, h_pla=r.hpla, h_offg=r.hoffg)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+        print(name, m.ne_hex, "hex", m.ne_q4, "q4", c["cycles"], "cycles, tt", r.tt)
 
 
 if __name__ == "__main__":

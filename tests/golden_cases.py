@@ -22,3 +22,41 @@ def build_case(c):
     mat = M.Law2(rho=7.85e-9, young=210000.0, nu=0.3, a=300.0, b=800.0, n=1.0)
     v = 3.0e3 * rng.standard_normal(x.shape)
     return M.build_hex_model(x, conn, mat, v0=v)
+
+
+# ---------------------------------------------------------------- Q4 shells and mixed models
+Q4_CASES = {
+    # axial crush of a small square tube (configs[1] family): Newton return, thickness change
+    "tube_8x20_c150": dict(kind="tube", width=8.0, length=20.0, ipla=1, ithk=1, v_top=-1.5e4, cycles=150),
+    # strain-rate hardening with filtering and progressive rupture at 1% plastic strain
+    "tube_rupture_c150": dict(kind="tube_rupture", width=8.0, length=20.0, v_top=-4.0e4, cycles=150),
+    # distorted plate with random translational and rotational velocities, one triangle,
+    # radial-return plasticity (IPLA=0) and constant thickness (ITHK=0)
+    "plate_bend_c120": dict(kind="plate", seed=5, ipla=0, ithk=0, cycles=120),
+    # hex block with a shell plate tied on its top face (configs[2] family)
+    "mixed_c120": dict(kind="mixed", seed=7, cycles=120),
+}
+
+
+def build_q4_case(c):
+    if c["kind"] == "tube":
+        prop = M.ShellProp(thick=1.5, ipla=c["ipla"], ithk=c["ithk"])
+        return M.square_tube(width=c["width"], length=c["length"], prop=prop, v_top=c["v_top"])
+    if c["kind"] == "tube_rupture":
+        mat = M.Law2(rho=2.7e-9, young=70000.0, nu=0.33, a=200.0, b=150.0, n=0.3, c=0.02, eps0=1.0,
+                     fcut=1000.0, epsmax=0.01)
+        return M.square_tube(width=c["width"], length=c["length"], mat=mat, v_top=c["v_top"])
+    rng = np.random.default_rng(c["seed"])
+    if c["kind"] == "plate":
+        x, conn = M.flat_plate(6, 5, h=1.0, thick=1.2)
+        x = x + 0.15 * rng.uniform(-1, 1, x.shape)
+        conn[3, 3] = conn[3, 2]
+        nn = x.shape[0]
+        v = 2.0e3 * rng.standard_normal((nn, 3))
+        vr = 500.0 * rng.standard_normal((nn, 3))
+        prop = M.ShellProp(thick=1.2, ipla=c["ipla"], ithk=c["ithk"])
+        return M.build_model(x, q4_conn=conn, q4_mat=M.STEEL, q4_prop=prop, v0=v, vr0=vr)
+    # mixed: 3x3x3 hex block, 5x5 plate on top sharing the block's top nodes
+    xh, ch = M.hex_block(3, 3, 3, 1.0)
+    xp, cp = M.flat_plate(5, 5, h=1.0, z=3.0, origin=(-1.0, -1.0))
+    return M.tie_q4_on_hex(xh, ch, xp, cp, rng)
