@@ -1,14 +1,17 @@
 #!/usr/bin/env python3
-"""Throughput of the explicit hex path on one MI355X.
+"""Throughput of the explicit hex + Q4 path on one MI355X.
 
-A "step" is one explicit cycle (element forces + assembly + time integration)
-of the whole model, inputs resident in HBM.  Workload: the 160x160x160 hex
-block (4,096,000 SFORC3 elements, 1 mm edge) of the GPU-filling roofline
-model, law-2 steel, initial velocity 10 m/s along z plus N(0, 0.5 m/s) noise
-per component from a fixed seed.  value = element-updates/s of the plain FP64
-method; the split-precision form is reported beside it.
+A "step" is one explicit cycle (hex and shell element forces, assembly, time
+integration) of the whole model, inputs resident in HBM.  Workload: the
+GPU-filling roofline model -- a 160x160x160 hex block (4,096,000 SFORC3
+elements, 1 mm edge) with a 1000x1000 Q4 plate (1,000,000 CFORC3 shells,
+t = 2 mm, 5 points) tied to the block's top face at 25,921 shared nodes,
+law-2 steel, initial velocity 10 m/s along z plus N(0, 0.5 m/s) per component
+from a fixed seed.  value = element-updates/s of the plain FP64 method; the
+split-precision form (FP32 element arithmetic and state, FP64 nodes) is
+reported beside it with its displacement and energy error against FP64.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 160]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 160] [--plate 1000]
 """
 from __future__ import annotations
 
@@ -25,45 +28,39 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20261203
 
 
-def build_workload(n: int):
-    import numpy as np
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+
+def build_workload(n: int, plate: int):
     from openradioss_amd import model as M
-
-    x, conn = M.hex_block(n, n, n, 1.0)
-    rng = np.random.default_rng(SEED)
-    v = 500.0 * rng.standard_normal(x.shape)
-    v[:, 2] += 1.0e4
-    return M.build_hex_model(x, conn, M.STEEL, v0=v)
+    return M.block_with_plate(n=n, plate=plate, seed=SEED)
 
 
 def gpu_run(model, precision: str, steps: int, warmup: int):
     import numpy as np
 
     from openradioss_amd import engine as E
-    from openradioss_amd import traffic as T
 
-    g = E.GpuHexModel(model, precision=precision)
+    g = E.GpuModel(model, precision=precision)
     g.run(warmup)
     g.sync()
     t0 = time.perf_counter()
     g.run(steps, sync=False)
     g.sync()
     wall = time.perf_counter() - t0
-    # kernel durations with HIP events on the model stream (separate short run)
+    # kernel durations with HIP events on the model's stream (a separate short run)
     g.set_timing(True)
     g.kernel_times(reset=True)
-    nt = max(10, min(steps, 50))
-    g.run(nt)
-    elem_ms, node_ms, ncyc = g.kernel_times(reset=True)
+    g.run(max(10, min(steps, 50)))
+    hex_ms, q4_ms, node_ms, _ = g.kernel_times(reset=True)
     g.set_timing(False)
     ts = g.time_state()
-    x, v, d = g.nodes()
+    x, v, vr, d = g.nodes6()
+    e = g.energy()
+    foot = g.footprint()
     g.close()
-    ne, nn = model.ne, model.nn
-    eb = T.hex_element_kernel_bytes(ne, nn, precision)
-    nb = T.node_kernel_bytes(ne, nn, precision)
-    return dict(wall=wall, elem_ms=elem_ms, node_ms=node_ms, ts=ts, d=d, eb=eb, nb=nb,
+    return dict(wall=wall, hex_ms=hex_ms, q4_ms=q4_ms, node_ms=node_ms, ts=ts, d=d, energy=e, footprint=foot,
                 finite=bool(np.isfinite(d).all()))
 
 
@@ -72,7 +69,7 @@ def cpu_baseline(model, budget_s: float = 12.0):
     import oracle as O
 
     threads = max(1, min(16, os.cpu_count() or 1))
-    o = O.HexOracle(model, nthreads=threads)
+    o = O.ModelOracle(model, nthreads=threads)
     o.cycle(1)
     n = 0
     t0 = time.perf_counter()
@@ -82,9 +79,10 @@ def cpu_baseline(model, budget_s: float = 12.0):
         el = time.perf_counter() - t0
         if el >= budget_s or n >= 200:
             break
-    return dict(value=model.ne * n / el, unit="element-updates/s", cores=threads, kind="port",
-                sample=f"C oracle (FP64, OpenMP over groups of 128) on the same {model.ne}-element block, "
-                       f"{n} cycles after 1 warm-up cycle, {el:.1f} s")
+    ne = model.ne_hex + model.ne_q4
+    return dict(value=ne * n / el, unit="element-updates/s", cores=threads, kind="port",
+                sample=f"C oracle (FP64 restatement of SFORC3/CFORC3 + nodal update, OpenMP dynamic over groups of "
+                       f"128) on the same {ne}-element model, {n} cycles after 1 warm-up cycle, {el:.1f} s")
 
 
 def main():
@@ -92,27 +90,34 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=160, help="block edge in elements (160 = roofline model)")
+    ap.add_argument("--n", type=int, default=160, help="hex block edge in elements (160 = roofline model)")
+    ap.add_argument("--plate", type=int, default=1000, help="Q4 plate edge in elements (1000 = roofline model)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision form")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "this build runs on one GPU; runs on several GPUs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    import numpy as np
+
+    from openradioss_amd import traffic as T
+
     t0 = time.perf_counter()
-    model = build_workload(args.n)
-    print(f"[bench] model built: {model.ne} hex, {model.nn} nodes in {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
+    model = build_workload(args.n, args.plate)
+    ne = model.ne_hex + model.ne_q4
+    log(f"model built: {model.ne_hex} hex + {model.ne_q4} Q4, {model.nn} nodes in {time.perf_counter() - t0:.1f} s")
 
     r64 = gpu_run(model, "fp64", args.steps, args.warmup)
-    eu64 = model.ne * args.steps / r64["wall"]
-    print(f"[bench] fp64: {eu64:.3e} element-updates/s, elem {r64['elem_ms']:.3f} ms, node {r64['node_ms']:.3f} ms",
-          file=sys.stderr, flush=True)
-    elem_gbs = r64["eb"]["as_implemented"] / (r64["elem_ms"] * 1e-3) / 1e9
+    eu64 = ne * args.steps / r64["wall"]
+    log(f"fp64: {eu64:.3e} element-updates/s; hex {r64['hex_ms']:.3f} ms, q4 {r64['q4_ms']:.3f} ms, "
+        f"node {r64['node_ms']:.3f} ms per cycle")
+    tb = T.cycle_bytes(model, "fp64")
+    hex_gbs = tb["hex"]["as_implemented"] / (r64["hex_ms"] * 1e-3) / 1e9
+    cyc_ms = r64["hex_ms"] + r64["q4_ms"] + r64["node_ms"]
     out = {
-        "metric": "element-updates/s (explicit hex cycle, plain FP64 method)",
+        "metric": "element-updates/s (explicit hex + Q4 cycle, plain FP64 method)",
         "value": eu64,
         "unit": "element-updates/s",
         "n_gpus": 1,
@@ -124,30 +129,42 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"hex block {args.n}^3 (1 mm), law-2 steel, v0 = 10 m/s + N(0,0.5) seed {SEED}",
-                   "elements": model.ne, "nodes": model.nn, "cycles": args.steps},
-        "roofline": {"bound": "hbm", "achieved": elem_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": elem_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_hex_forces<double>", "kernel_ms": r64["elem_ms"],
-                     "bytes_per_launch": r64["eb"]["as_implemented"]},
-        "node_kernel": {"ms": r64["node_ms"],
-                        "gbs": r64["nb"]["as_implemented"] / (r64["node_ms"] * 1e-3) / 1e9},
+        "config": {"workload": f"hex block {args.n}^3 (1 mm) + Q4 plate {args.plate}^2 (1 mm, t=2 mm, 5 points) "
+                               f"tied on the block top, law-2 steel, v0 = 10 m/s + N(0,0.5) seed {SEED}",
+                   "hex_elements": model.ne_hex, "q4_elements": model.ne_q4, "nodes": model.nn},
+        "roofline": {"bound": "hbm", "achieved": hex_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": hex_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_hex_forces<double,double>", "kernel_ms": r64["hex_ms"],
+                     "bytes_per_launch": tb["hex"]["as_implemented"],
+                     "compulsory_bytes_per_launch": tb["hex"]["compulsory"]},
+        "kernels_fp64": {
+            k: {"ms": r64[f"{k}_ms"], "bytes": tb[k]["as_implemented"],
+                "gbs": tb[k]["as_implemented"] / (r64[f"{k}_ms"] * 1e-3) / 1e9} for k in ("hex", "q4", "node")},
+        "cycle_fp64": {"kernel_ms": cyc_ms, "bytes_as_implemented": tb["total"]["as_implemented"],
+                       "bytes_compulsory": tb["total"]["compulsory"],
+                       "gbs": tb["total"]["as_implemented"] / (cyc_ms * 1e-3) / 1e9,
+                       "bytes_per_element_update": tb["total"]["as_implemented"] / ne},
         "final_time": r64["ts"].tt,
+        "footprint_gb_fp64": r64["footprint"] / 1e9,
     }
     if not args.no_split:
         rs = gpu_run(model, "split", args.steps, args.warmup)
-        import numpy as np
-        eus = model.ne * args.steps / rs["wall"]
+        eus = ne * args.steps / rs["wall"]
         derr = float(np.abs(rs["d"] - r64["d"]).max() / np.abs(r64["d"]).max())
-        out["split_form"] = {"value": eus, "speedup_vs_fp64": eus / eu64, "ms_per_step": 1e3 * rs["wall"] / args.steps,
-                             "elem_kernel_ms": rs["elem_ms"], "node_kernel_ms": rs["node_ms"],
-                             "elem_kernel_gbs": rs["eb"]["as_implemented"] / (rs["elem_ms"] * 1e-3) / 1e9,
-                             "disp_rel_err_vs_fp64_gpu": derr, "dtype": "f32 element / f64 node"}
-        print(f"[bench] split: {eus:.3e} element-updates/s (x{eus / eu64:.2f}), disp err {derr:.2e}",
-              file=sys.stderr, flush=True)
+        eerr = float(abs(rs["energy"].sum() - r64["energy"].sum()) / abs(r64["energy"].sum()))
+        ts32 = T.cycle_bytes(model, "split")
+        cyc32 = rs["hex_ms"] + rs["q4_ms"] + rs["node_ms"]
+        out["split_form"] = {
+            "value": eus, "speedup_vs_fp64": eus / eu64, "ms_per_step": 1e3 * rs["wall"] / args.steps,
+            "kernels": {k: {"ms": rs[f"{k}_ms"], "gbs": ts32[k]["as_implemented"] / (rs[f"{k}_ms"] * 1e-3) / 1e9}
+                        for k in ("hex", "q4", "node")},
+            "cycle_gbs": ts32["total"]["as_implemented"] / (cyc32 * 1e-3) / 1e9,
+            "disp_rel_err_vs_fp64_gpu": derr, "energy_rel_err_vs_fp64_gpu": eerr,
+            "dtype": "f32 element / f64 node"}
+        log(f"split: {eus:.3e} element-updates/s (x{eus / eu64:.2f}), disp err {derr:.2e}, energy err {eerr:.2e}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(model)
-        print(f"[bench] cpu: {out['cpu_baseline']['value']:.3e} element-updates/s", file=sys.stderr, flush=True)
+        log(f"cpu: {out['cpu_baseline']['value']:.3e} element-updates/s")
     print(json.dumps(out))
 
 

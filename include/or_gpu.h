@@ -4,23 +4,31 @@
  * Plain pointers and sizes only.  Every entry point returns an int status
  * (OR_OK == 0); or_last_error() returns the message of the last failure of
  * the calling thread.  Host arrays use the reference's Fortran layouts:
- *   X(3,NUMNOD), V(3,NUMNOD), D(3,NUMNOD)   node-major triples (nodal_arrays.F90:109-177)
- *   SIG(NEL,6)                              component-major (elbufdef_mod.F90 g_bufel_%SIG)
- *   IXS(11,NUMELS) rows 2..9                1-based node ids (element_mod.F90:1174)
- *   FSKY(8,LSKY), IADS(8,*), ADSKY(NUMNOD+1) Parith/ON skyline (scumu3p.F, asspar4.F:164-180)
+ *   X(3,NUMNOD), V(3,NUMNOD), VR(3,NUMNOD), D(3,NUMNOD)   node-major triples
+ *                                       (common_source/modules/nodal_arrays.F90:109-177)
+ *   SIG(NEL,6), FOR(NEL,5), MOM(NEL,3), STRA(NEL,8), HOURG(NEL,5), EINT(NEL,2)
+ *                                       component-major (elbufdef_mod.F90:739-1016)
+ *   LBUF SIG(NEL,5) per through-thickness point, points outermost
+ *   IXS(11,NUMELS) rows 2..9, IXC(7,NUMELC) rows 2..5   1-based node ids (element_mod.F90:1174,1178)
+ *   FSKY(8,LSKY), IADS(8,*), IADC(4,*), ADSKY(NUMNOD+1)  Parith/ON skyline
+ *                                       (scumu3p.F, cupdt3.F:967, asspar4.F:164-180)
  * The resident-model API takes 0-based indices (documented per call).
  *
  * Reference interfaces replaced (file:line in the OpenRadioss tree):
- *   or_model_*            the resident GPU handle of shell_gpu_mod.F90:279-715 /
- *                         shell_gpu_driver.h:35-210 (create/allocate/upload/step/download/destroy),
- *                         extended to hex elements and to the nodal update.
- *   or_model_run          one RESOL cycle of the path per iteration: FORINT -> SFORC3
- *                         (resol.F:4225, forint.F:1159), ASSPAR4 (resol.F:4858), the DT2
- *                         pipeline (resol.F:6125-6129, 6352, 6494-6497), ACCELE (resol.F:6921),
+ *   or_model_*            the resident GPU handle of the shell offload module
+ *                         (engine/source/elements/shell/coque/shell_gpu_mod.F90:279-715,
+ *                         shell_gpu_driver.h:35-210: create / allocate / upload / step /
+ *                         download / destroy), extended to hexes and to the nodal update.
+ *   or_model_run          one RESOL cycle of the path per iteration: FORINTC -> CFORC3
+ *                         (resol.F:4138, forintc.F:452), FORINT -> SFORC3 (resol.F:4225,
+ *                         forint.F:1159), ASSPAR4 (resol.F:4858), the DT2 pipeline
+ *                         (resol.F:6125-6129, 6352, 6494-6497), ACCELE (resol.F:6921),
  *                         BCS10 (resol.F:7322), FIXVEL (resol.F:7610), VELOCITY (resol.F:8947),
  *                         DEPLA (resol.F:9042), time advance (resol.F:8599-8613).
  *   or_sforc3_group       SFORC3 for one element group (sforc3.F:131-166) on the Isolid=1,
  *                         law-2, Lagrangian branch; writes FSKY rows like SCUMU3P (scumu3p.F:37).
+ *   or_cforc3_group       CFORC3 for one element group (cforc3.F:85-113) on the Ishell=1,
+ *                         law-2 branch; writes FSKY rows like CUPDT3P (cupdt3.F:967).
  *   or_model_energy       internal / kinetic energy sums of ECRIT (ecrit.F:222-275, 344).
  */
 #ifndef OR_GPU_H
@@ -41,6 +49,18 @@ extern "C" {
 #define OR_PREC_FP64 0    /* plain method: everything in FP64 */
 #define OR_PREC_SPLIT 1   /* FP32 element arithmetic/state, FP64 nodes and reductions */
 
+/* Node code bits (bcs1.F LCOD convention per direction):
+ * BCS translations 4=x 2=y 1=z, BCS rotations 32=x 16=y 8=z, imposed velocity 256=x 128=y 64=z. */
+#define OR_BC_TX 4
+#define OR_BC_TY 2
+#define OR_BC_TZ 1
+#define OR_BC_RX 32
+#define OR_BC_RY 16
+#define OR_BC_RZ 8
+#define OR_IMP_X 256
+#define OR_IMP_Y 128
+#define OR_IMP_Z 64
+
 /* Law 2 (Johnson-Cook) + /PROP/SOLID Isolid=1 + run control, resolved as the
  * Starter stores them (hm_read_mat02_jc.F90, hm_read_prop14.F:355-369). */
 typedef struct {
@@ -51,6 +71,23 @@ typedef struct {
   double dtfac; /* DTFAC1(1) */
   double dtmx;  /* DTMX */
 } or_hex_params;
+
+/* Law 2 for shells as SIGEPS02C reads it, the PM words CCOEF3 reads, and
+ * /PROP/SHELL Ishell=1 IGTYP=1 (hm_read_mat02_jc.F90:236-290, hm_read_prop01.F:156-245). */
+typedef struct {
+  double rho, young, nu, shear, ssp;            /* PM(1), PM(20), PM(21), PM(22), PM(27) */
+  double ca, cb, cn, epmx, ymax, cc, epdr, m_exp; /* uparam(1..7,10) */
+  double asrate;                                /* strain-rate filter 2*pi*fcut (israte>0) */
+  int32_t icc, vp, israte, iform;
+  int32_t npt;                                  /* through-thickness points, 2..10 */
+  int32_t ipla;                                 /* Iplas 0, 1 (Newton), 2 */
+  int32_t ithk;                                 /* Ithick */
+  int32_t ishear;                               /* GEO(37) */
+  double hm, hf, hr;                            /* GEO(13..15) hourglass coefficients */
+  double fsh;                                   /* GEO(38) shear factor */
+  double dtfac;                                 /* DTFAC1(3) */
+  double dtmin;                                 /* DTMIN1(3) */
+} or_q4_params;
 
 /* Cycle scalars (resol.F: TT, DT1, DT2, DT12, DT2OLD, NCYCLE, MSTOP). */
 typedef struct {
@@ -67,35 +104,51 @@ const char *or_last_error(void);
 int or_device_count(int *count);
 
 /* ---- resident model (upload once, step N cycles, state stays in HBM) ---- */
-/* nn nodes, ne_hex 8-node hexes, precision OR_PREC_*. */
-int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int precision);
+/* nn nodes, ne_hex 8-node hexes, ne_q4 Q4 shells with npt through-thickness points,
+ * precision OR_PREC_*. */
+int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int64_t ne_q4, int32_t npt,
+                    int precision);
 int or_model_destroy(or_model *m);
 int or_model_set_hex_params(or_model *m, const or_hex_params *p);
-/* x,v: X(3,nn), V(3,nn); ms: (nn); bcs: (nn) BCS code (4=x,2=y,1=z), may be NULL;
- * vimp: V(3,nn) imposed velocities used where impcode bit (4=x,2=y,1=z) is set, may be NULL. */
-int or_model_upload_nodes(or_model *m, const double *x, const double *v, const double *ms,
-                          const int32_t *bcs, const int32_t *impcode, const double *vimp);
+int or_model_set_q4_params(or_model *m, const or_q4_params *p);
+/* x,v: X(3,nn), V(3,nn); vr: VR(3,nn) or NULL (zero); ms: (nn); in: (nn) nodal rotational
+ * inertia, required when the model has shells; code: (nn) node code bits or NULL;
+ * vimp: V(3,nn) imposed velocities, required when any imposed-velocity bit is set. */
+int or_model_upload_nodes(or_model *m, const double *x, const double *v, const double *vr, const double *ms,
+                          const double *in, const int32_t *code, const double *vimp);
 /* conn: (ne,8) 0-based node ids; state arrays: sig SIG(NEL,6) component-major over all hexes,
- * eint, rho, vol0, pla, epsd, offg, qvis (ne each). */
+ * eint, rho, vol0, pla, epsd, offg, qvis (ne each); NULL state arrays start at zero (offg at one). */
 int or_model_upload_hex(or_model *m, const int32_t *conn, const double *sig, const double *eint,
                         const double *rho, const double *vol0, const double *pla, const double *epsd,
                         const double *offg, const double *qvis);
-/* adsky: (nn+1) 0-based CSR pointers; iads: (ne,8) 0-based row of each element node. */
-int or_model_upload_skyline(or_model *m, const int64_t *adsky, const int64_t *iads);
+/* conn: (ne,4) 0-based node ids (node 4 == node 3 for a triangle); thke: initial thickness;
+ * GBUF: offg, thk, forc FOR(NEL,5), mom MOM(NEL,3), gstr STRA(NEL,8), hourg HOURG(NEL,5),
+ * eint EINT(NEL,2), epsd; LBUF per point: sig (npt,5,ne), pla (npt,ne), epsd_ip (npt,ne).
+ * NULL state arrays start at the Starter's values (OFF=1, THK=THKE, everything else zero). */
+int or_model_upload_q4(or_model *m, const int32_t *conn, const double *thke, const double *offg, const double *thk,
+                       const double *forc, const double *mom, const double *gstr, const double *hourg,
+                       const double *eint, const double *epsd, const double *sig, const double *pla,
+                       const double *epsd_ip);
+/* adsky: (nn+1) 0-based CSR pointers; iads: (ne_hex,8) and iadc: (ne_q4,4) 0-based global rows.
+ * The rows of a node must list its solid rows before its shell rows (FILLCNE order). */
+int or_model_upload_skyline(or_model *m, const int64_t *adsky, const int64_t *iads, const int64_t *iadc);
 int or_model_set_time(or_model *m, const or_time_state *ts);
 int or_model_get_time(or_model *m, or_time_state *ts);
 /* Run ncycles cycles asynchronously on the model's stream. */
 int or_model_run(or_model *m, int64_t ncycles);
 int or_model_sync(or_model *m);
-int or_model_download_nodes(or_model *m, double *x, double *v, double *d, double *stifn);
+/* Any output pointer may be NULL. */
+int or_model_download_nodes(or_model *m, double *x, double *v, double *vr, double *d);
 int or_model_download_hex(or_model *m, double *sig, double *eint, double *rho, double *pla,
                           double *epsd, double *offg, double *qvis);
-/* out[0] = sum EINT*VOL0 (internal), out[1] = 0.5*sum MS*|V|^2 (kinetic);
- * fixed-order deterministic reductions in FP64. */
-int or_model_energy(or_model *m, double out[2]);
-/* Average device time per launch (ms) of the element and node kernels over the
- * cycles run since the last reset, measured with HIP events on the model stream. */
-int or_model_kernel_times(or_model *m, int reset, double *elem_ms, double *node_ms, int64_t *cycles);
+int or_model_download_q4(or_model *m, double *offg, double *thk, double *forc, double *mom, double *gstr,
+                         double *hourg, double *eint, double *epsd, double *sig, double *pla, double *epsd_ip);
+/* out[0] = internal energy (hex EINT*VOL0 + shell EINT(1)+EINT(2)), out[1] = 0.5*sum MS*|V|^2,
+ * out[2] = 0.5*sum IN*|VR|^2; fixed-order deterministic reductions in FP64. */
+int or_model_energy(or_model *m, double out[3]);
+/* Average device time per launch (ms) of the hex, Q4 and node kernels over the cycles run
+ * since the last reset, measured with HIP events on the model stream. */
+int or_model_kernel_times(or_model *m, int reset, double *hex_ms, double *q4_ms, double *node_ms, int64_t *cycles);
 int or_model_set_timing(or_model *m, int enable);
 /* Bytes of device memory held by the model. */
 int or_model_footprint(const or_model *m, int64_t *bytes);
@@ -110,6 +163,16 @@ int or_sforc3_group(int device, const or_hex_params *p, int64_t numnod, int32_t 
                     double *rho, const double *vol, double *pla, double *epsd, double *off,
                     double *qvis, int64_t lsky, double *fsky, const int32_t *iads, double dt1,
                     double *dt2t, int32_t *mstop);
+
+/* ---- group-level drop-in for CFORC3 (one group of nel <= 128 Q4 shells) ----
+ * x, v, vr X/V/VR(3,numnod); ixc IXC(7,*) from column nft+1; GBUF arrays of the group;
+ * LBUF sig (npt,5,nel), pla (npt,nel), epsd_ip (npt,nel); fsky FSKY(8,lsky);
+ * iadc IADC(4,*) 1-based.  *dt2t is min-folded with the group's time steps like CDT3. */
+int or_cforc3_group(int device, const or_q4_params *p, int64_t numnod, int32_t nel, int64_t nft,
+                    const double *x, const double *v, const double *vr, const int32_t *ixc, double *offg,
+                    double *thk, const double *thke, double *forc, double *mom, double *gstr, double *hourg,
+                    double *eint, double *epsd, double *sig, double *pla, double *epsd_ip, int64_t lsky,
+                    double *fsky, const int32_t *iadc, double dt1, double *dt2t);
 
 #ifdef __cplusplus
 }

@@ -1,12 +1,14 @@
-// or_capi.hip -- kernels and C ABI of the MI355X explicit hex path (gfx950).
+// or_capi.hip -- kernels and C ABI of the MI355X explicit hex / Q4 path (gfx950).
 //
 // Per cycle, on one stream, with every scalar kept on the device:
-//   k_hex_forces : SFORC3 chain per element -> skyline rows + block min of DTX
-//                  folded into a device word with an integer atomicMin (min is exact,
-//                  so the result does not depend on block order)
-//   k_node_update: ASSPAR4 fixed-order segmented sum per node, DT2 pipeline,
-//                  ACCELE, BCS10, FIXVEL, VELOCITY, DEPLA; thread 0 advances the
-//                  cycle scalars (double-buffered by cycle parity)
+//   k_hex_forces : SFORC3 chain per 8-node hex -> 4-word skyline rows
+//   k_q4_forces  : CFORC3 chain per Q4 shell   -> 8-word skyline rows
+//                  (both fold their element time steps into one device word with an
+//                  integer atomicMin on the FP64 bit pattern: min is exact, so the
+//                  result does not depend on block order)
+//   k_node_update: ASSPAR4 fixed-order segmented sum per node (hex rows, then Q4 rows,
+//                  the FILLCNE order), DT2 pipeline, ACCELE, BCS10, FIXVEL, VELOCITY,
+//                  DEPLA; thread 0 advances the cycle scalars (double-buffered by parity)
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -18,6 +20,7 @@
 
 #include "../../include/or_gpu.h"
 #include "or_hex.h"
+#include "or_q4.h"
 
 using namespace orgpu;
 
@@ -39,43 +42,45 @@ int fail(int code, const std::string &msg) {
 
 constexpr int kBlock = 256;
 
-struct alignas(32) RowD { double fx, fy, fz, sti; };
-struct alignas(16) RowF { float fx, fy, fz, sti; };
+// node code bits (include/or_gpu.h)
+constexpr int32_t kTX = 4, kTY = 2, kTZ = 1, kRX = 32, kRY = 16, kRZ = 8, kIX = 256, kIY = 128, kIZ = 64;
+
+template <typename S> struct alignas(4 * sizeof(S)) Row4 { S fx, fy, fz, sti; };
+template <typename S> struct alignas(4 * sizeof(S)) Row8 { S fx, fy, fz, mx, my, mz, sti, stir; };
 
 template <typename S> struct HexPtrs {
   const int32_t *conn;  // [8][ne]
-  const int32_t *iads;  // [8][ne]
+  const int32_t *iads;  // [8][ne] hex-row index
   S *sig;               // [6][ne]
   S *eint, *rho, *pla, *epsd, *offg, *qvis;
   const S *vol0;
   int64_t ne;
 };
 
+template <typename S> struct Q4Ptrs {
+  const int32_t *conn;  // [4][ne]
+  const int32_t *iadc;  // [4][ne] Q4-row index
+  S *offg, *thk;
+  const S *thke;
+  S *forc, *mom, *gstr, *hourg, *eint, *epsd; // [5][ne] [3][ne] [8][ne] [5][ne] [2][ne] [ne]
+  S *sig, *pla, *epsd_ip;                     // [npt][5][ne] [npt][ne] [npt][ne] (epsd_ip null when israte=0)
+  int64_t ne;
+};
+
 struct NodePtrs {
-  double *xv;          // [nn][6] x,y,z,vx,vy,vz
-  double *d;           // [nn][3]
-  const double *ms;    // [nn]
-  const int32_t *code; // [nn] bits 0-2 BCS (z,y,x), bits 3-5 imposed velocity (z,y,x)
-  const double *vimp;  // [nn][3] or null
-  double *stifn;       // [nn] or null
-  const int32_t *adsky; // [nn+1]
+  double *xv;            // [nn][6] x,y,z,vx,vy,vz
+  double *vr;            // [nn][4] rotational velocity (padded) or null
+  double *d;             // [nn][3]
+  const double *ms;      // [nn]
+  const double *in;      // [nn] or null
+  const int32_t *code;   // [nn]
+  const double *vimp;    // [nn][3]
+  const int32_t *adsky_h, *adsky_q; // [nn+1] each (null when the family is absent)
   int64_t nn;
 };
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
-
-template <typename T> __device__ __forceinline__ T to_r(double v) { return T(v); }
-
-template <typename Row, typename R> __device__ __forceinline__ void store_row(Row *p, R fx, R fy, R fz, R sti);
-template <> __device__ __forceinline__ void store_row<RowD, double>(RowD *p, double fx, double fy, double fz, double sti) {
-  double2 *q = reinterpret_cast<double2 *>(p);
-  q[0] = make_double2(fx, fy);
-  q[1] = make_double2(fz, sti);
-}
-template <> __device__ __forceinline__ void store_row<RowF, float>(RowF *p, float fx, float fy, float fz, float sti) {
-  *reinterpret_cast<float4 *>(p) = make_float4(fx, fy, fz, sti);
-}
 
 __device__ __forceinline__ double block_min(double v) {
   __shared__ double red[kBlock / 64];
@@ -93,12 +98,18 @@ __device__ __forceinline__ double block_min(double v) {
   return v;
 }
 
-// ---------------------------------------------------------------- element kernel
-template <typename R, typename S, typename Row>
+__device__ __forceinline__ void load_node(const double *xv, int64_t n, double &x, double &y, double &z, double &vx,
+                                          double &vy, double &vz) {
+  const double2 *p = reinterpret_cast<const double2 *>(xv + 6 * n);
+  double2 a = p[0], b = p[1], q = p[2];
+  x = a.x; y = a.y; z = b.x; vx = b.y; vy = q.x; vz = q.y;
+}
+
+// ---------------------------------------------------------------- hex element kernel
+template <typename R, typename S>
 __global__ __launch_bounds__(kBlock) void k_hex_forces(HexPtrs<S> e, const double *__restrict__ xv,
-                                                       Row *__restrict__ sky, HexConsts c, const DevTime *ts,
-                                                       int parity, unsigned long long *dtmin, int *flags,
-                                                       S *__restrict__ ehou) {
+                                                       Row4<S> *__restrict__ sky, HexConsts c, const DevTime *ts,
+                                                       int parity, unsigned long long *dtmin, int *flags) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double dtx = kEP06;
   int mstop = 0;
@@ -108,11 +119,10 @@ __global__ __launch_bounds__(kBlock) void k_hex_forces(HexPtrs<S> e, const doubl
     R vx[8], vy[8], vz[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int64_t n = e.conn[(int64_t)k * e.ne + i];
-      const double2 *p = reinterpret_cast<const double2 *>(xv + 6 * n);
-      double2 a = p[0], b = p[1], q = p[2];
-      xd[k] = a.x; yd[k] = a.y; zd[k] = b.x;
-      vx[k] = R(b.y); vy[k] = R(q.x); vz[k] = R(q.y);
+      double a, b, cc, u, v, w;
+      load_node(xv, e.conn[(int64_t)k * e.ne + i], a, b, cc, u, v, w);
+      xd[k] = a; yd[k] = b; zd[k] = cc;
+      vx[k] = R(u); vy[k] = R(v); vz[k] = R(w);
     }
     HexState<S> st;
 #pragma unroll
@@ -125,34 +135,106 @@ __global__ __launch_bounds__(kBlock) void k_hex_forces(HexPtrs<S> e, const doubl
     for (int k = 0; k < 6; ++k) e.sig[(int64_t)k * e.ne + i] = st.sig[k];
     e.eint[i] = st.eint; e.rho[i] = st.rho; e.pla[i] = st.pla;
     e.epsd[i] = st.epsd; e.offg[i] = st.offg; e.qvis[i] = st.qvis;
-    if (ehou) ehou[i] = S(R(ehou[i]) + o.ehou);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int64_t K = e.iads[(int64_t)k * e.ne + i];
-      store_row<Row, R>(sky + K, o.f1[k], o.f2[k], o.f3[k], o.sti4);
+      Row4<S> r;
+      r.fx = S(o.f1[k]); r.fy = S(o.f2[k]); r.fz = S(o.f3[k]); r.sti = S(o.sti4);
+      sky[e.iads[(int64_t)k * e.ne + i]] = r;
     }
     dtx = o.dtx;
     mstop = o.mstop;
   }
-  if (__any(mstop) && mstop) atomicOr(flags, 1);
+  if (mstop) atomicOr(flags, 1);
   const double m = block_min(dtx);
   if (threadIdx.x == 0) atomicMin(dtmin + parity, dbits(m));
 }
 
-// ---------------------------------------------------------------- node kernel
-__device__ __forceinline__ void accumulate_row(const RowD *r, double &ax, double &ay, double &az, double &sn) {
-  const double2 *q = reinterpret_cast<const double2 *>(r);
-  double2 a = q[0], b = q[1];
-  ax = ax + a.x; ay = ay + a.y; az = az + b.x; sn = sn + b.y;
-}
-__device__ __forceinline__ void accumulate_row(const RowF *r, double &ax, double &ay, double &az, double &sn) {
-  float4 a = *reinterpret_cast<const float4 *>(r);
-  ax = ax + double(a.x); ay = ay + double(a.y); az = az + double(a.z); sn = sn + double(a.w);
+// ---------------------------------------------------------------- Q4 element kernel
+template <typename R, typename S> struct Q4IpIO {
+  const Q4Ptrs<S> &q;
+  int64_t i;
+  __device__ __forceinline__ void load(int ip, R (&s)[5], R &pla, R &epsd_ip) const {
+    const S *sg = q.sig + (int64_t)ip * 5 * q.ne + i;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s[k] = R(sg[(int64_t)k * q.ne]);
+    pla = R(q.pla[(int64_t)ip * q.ne + i]);
+    epsd_ip = q.epsd_ip ? R(q.epsd_ip[(int64_t)ip * q.ne + i]) : R(0);
+  }
+  __device__ __forceinline__ void store(int ip, const R (&s)[5], R pla, R epsd_ip) const {
+    S *sg = q.sig + (int64_t)ip * 5 * q.ne + i;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) sg[(int64_t)k * q.ne] = S(s[k]);
+    q.pla[(int64_t)ip * q.ne + i] = S(pla);
+    if (q.epsd_ip) q.epsd_ip[(int64_t)ip * q.ne + i] = S(epsd_ip);
+  }
+};
+
+template <typename R, typename S>
+__global__ __launch_bounds__(kBlock) void k_q4_forces(Q4Ptrs<S> q, const double *__restrict__ xv,
+                                                      const double *__restrict__ vr, Row8<S> *__restrict__ sky,
+                                                      Q4Consts c, const DevTime *ts, int parity,
+                                                      unsigned long long *dtmin) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double dt = kEP06;
+  if (i < q.ne) {
+    const R DT1 = R(ts[parity].dt2);
+    double xg[4], yg[4], zg[4];
+    R vl[4][3], vrl[4][3];
+    int32_t nc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      nc[k] = q.conn[(int64_t)k * q.ne + i];
+      double a, b, cc, u, v, w;
+      load_node(xv, nc[k], a, b, cc, u, v, w);
+      xg[k] = a; yg[k] = b; zg[k] = cc;
+      vl[k][0] = R(u); vl[k][1] = R(v); vl[k][2] = R(w);
+      const double2 *p = reinterpret_cast<const double2 *>(vr + 4 * (int64_t)nc[k]);
+      double2 r0 = p[0], r1 = p[1];
+      vrl[k][0] = R(r0.x); vrl[k][1] = R(r0.y); vrl[k][2] = R(r1.x);
+    }
+    Q4Elem<R> st;
+    st.offg = R(q.offg[i]); st.thk = R(q.thk[i]); st.thke = R(q.thke[i]); st.epsd = R(q.epsd[i]);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) st.forc[k] = R(q.forc[(int64_t)k * q.ne + i]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) st.mom[k] = R(q.mom[(int64_t)k * q.ne + i]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) st.gstr[k] = R(q.gstr[(int64_t)k * q.ne + i]);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) st.hourg[k] = R(q.hourg[(int64_t)k * q.ne + i]);
+    st.eint[0] = R(q.eint[i]); st.eint[1] = R(q.eint[q.ne + i]);
+    Q4IpIO<R, S> io{q, i};
+    Q4Out<R> o;
+    q4_element<R>(xg, yg, zg, vl, vrl, nc[2] == nc[3], st, c, DT1, io, o);
+    q.offg[i] = S(st.offg); q.thk[i] = S(st.thk); q.epsd[i] = S(st.epsd);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q.forc[(int64_t)k * q.ne + i] = S(st.forc[k]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) q.mom[(int64_t)k * q.ne + i] = S(st.mom[k]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) q.gstr[(int64_t)k * q.ne + i] = S(st.gstr[k]);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q.hourg[(int64_t)k * q.ne + i] = S(st.hourg[k]);
+    q.eint[i] = S(st.eint[0]); q.eint[q.ne + i] = S(st.eint[1]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      Row8<S> r;
+      r.fx = S(-o.f[0][k]); r.fy = S(-o.f[1][k]); r.fz = S(-o.f[2][k]);
+      r.mx = S(-o.m[0][k]); r.my = S(-o.m[1][k]); r.mz = S(-o.m[2][k]);
+      r.sti = S(o.sti); r.stir = S(0);
+      sky[q.iadc[(int64_t)k * q.ne + i]] = r;
+    }
+    dt = o.dt;
+  }
+  const double m = block_min(dt);
+  if (threadIdx.x == 0) atomicMin(dtmin + parity, dbits(m));
 }
 
-template <typename Row>
-__global__ __launch_bounds__(kBlock) void k_node_update(NodePtrs nd, const Row *__restrict__ sky, DevTime *ts,
-                                                        int parity, unsigned long long *dtmin, const int *flags,
+// ---------------------------------------------------------------- node kernel
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_node_update(NodePtrs nd, const Row4<S> *__restrict__ skh,
+                                                        const Row8<S> *__restrict__ skq, DevTime *ts, int parity,
+                                                        unsigned long long *dtmin, const int *flags,
                                                         double dt_growth, double dtmx) {
   const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   // DT2 pipeline (resol.F:2721-2722, 2826, 6125-6129, 6352, 6494-6497)
@@ -181,12 +263,24 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodePtrs nd, const Row *
     dtmin[parity ^ 1] = dbits(kEP06);
   }
   if (n >= nd.nn) return;
-  // ASSPAR4: rows of this node in skyline order, starting from zero
-  double ax = 0.0, ay = 0.0, az = 0.0, sn = 0.0;
-  const int32_t k0 = nd.adsky[n], k1 = nd.adsky[n + 1];
-  for (int32_t k = k0; k < k1; ++k) accumulate_row(sky + k, ax, ay, az, sn);
-  if (nd.stifn) nd.stifn[n] = sn;
-  // ACCELE (accele.F:65-80)
+  // ASSPAR4: rows of this node in skyline order (solids, then shells), from zero
+  double ax = 0.0, ay = 0.0, az = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
+  if (skh) {
+    const int32_t k0 = nd.adsky_h[n], k1 = nd.adsky_h[n + 1];
+    for (int32_t k = k0; k < k1; ++k) {
+      const Row4<S> r = skh[k];
+      ax = ax + double(r.fx); ay = ay + double(r.fy); az = az + double(r.fz);
+    }
+  }
+  if (skq) {
+    const int32_t k0 = nd.adsky_q[n], k1 = nd.adsky_q[n + 1];
+    for (int32_t k = k0; k < k1; ++k) {
+      const Row8<S> r = skq[k];
+      ax = ax + double(r.fx); ay = ay + double(r.fy); az = az + double(r.fz);
+      rx = rx + double(r.mx); ry = ry + double(r.my); rz = rz + double(r.mz);
+    }
+  }
+  // ACCELE (accele.F:65-80, 97-109)
   const double msn = nd.ms[n];
   if (msn > 0.0) {
     const double rtmp = 1.0 / msn;
@@ -194,23 +288,45 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodePtrs nd, const Row *
   } else {
     ax = 0.0; ay = 0.0; az = 0.0;
   }
+  double wx = 0.0, wy = 0.0, wz = 0.0;
+  double2 *pr = nullptr;
+  if (nd.vr) {
+    const double inn = nd.in[n];
+    if (inn > 0.0) {
+      const double rtmp = 1.0 / inn;
+      rx = rx * rtmp; ry = ry * rtmp; rz = rz * rtmp;
+    } else {
+      rx = 0.0; ry = 0.0; rz = 0.0;
+    }
+    pr = reinterpret_cast<double2 *>(nd.vr + 4 * n);
+    double2 r0 = pr[0], r1 = pr[1];
+    wx = r0.x; wy = r0.y; wz = r1.x;
+  }
   double2 *p = reinterpret_cast<double2 *>(nd.xv + 6 * n);
   double2 pa = p[0], pb = p[1], pc = p[2];
   double x = pa.x, y = pa.y, z = pb.x, vx = pb.y, vy = pc.x, vz = pc.y;
   const int32_t code = nd.code[n];
   if (code) {
-    // BCS1V global frame (bcs1.F): LCOD bit 4 = x, 2 = y, 1 = z
-    if (code & 4) { ax = 0.0; vx = 0.0; }
-    if (code & 2) { ay = 0.0; vy = 0.0; }
-    if (code & 1) { az = 0.0; vz = 0.0; }
+    // BCS1V global frame (bcs1.F)
+    if (code & kTX) { ax = 0.0; vx = 0.0; }
+    if (code & kTY) { ay = 0.0; vy = 0.0; }
+    if (code & kTZ) { az = 0.0; vz = 0.0; }
+    if (code & kRX) { rx = 0.0; wx = 0.0; }
+    if (code & kRY) { ry = 0.0; wy = 0.0; }
+    if (code & kRZ) { rz = 0.0; wz = 0.0; }
     // FIXVEL global frame (fixvel.F:366-379): A = (VIMP - V)/DT12
-    if (code & (4 << 3)) ax = (nd.vimp[3 * n + 0] - vx) / DT12;
-    if (code & (2 << 3)) ay = (nd.vimp[3 * n + 1] - vy) / DT12;
-    if (code & (1 << 3)) az = (nd.vimp[3 * n + 2] - vz) / DT12;
+    if (code & kIX) ax = (nd.vimp[3 * n + 0] - vx) / DT12;
+    if (code & kIY) ay = (nd.vimp[3 * n + 1] - vy) / DT12;
+    if (code & kIZ) az = (nd.vimp[3 * n + 2] - vz) / DT12;
   }
-  // VELOCITY (velocity.F:58-66)
+  // VELOCITY (velocity.F:58-86)
   vx = vx + DT12 * ax; vy = vy + DT12 * ay; vz = vz + DT12 * az;
-  // DEPLA (displacement.F, IRESP=0 form in FP64)
+  if (pr) {
+    wx = wx + DT12 * rx; wy = wy + DT12 * ry; wz = wz + DT12 * rz;
+    pr[0] = make_double2(wx, wy);
+    pr[1] = make_double2(wz, 0.0);
+  }
+  // DEPLA (displacement.F:108-118, IRESP=0 form in FP64)
   double *dd = nd.d + 3 * n;
   double vdt;
   vdt = DT2 * vx; dd[0] = dd[0] + vdt; x = x + vdt;
@@ -236,41 +352,52 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_energy_partial(const S *eint, const S *vol0, int64_t ne, const double *xv,
-                                                           const double *ms, int64_t nn, int64_t per_block,
-                                                           double *partial) {
-  double ie = 0.0, ke = 0.0;
+__global__ __launch_bounds__(kBlock) void k_energy_partial(const S *heint, const S *hvol0, int64_t neh,
+                                                           const S *qeint, int64_t neq, const double *xv,
+                                                           const double *vr, const double *ms, const double *in,
+                                                           int64_t nn, int64_t per_block, double *partial) {
+  double ie = 0.0, ke = 0.0, kr = 0.0;
   const int64_t b0 = (int64_t)blockIdx.x * per_block;
   for (int64_t j = b0 + threadIdx.x; j < b0 + per_block; j += kBlock) {
-    if (j < ne) ie = ie + double(eint[j]) * double(vol0[j]);
+    if (j < neh) ie = ie + double(heint[j]) * double(hvol0[j]);
+    if (j < neq) ie = ie + (double(qeint[j]) + double(qeint[neq + j]));
     if (j < nn) {
       const double *p = xv + 6 * j;
       ke = ke + ms[j] * (p[3] * p[3] + p[4] * p[4] + p[5] * p[5]);
+      if (vr) {
+        const double *w = vr + 4 * j;
+        kr = kr + in[j] * (w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+      }
     }
   }
   ie = block_sum(ie);
   ke = block_sum(ke);
+  kr = block_sum(kr);
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x + 0] = ie;
-    partial[2 * blockIdx.x + 1] = ke;
+    partial[3 * blockIdx.x + 0] = ie;
+    partial[3 * blockIdx.x + 1] = ke;
+    partial[3 * blockIdx.x + 2] = kr;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_energy_final(const double *partial, int nb, double *out) {
-  double ie = 0.0, ke = 0.0;
+  double ie = 0.0, ke = 0.0, kr = 0.0;
   for (int j = threadIdx.x; j < nb; j += kBlock) {
-    ie = ie + partial[2 * j];
-    ke = ke + partial[2 * j + 1];
+    ie = ie + partial[3 * j];
+    ke = ke + partial[3 * j + 1];
+    kr = kr + partial[3 * j + 2];
   }
   ie = block_sum(ie);
   ke = block_sum(ke);
+  kr = block_sum(kr);
   if (threadIdx.x == 0) {
     out[0] = ie;
     out[1] = 0.5 * ke;
+    out[2] = 0.5 * kr;
   }
 }
 
-// ---------------------------------------------------------------- group drop-in kernel
+// ---------------------------------------------------------------- group drop-in kernels
 // SFORC3 on one group with the reference host layouts (X(3,*), V(3,*), IXS(11,*),
 // SIG(NEL,6), FSKY(8,*), IADS(8,*) 1-based).
 __global__ __launch_bounds__(kBlock) void k_sforc3_group(int nel, int64_t nft, const double *x, const double *v,
@@ -312,6 +439,80 @@ __global__ __launch_bounds__(kBlock) void k_sforc3_group(int nel, int64_t nft, c
   mstop_out[i] = o.mstop;
 }
 
+// Per-point state of the CFORC3 drop-in: LBUF arrays of one group, SIG(NEL,5) per point.
+struct GroupIpIO {
+  double *sig, *pla, *epsd_ip;
+  int nel, i;
+  __device__ __forceinline__ void load(int ip, double (&s)[5], double &p, double &e) const {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s[k] = sig[((int64_t)ip * 5 + k) * nel + i];
+    p = pla[(int64_t)ip * nel + i];
+    e = epsd_ip[(int64_t)ip * nel + i];
+  }
+  __device__ __forceinline__ void store(int ip, const double (&s)[5], double p, double e) const {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) sig[((int64_t)ip * 5 + k) * nel + i] = s[k];
+    pla[(int64_t)ip * nel + i] = p;
+    epsd_ip[(int64_t)ip * nel + i] = e;
+  }
+};
+
+// CFORC3 on one group with the reference host layouts (X,V,VR(3,*), IXC(7,*), GBUF arrays of
+// the group, LBUF SIG(NEL,5) / PLA(NEL) / EPSD(NEL) per point, FSKY(8,*), IADC(4,*) 1-based).
+__global__ __launch_bounds__(kBlock) void k_cforc3_group(int nel, int64_t nft, const double *x, const double *v,
+                                                         const double *vr, const int32_t *ixc, double *offg,
+                                                         double *thk, const double *thke, double *forc,
+                                                         double *mom, double *gstr, double *hourg, double *eint,
+                                                         double *epsd, double *sig, double *pla, double *epsd_ip,
+                                                         double *fsky, const int32_t *iadc, Q4Consts c, double dt1,
+                                                         double *dt_out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nel) return;
+  const int64_t col = nft + i;
+  double xg[4], yg[4], zg[4], vl[4][3], vrl[4][3];
+  int32_t nc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    nc[k] = ixc[7 * col + 1 + k];
+    const int64_t n = (int64_t)nc[k] - 1;
+    xg[k] = x[3 * n]; yg[k] = x[3 * n + 1]; zg[k] = x[3 * n + 2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { vl[k][d] = v[3 * n + d]; vrl[k][d] = vr[3 * n + d]; }
+  }
+  Q4Elem<double> st;
+  st.offg = offg[i]; st.thk = thk[i]; st.thke = thke[i]; st.epsd = epsd[i];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) st.forc[k] = forc[(int64_t)k * nel + i];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) st.mom[k] = mom[(int64_t)k * nel + i];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) st.gstr[k] = gstr[(int64_t)k * nel + i];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) st.hourg[k] = hourg[(int64_t)k * nel + i];
+  st.eint[0] = eint[i]; st.eint[1] = eint[nel + i];
+  GroupIpIO io{sig, pla, epsd_ip, nel, i};
+  Q4Out<double> o;
+  q4_element<double>(xg, yg, zg, vl, vrl, nc[2] == nc[3], st, c, dt1, io, o);
+  offg[i] = st.offg; thk[i] = st.thk; epsd[i] = st.epsd;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) forc[(int64_t)k * nel + i] = st.forc[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) mom[(int64_t)k * nel + i] = st.mom[k];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) gstr[(int64_t)k * nel + i] = st.gstr[k];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) hourg[(int64_t)k * nel + i] = st.hourg[k];
+  eint[i] = st.eint[0]; eint[nel + i] = st.eint[1];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double *f = fsky + 8 * ((int64_t)iadc[4 * col + k] - 1);
+    f[0] = -o.f[0][k]; f[1] = -o.f[1][k]; f[2] = -o.f[2][k];
+    f[3] = -o.m[0][k]; f[4] = -o.m[1][k]; f[5] = -o.m[2][k];
+    f[6] = o.sti; f[7] = 0.0;
+  }
+  dt_out[i] = o.dt;
+}
+
 // ---------------------------------------------------------------- host helpers
 double host_onep333() {
   volatile double zep3 = 3.0 / 10.0, zep03 = 3.0 / 100.0, threeem3 = 3.0 / 1000.0;
@@ -320,7 +521,7 @@ double host_onep333() {
   return onep33 + threeem3;
 }
 
-HexConsts make_consts(const or_hex_params &p) {
+HexConsts make_hex_consts(const or_hex_params &p) {
   HexConsts c;
   c.rho0 = p.rho0; c.bulk = p.bulk; c.shear = p.shear; c.ca = p.ca; c.cb = p.cb; c.cn = p.cn;
   c.epmx = p.epmx; c.sigmx = p.sigmx; c.cc = p.cc; c.epdr = p.epdr; c.fisokin = p.fisokin;
@@ -332,10 +533,77 @@ HexConsts make_consts(const or_hex_params &p) {
   return c;
 }
 
-int check_params(const or_hex_params &p) {
+int check_hex_params(const or_hex_params &p) {
   if (p.fisokin != 0.0) return fail(OR_ERR_ARG, "law 2 kinematic hardening (FISOKIN>0) is not on this path");
   if (p.iform != 0) return fail(OR_ERR_ARG, "law 2 Zerilli form (IFORM=1) is not on this path");
   if (!(p.rho0 > 0.0) || !(p.dtfac > 0.0)) return fail(OR_ERR_ARG, "rho0 and dtfac must be positive");
+  return OR_OK;
+}
+
+// Tables of COQINI (coqini.F:46-152), single-precision DATA literals.
+const float kZ0[11][11] = {
+    {0.f}, {-.5f, 0.5f}, {-.5f, 0.f, 0.5f}, {-.5f, -.1666667f, 0.1666667f, 0.5f}, {-.5f, -.25f, 0.f, 0.25f, 0.5f},
+    {-.5f, -.3f, -.1f, 0.1f, 0.3f, 0.5f}, {-.5f, -.3333333f, -.1666667f, 0.0f, 0.1666667f, 0.3333333f, 0.5f},
+    {-.5f, -.3571429f, -.2142857f, -.0714286f, 0.0714286f, 0.2142857f, 0.3571429f, 0.5f},
+    {-.5f, -.375f, -.25f, -.125f, 0.0f, 0.125f, 0.25f, 0.375f, 0.5f},
+    {-.5f, -.3888889f, -.2777778f, -.1666667f, -.0555555f, 0.0555555f, 0.1666667f, 0.2777778f, 0.3888889f, 0.5f},
+    {-.5f, -.4f, -.3f, -.2f, -.1f, 0.f, 0.1f, 0.2f, 0.3f, 0.4f, 0.5f}};
+const float kWF[11][11] = {
+    {1.f}, {0.5f, 0.5f}, {0.25f, 0.5f, 0.25f}, {0.1666667f, 0.3333333f, 0.3333333f, 0.1666667f},
+    {0.125f, 0.25f, 0.25f, 0.25f, 0.125f}, {0.1f, 0.2f, 0.2f, 0.2f, 0.2f, 0.1f},
+    {0.0833333f, 0.1666667f, 0.1666667f, 0.1666667f, 0.1666667f, 0.1666667f, 0.0833333f},
+    {0.0714286f, 0.1428571f, 0.1428571f, 0.1428571f, 0.1428571f, 0.1428571f, 0.1428571f, 0.0714286f},
+    {0.0625f, 0.125f, 0.125f, 0.125f, 0.125f, 0.125f, 0.125f, 0.125f, 0.0625f},
+    {0.0555556f, 0.1111111f, 0.1111111f, 0.1111111f, 0.1111111f, 0.1111111f, 0.1111111f, 0.1111111f, 0.1111111f,
+     0.0555556f},
+    {0.05f, 0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.05f}};
+const float kWM[11][11] = {
+    {0.f}, {-.0833333f, 0.0833333f}, {-.0833333f, 0.f, 0.0833333f}, {-.0648148f, -.0555556f, 0.0555556f, 0.0648148f},
+    {-.0520833f, -.0625f, 0.f, 0.0625f, 0.0520833f},
+    {-.0433333f, -.0600000f, -.0200000f, 0.0200000f, 0.0600000f, 0.0433333f},
+    {-.0370370f, -.0555556f, -.0277778f, 0.0000000f, 0.0277778f, 0.0555556f, 0.0370370f},
+    {-.0323129f, -.0510204f, -.0306122f, -.0102041f, 0.0102041f, 0.0306122f, 0.0510204f, 0.0323129f},
+    {-.0286458f, -.0468750f, -.0312500f, -.0156250f, 0.0000000f, 0.0156250f, 0.0312500f, 0.0468750f, 0.0286458f},
+    {-.0257202f, -.0432099f, -.0308642f, -.0185185f, -.0061728f, 0.0061728f, 0.0185185f, 0.0308642f, 0.0432099f,
+     0.0257202f},
+    {-.0233333f, -.0400000f, -.0300000f, -.0200000f, -.0100000f, 0.0000000f, 0.0100000f, 0.0200000f, 0.0300000f,
+     0.0400000f, 0.0233333f}};
+
+int make_q4_consts(const or_q4_params &p, Q4Consts &c) {
+  if (p.npt < 2 || p.npt > 10) return fail(OR_ERR_ARG, "shell NPT must be in 2..10 (NPT=1 uses MHVIS3, not on this path)");
+  if (p.ipla < 0 || p.ipla > 2) return fail(OR_ERR_ARG, "Iplas must be 0, 1 or 2");
+  if (p.iform != 0) return fail(OR_ERR_ARG, "law 2 Zerilli form (IFORM=1) is not on this path");
+  if (p.vp != 2) return fail(OR_ERR_ARG, "law 2 shells: only VP=2 (total strain rate) is on this path");
+  if (!(p.rho > 0.0) || !(p.young > 0.0) || !(p.dtfac > 0.0) || !(p.ssp > 0.0))
+    return fail(OR_ERR_ARG, "rho, young, ssp and dtfac must be positive");
+  std::memset(&c, 0, sizeof(c));
+  c.rho = p.rho; c.young = p.young; c.nu = p.nu; c.shear = p.shear; c.ssp = p.ssp; c.ca = p.ca; c.cb = p.cb;
+  c.cn = p.cn; c.epmx = p.epmx; c.ymax = p.ymax; c.cc = p.cc; c.epdr = p.epdr; c.m_exp = p.m_exp;
+  c.asrate = p.asrate; c.icc = p.icc; c.vp = p.vp; c.israte = p.israte; c.npt = p.npt; c.ipla = p.ipla;
+  c.ithk = p.ithk; c.ishear = p.ishear; c.hm = p.hm; c.hf = p.hf; c.hr = p.hr; c.fsh = p.fsh; c.dtfac = p.dtfac;
+  c.dtmin = p.dtmin;
+  {
+    volatile double zep1 = 1.0 / 10.0, zep01 = 1.0 / 100.0, zep03 = 3.0 / 100.0;
+    volatile double zep09 = zep1 - zep01;
+    volatile double zep19 = zep1 + zep09;
+    c.zep81 = 1.0 - zep19;
+    volatile double zep07 = zep1 - zep03, twoem3 = 2.0 / 1000.0;
+    volatile double ep04 = (100.0 * 10.0) * 10.0, ep05 = ep04 * 10.0, ep06 = ep05 * 10.0;
+    volatile double em04 = 1.0 / ep04, em05 = 1.0 / ep05, em06 = 1.0 / ep06;
+    volatile double sixem5 = em04 - 4.0 / ep05, nineem6 = em05 - em06;
+    volatile double s = zep07 + twoem3;
+    s = s + em04;
+    s = s + sixem5;
+    c.zep072169 = s + nineem6;
+    volatile double two = 2.0;
+    c.sr2d2 = std::sqrt(two) * 0.5;
+  }
+  c.srh1 = std::sqrt(p.hm); c.srh2 = std::sqrt(p.hf); c.srh3 = std::sqrt(p.hr);
+  for (int k = 0; k < p.npt; ++k) {
+    c.z0[k] = (double)kZ0[p.npt - 1][k];
+    c.wf[k] = (double)kWF[p.npt - 1][k];
+    c.wm[k] = (double)kWM[p.npt - 1][k];
+  }
   return OR_OK;
 }
 
@@ -352,38 +620,46 @@ template <typename T> int dalloc(T **p, int64_t n, int64_t &acc) {
 struct or_model {
   int device = 0;
   int prec = OR_PREC_FP64;
-  int64_t nn = 0, ne = 0, lsky = 0;
+  int64_t nn = 0, neh = 0, neq = 0;
+  int npt = 0;
+  int64_t lsky_h = 0, lsky_q = 0;
   int64_t bytes = 0;
   hipStream_t stream = nullptr;
-  bool have_params = false, have_nodes = false, have_hex = false, have_sky = false;
-  HexConsts consts{};
+  bool have_hparams = false, have_qparams = false, have_nodes = false, have_hex = false, have_q4 = false,
+       have_sky = false;
+  HexConsts hc{};
+  Q4Consts qc{};
   // nodes
-  double *xv = nullptr, *d = nullptr, *ms = nullptr, *vimp = nullptr, *stifn = nullptr;
-  int32_t *code = nullptr, *adsky = nullptr;
+  double *xv = nullptr, *vr = nullptr, *d = nullptr, *ms = nullptr, *in = nullptr, *vimp = nullptr;
+  int32_t *code = nullptr, *adsky_h = nullptr, *adsky_q = nullptr;
   // hex (storage type depends on precision)
-  int32_t *conn = nullptr, *iads = nullptr;
-  void *sig = nullptr, *eint = nullptr, *rho = nullptr, *vol0 = nullptr, *pla = nullptr, *epsd = nullptr,
-       *offg = nullptr, *qvis = nullptr;
-  void *sky = nullptr;
+  int32_t *hconn = nullptr, *hiads = nullptr;
+  void *hsig = nullptr, *heint = nullptr, *hrho = nullptr, *hvol0 = nullptr, *hpla = nullptr, *hepsd = nullptr,
+       *hoffg = nullptr, *hqvis = nullptr, *skh = nullptr;
+  // Q4
+  int32_t *qconn = nullptr, *qiadc = nullptr;
+  void *qoffg = nullptr, *qthk = nullptr, *qthke = nullptr, *qforc = nullptr, *qmom = nullptr, *qgstr = nullptr,
+       *qhourg = nullptr, *qeint = nullptr, *qepsd = nullptr, *qsig = nullptr, *qpla = nullptr, *qepsdip = nullptr,
+       *skq = nullptr;
   // scalars
   DevTime *ts = nullptr;               // [2]
   unsigned long long *dtmin = nullptr; // [2]
   int *flags = nullptr;
-  double *energy = nullptr;            // [2 + 2*nb]
+  double *energy = nullptr;            // [3 + 3*nb]
   int64_t parity = 0;
   // timing
   bool timing = false;
-  std::vector<hipEvent_t> ev;          // ring of 3 events per cycle
+  std::vector<hipEvent_t> ev;          // ring of 4 events per cycle (hex | q4 | node)
   int ring_used = 0;
-  double t_elem = 0.0, t_node = 0.0;
+  double t_hex = 0.0, t_q4 = 0.0, t_node = 0.0;
   int64_t t_cycles = 0;
 };
 
-static size_t state_size(const or_model *m) { return m->prec == OR_PREC_FP64 ? sizeof(double) : sizeof(float); }
+static size_t ssize(const or_model *m) { return m->prec == OR_PREC_FP64 ? sizeof(double) : sizeof(float); }
 
 // Entry points: C linkage comes from their declarations in or_gpu.h.
 
-int or_version(void) { return 100; }
+int or_version(void) { return 200; }
 
 const char *or_last_error(void) { return g_err.c_str(); }
 
@@ -396,12 +672,14 @@ int or_device_count(int *count) {
   return OR_OK;
 }
 
-int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int precision) {
+int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int64_t ne_q4, int32_t npt,
+                    int precision) {
   if (!out) return fail(OR_ERR_ARG, "null out");
   *out = nullptr;
-  if (nn <= 0 || ne_hex < 0) return fail(OR_ERR_ARG, "nn must be > 0 and ne_hex >= 0");
+  if (nn <= 0 || ne_hex < 0 || ne_q4 < 0) return fail(OR_ERR_ARG, "nn must be > 0, ne_hex and ne_q4 >= 0");
+  if (ne_q4 > 0 && (npt < 2 || npt > 10)) return fail(OR_ERR_ARG, "shell NPT must be in 2..10");
   if (precision != OR_PREC_FP64 && precision != OR_PREC_SPLIT) return fail(OR_ERR_ARG, "bad precision");
-  if (8 * ne_hex >= (int64_t)1 << 31 || nn >= (int64_t)1 << 31)
+  if (8 * ne_hex >= (int64_t)1 << 31 || 4 * ne_q4 >= (int64_t)1 << 31 || nn >= (int64_t)1 << 31)
     return fail(OR_ERR_ARG, "skyline rows and node ids must fit in int32");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(OR_ERR_NODEV, "no HIP device");
@@ -411,43 +689,60 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int 
   m->device = device;
   m->prec = precision;
   m->nn = nn;
-  m->ne = ne_hex;
-  m->lsky = 8 * ne_hex;
+  m->neh = ne_hex;
+  m->neq = ne_q4;
+  m->npt = ne_q4 > 0 ? npt : 0;
+  m->lsky_h = 8 * ne_hex;
+  m->lsky_q = 4 * ne_q4;
   int rc = OR_OK;
   auto A = [&](auto **p, int64_t n) { if (rc == OR_OK) rc = dalloc(p, n, m->bytes); };
-  A(&m->xv, 6 * nn);
-  A(&m->d, 3 * nn);
-  A(&m->ms, nn);
-  A(&m->vimp, 3 * nn);
-  A(&m->code, nn);
-  A(&m->adsky, nn + 1);
-  A(&m->conn, 8 * ne_hex);
-  A(&m->iads, 8 * ne_hex);
-  const size_t ss = state_size(m);
+  const size_t ss = ssize(m);
   auto AS = [&](void **p, int64_t n) {
     if (rc != OR_OK || n <= 0) return;
     if (hipMalloc(p, ss * (size_t)n) != hipSuccess) { rc = fail(OR_ERR_HIP, "hipMalloc state"); return; }
     m->bytes += (int64_t)(ss * (size_t)n);
   };
-  AS(&m->sig, 6 * ne_hex);
-  AS(&m->eint, ne_hex); AS(&m->rho, ne_hex); AS(&m->vol0, ne_hex); AS(&m->pla, ne_hex);
-  AS(&m->epsd, ne_hex); AS(&m->offg, ne_hex); AS(&m->qvis, ne_hex);
-  const size_t row = (precision == OR_PREC_FP64) ? sizeof(RowD) : sizeof(RowF);
-  if (rc == OR_OK && m->lsky > 0) {
-    if (hipMalloc(&m->sky, row * (size_t)m->lsky) != hipSuccess) rc = fail(OR_ERR_HIP, "hipMalloc skyline");
-    else m->bytes += (int64_t)(row * (size_t)m->lsky);
+  A(&m->xv, 6 * nn);
+  A(&m->d, 3 * nn);
+  A(&m->ms, nn);
+  A(&m->vimp, 3 * nn);
+  A(&m->code, nn);
+  if (ne_q4 > 0) {
+    A(&m->vr, 4 * nn);
+    A(&m->in, nn);
+  }
+  if (ne_hex > 0) {
+    A(&m->adsky_h, nn + 1);
+    A(&m->hconn, 8 * ne_hex);
+    A(&m->hiads, 8 * ne_hex);
+    AS(&m->hsig, 6 * ne_hex);
+    AS(&m->heint, ne_hex); AS(&m->hrho, ne_hex); AS(&m->hvol0, ne_hex); AS(&m->hpla, ne_hex);
+    AS(&m->hepsd, ne_hex); AS(&m->hoffg, ne_hex); AS(&m->hqvis, ne_hex);
+    AS(&m->skh, 4 * m->lsky_h);
+  }
+  if (ne_q4 > 0) {
+    A(&m->adsky_q, nn + 1);
+    A(&m->qconn, 4 * ne_q4);
+    A(&m->qiadc, 4 * ne_q4);
+    AS(&m->qoffg, ne_q4); AS(&m->qthk, ne_q4); AS(&m->qthke, ne_q4); AS(&m->qepsd, ne_q4);
+    AS(&m->qforc, 5 * ne_q4); AS(&m->qmom, 3 * ne_q4); AS(&m->qgstr, 8 * ne_q4); AS(&m->qhourg, 5 * ne_q4);
+    AS(&m->qeint, 2 * ne_q4);
+    AS(&m->qsig, (int64_t)npt * 5 * ne_q4);
+    AS(&m->qpla, (int64_t)npt * ne_q4);
+    AS(&m->skq, 8 * m->lsky_q);
   }
   A(&m->ts, 2);
   A(&m->dtmin, 2);
   A(&m->flags, 1);
   const int64_t nb = 1024;
-  A(&m->energy, 2 + 2 * nb);
+  A(&m->energy, 3 + 3 * nb);
   if (rc == OR_OK && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(OR_ERR_HIP, "hipStreamCreate");
   if (rc == OR_OK) {
     if (hipMemsetAsync(m->d, 0, sizeof(double) * 3 * nn, m->stream) != hipSuccess ||
         hipMemsetAsync(m->code, 0, sizeof(int32_t) * nn, m->stream) != hipSuccess ||
         hipMemsetAsync(m->vimp, 0, sizeof(double) * 3 * nn, m->stream) != hipSuccess ||
+        (m->vr && hipMemsetAsync(m->vr, 0, sizeof(double) * 4 * nn, m->stream) != hipSuccess) ||
         hipMemsetAsync(m->flags, 0, sizeof(int), m->stream) != hipSuccess)
       rc = fail(OR_ERR_HIP, "hipMemsetAsync init");
     DevTime t0{};
@@ -457,8 +752,9 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int 
     double big = kEP06;
     std::memcpy(&dm[0], &big, 8);
     dm[1] = dm[0];
-    if (rc == OR_OK && (hipMemcpy(m->ts, tsh, sizeof(tsh), hipMemcpyHostToDevice) != hipSuccess ||
-                        hipMemcpy(m->dtmin, dm, sizeof(dm), hipMemcpyHostToDevice) != hipSuccess))
+    if (rc == OR_OK && (hipMemcpyAsync(m->ts, tsh, sizeof(tsh), hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+                        hipMemcpyAsync(m->dtmin, dm, sizeof(dm), hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+                        hipStreamSynchronize(m->stream) != hipSuccess))
       rc = fail(OR_ERR_HIP, "init scalars");
   }
   if (rc != OR_OK) {
@@ -473,91 +769,115 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int 
 
 int or_model_destroy(or_model *m) {
   if (!m) return OR_OK;
-  hipSetDevice(m->device);
-  if (m->stream) hipStreamSynchronize(m->stream);
-  void *ptrs[] = {m->xv, m->d, m->ms, m->vimp, m->stifn, m->code, m->adsky, m->conn, m->iads, m->sig,
-                  m->eint, m->rho, m->vol0, m->pla, m->epsd, m->offg, m->qvis, m->sky, m->ts, m->dtmin,
-                  m->flags, m->energy};
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  void *ptrs[] = {m->xv, m->vr, m->d, m->ms, m->in, m->vimp, m->code, m->adsky_h, m->adsky_q, m->hconn, m->hiads,
+                  m->hsig, m->heint, m->hrho, m->hvol0, m->hpla, m->hepsd, m->hoffg, m->hqvis, m->skh, m->qconn,
+                  m->qiadc, m->qoffg, m->qthk, m->qthke, m->qforc, m->qmom, m->qgstr, m->qhourg, m->qeint, m->qepsd,
+                  m->qsig, m->qpla, m->qepsdip, m->skq, m->ts, m->dtmin, m->flags, m->energy};
   for (void *p : ptrs)
-    if (p) hipFree(p);
-  for (hipEvent_t e : m->ev) hipEventDestroy(e);
-  if (m->stream) hipStreamDestroy(m->stream);
+    if (p) (void)hipFree(p);
+  for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return OR_OK;
 }
 
 int or_model_set_hex_params(or_model *m, const or_hex_params *p) {
   if (!m || !p) return fail(OR_ERR_ARG, "null argument");
-  int rc = check_params(*p);
+  int rc = check_hex_params(*p);
   if (rc) return rc;
-  m->consts = make_consts(*p);
-  m->have_params = true;
+  m->hc = make_hex_consts(*p);
+  m->have_hparams = true;
   return OR_OK;
 }
 
-int or_model_upload_nodes(or_model *m, const double *x, const double *v, const double *ms, const int32_t *bcs,
-                          const int32_t *impcode, const double *vimp) {
+int or_model_set_q4_params(or_model *m, const or_q4_params *p) {
+  if (!m || !p) return fail(OR_ERR_ARG, "null argument");
+  if (m->neq > 0 && p->npt != m->npt) return fail(OR_ERR_ARG, "q4 params NPT differs from the model's NPT");
+  Q4Consts c;
+  int rc = make_q4_consts(*p, c);
+  if (rc) return rc;
+  // LBUF%EPSD is only distinct from GBUF%EPSD when the strain-rate filter is on (ASRATE < 1)
+  if (c.israte > 0 && m->neq > 0 && !m->qepsdip) {
+    const size_t ss = ssize(m);
+    OR_HIP(hipSetDevice(m->device));
+    OR_HIP(hipMalloc(&m->qepsdip, ss * (size_t)(m->npt * m->neq)));
+    OR_HIP(hipMemsetAsync(m->qepsdip, 0, ss * (size_t)(m->npt * m->neq), m->stream));
+    OR_HIP(hipStreamSynchronize(m->stream));
+    m->bytes += (int64_t)(ss * (size_t)(m->npt * m->neq));
+  }
+  m->qc = c;
+  m->have_qparams = true;
+  return OR_OK;
+}
+
+int or_model_upload_nodes(or_model *m, const double *x, const double *v, const double *vr, const double *ms,
+                          const double *in, const int32_t *code, const double *vimp) {
   if (!m || !x || !v || !ms) return fail(OR_ERR_ARG, "null argument");
-  if (impcode && !vimp) return fail(OR_ERR_ARG, "impcode given without vimp");
+  if (m->neq > 0 && !in) return fail(OR_ERR_ARG, "a model with shells needs nodal inertias IN");
   OR_HIP(hipSetDevice(m->device));
   const int64_t nn = m->nn;
   std::vector<double> xv((size_t)(6 * nn));
-  std::vector<int32_t> code((size_t)nn, 0);
   for (int64_t n = 0; n < nn; ++n) {
     xv[6 * n + 0] = x[3 * n]; xv[6 * n + 1] = x[3 * n + 1]; xv[6 * n + 2] = x[3 * n + 2];
     xv[6 * n + 3] = v[3 * n]; xv[6 * n + 4] = v[3 * n + 1]; xv[6 * n + 5] = v[3 * n + 2];
-    int32_t cd = 0;
-    if (bcs) {
-      if (bcs[n] < 0 || bcs[n] > 7) return fail(OR_ERR_ARG, "BCS code must be in 0..7");
-      cd |= bcs[n];
-    }
-    if (impcode) {
-      if (impcode[n] < 0 || impcode[n] > 7) return fail(OR_ERR_ARG, "impvel code must be in 0..7");
-      cd |= impcode[n] << 3;
-    }
-    code[n] = cd;
+    if (code && (code[n] & ~511)) return fail(OR_ERR_ARG, "node code has bits outside 0..511");
+    if (code && (code[n] & (kIX | kIY | kIZ)) && !vimp)
+      return fail(OR_ERR_ARG, "imposed-velocity bits given without vimp");
   }
   OR_HIP(hipMemcpyAsync(m->xv, xv.data(), sizeof(double) * 6 * nn, hipMemcpyHostToDevice, m->stream));
   OR_HIP(hipMemcpyAsync(m->ms, ms, sizeof(double) * nn, hipMemcpyHostToDevice, m->stream));
-  OR_HIP(hipMemcpyAsync(m->code, code.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, m->stream));
+  if (code) OR_HIP(hipMemcpyAsync(m->code, code, sizeof(int32_t) * nn, hipMemcpyHostToDevice, m->stream));
   if (vimp) OR_HIP(hipMemcpyAsync(m->vimp, vimp, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, m->stream));
+  std::vector<double> vr4;
+  if (m->vr) {
+    vr4.assign((size_t)(4 * nn), 0.0);
+    if (vr)
+      for (int64_t n = 0; n < nn; ++n) {
+        vr4[4 * n] = vr[3 * n]; vr4[4 * n + 1] = vr[3 * n + 1]; vr4[4 * n + 2] = vr[3 * n + 2];
+      }
+    OR_HIP(hipMemcpyAsync(m->vr, vr4.data(), sizeof(double) * 4 * nn, hipMemcpyHostToDevice, m->stream));
+    OR_HIP(hipMemcpyAsync(m->in, in, sizeof(double) * nn, hipMemcpyHostToDevice, m->stream));
+  }
   OR_HIP(hipMemsetAsync(m->d, 0, sizeof(double) * 3 * nn, m->stream));
   OR_HIP(hipStreamSynchronize(m->stream));
   m->have_nodes = true;
   return OR_OK;
 }
 
-template <typename S> static int upload_state(or_model *m, void *dst, const double *src, int64_t n) {
-  if (!src) return fail(OR_ERR_ARG, "null state array");
-  if (sizeof(S) == sizeof(double)) {
-    OR_HIP(hipMemcpy(dst, src, sizeof(double) * n, hipMemcpyHostToDevice));
-  } else {
-    std::vector<float> t((size_t)n);
-    for (int64_t i = 0; i < n; ++i) t[i] = (float)src[i];
-    OR_HIP(hipMemcpy(dst, t.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-  }
+template <typename S> static int put(or_model *m, void *dst, const double *src, int64_t n, double dflt) {
+  std::vector<S> t((size_t)n);
+  for (int64_t i = 0; i < n; ++i) t[i] = (S)(src ? src[i] : dflt);
+  OR_HIP(hipMemcpyAsync(dst, t.data(), sizeof(S) * n, hipMemcpyHostToDevice, m->stream));
+  OR_HIP(hipStreamSynchronize(m->stream));
   return OR_OK;
 }
-
-template <typename S> static int download_state(or_model *m, double *dst, const void *src, int64_t n) {
-  if (!dst) return OR_OK;
-  if (sizeof(S) == sizeof(double)) {
-    OR_HIP(hipMemcpy(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost));
-  } else {
-    std::vector<float> t((size_t)n);
-    OR_HIP(hipMemcpy(t.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < n; ++i) dst[i] = (double)t[i];
-  }
+static int put_state(or_model *m, void *dst, const double *src, int64_t n, double dflt) {
+  if (!dst || n <= 0) return OR_OK;
+  return m->prec == OR_PREC_FP64 ? put<double>(m, dst, src, n, dflt) : put<float>(m, dst, src, n, dflt);
+}
+template <typename S> static int get(or_model *m, double *dst, const void *src, int64_t n) {
+  std::vector<S> t((size_t)n);
+  OR_HIP(hipMemcpyAsync(t.data(), src, sizeof(S) * n, hipMemcpyDeviceToHost, m->stream));
+  OR_HIP(hipStreamSynchronize(m->stream));
+  for (int64_t i = 0; i < n; ++i) dst[i] = (double)t[i];
   return OR_OK;
+}
+static int get_state(or_model *m, double *dst, const void *src, int64_t n) {
+  if (!dst || n <= 0) return OR_OK;
+  if (!src) return fail(OR_ERR_STATE, "state array not held by this model");
+  return m->prec == OR_PREC_FP64 ? get<double>(m, dst, src, n) : get<float>(m, dst, src, n);
 }
 
 int or_model_upload_hex(or_model *m, const int32_t *conn, const double *sig, const double *eint, const double *rho,
                         const double *vol0, const double *pla, const double *epsd, const double *offg,
                         const double *qvis) {
-  if (!m || !conn) return fail(OR_ERR_ARG, "null argument");
-  OR_HIP(hipSetDevice(m->device));
-  const int64_t ne = m->ne;
+  if (!m) return fail(OR_ERR_ARG, "null model");
+  const int64_t ne = m->neh;
   if (ne == 0) { m->have_hex = true; return OR_OK; }
+  if (!conn || !vol0 || !rho) return fail(OR_ERR_ARG, "conn, rho and vol0 are required");
+  OR_HIP(hipSetDevice(m->device));
   std::vector<int32_t> soa((size_t)(8 * ne));
   for (int64_t e = 0; e < ne; ++e)
     for (int k = 0; k < 8; ++k) {
@@ -565,54 +885,126 @@ int or_model_upload_hex(or_model *m, const int32_t *conn, const double *sig, con
       if (n < 0 || n >= m->nn) return fail(OR_ERR_ARG, "connectivity node id out of range");
       soa[(size_t)(k * ne + e)] = n;
     }
-  OR_HIP(hipMemcpy(m->conn, soa.data(), sizeof(int32_t) * 8 * ne, hipMemcpyHostToDevice));
+  OR_HIP(hipMemcpyAsync(m->hconn, soa.data(), sizeof(int32_t) * 8 * ne, hipMemcpyHostToDevice, m->stream));
+  OR_HIP(hipStreamSynchronize(m->stream));
   int rc;
-  if (m->prec == OR_PREC_FP64) {
-    if ((rc = upload_state<double>(m, m->sig, sig, 6 * ne))) return rc;
-    if ((rc = upload_state<double>(m, m->eint, eint, ne))) return rc;
-    if ((rc = upload_state<double>(m, m->rho, rho, ne))) return rc;
-    if ((rc = upload_state<double>(m, m->vol0, vol0, ne))) return rc;
-    if ((rc = upload_state<double>(m, m->pla, pla, ne))) return rc;
-    if ((rc = upload_state<double>(m, m->epsd, epsd, ne))) return rc;
-    if ((rc = upload_state<double>(m, m->offg, offg, ne))) return rc;
-    if ((rc = upload_state<double>(m, m->qvis, qvis, ne))) return rc;
-  } else {
-    if ((rc = upload_state<float>(m, m->sig, sig, 6 * ne))) return rc;
-    if ((rc = upload_state<float>(m, m->eint, eint, ne))) return rc;
-    if ((rc = upload_state<float>(m, m->rho, rho, ne))) return rc;
-    if ((rc = upload_state<float>(m, m->vol0, vol0, ne))) return rc;
-    if ((rc = upload_state<float>(m, m->pla, pla, ne))) return rc;
-    if ((rc = upload_state<float>(m, m->epsd, epsd, ne))) return rc;
-    if ((rc = upload_state<float>(m, m->offg, offg, ne))) return rc;
-    if ((rc = upload_state<float>(m, m->qvis, qvis, ne))) return rc;
-  }
+  if ((rc = put_state(m, m->hsig, sig, 6 * ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->heint, eint, ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->hrho, rho, ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->hvol0, vol0, ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->hpla, pla, ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->hepsd, epsd, ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->hoffg, offg, ne, 1.0))) return rc;
+  if ((rc = put_state(m, m->hqvis, qvis, ne, 0.0))) return rc;
   m->have_hex = true;
   return OR_OK;
 }
 
-int or_model_upload_skyline(or_model *m, const int64_t *adsky, const int64_t *iads) {
-  if (!m || !adsky || (m->ne > 0 && !iads)) return fail(OR_ERR_ARG, "null argument");
+int or_model_upload_q4(or_model *m, const int32_t *conn, const double *thke, const double *offg, const double *thk,
+                       const double *forc, const double *mom, const double *gstr, const double *hourg,
+                       const double *eint, const double *epsd, const double *sig, const double *pla,
+                       const double *epsd_ip) {
+  if (!m) return fail(OR_ERR_ARG, "null model");
+  const int64_t ne = m->neq;
+  if (ne == 0) { m->have_q4 = true; return OR_OK; }
+  if (!conn || !thke) return fail(OR_ERR_ARG, "conn and thke are required");
   OR_HIP(hipSetDevice(m->device));
-  const int64_t nn = m->nn, ne = m->ne;
-  if (adsky[0] != 0 || adsky[nn] != m->lsky)
-    return fail(OR_ERR_ARG, "adsky must start at 0 and end at 8*ne");
-  std::vector<int32_t> a32((size_t)(nn + 1));
-  for (int64_t n = 0; n <= nn; ++n) {
-    if (n > 0 && adsky[n] < adsky[n - 1]) return fail(OR_ERR_ARG, "adsky must be non-decreasing");
-    a32[n] = (int32_t)adsky[n];
+  std::vector<int32_t> soa((size_t)(4 * ne));
+  for (int64_t e = 0; e < ne; ++e) {
+    if (!(thke[e] > 0.0)) return fail(OR_ERR_ARG, "shell thickness must be positive");
+    for (int k = 0; k < 4; ++k) {
+      const int32_t n = conn[4 * e + k];
+      if (n < 0 || n >= m->nn) return fail(OR_ERR_ARG, "connectivity node id out of range");
+      soa[(size_t)(k * ne + e)] = n;
+    }
   }
-  std::vector<int32_t> soa((size_t)(8 * ne));
-  std::vector<uint8_t> seen((size_t)m->lsky, 0);
-  for (int64_t e = 0; e < ne; ++e)
+  OR_HIP(hipMemcpyAsync(m->qconn, soa.data(), sizeof(int32_t) * 4 * ne, hipMemcpyHostToDevice, m->stream));
+  OR_HIP(hipStreamSynchronize(m->stream));
+  const int64_t np = m->npt;
+  int rc;
+  if ((rc = put_state(m, m->qthke, thke, ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->qoffg, offg, ne, 1.0))) return rc;
+  if ((rc = put_state(m, m->qthk, thk ? thk : thke, ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->qforc, forc, 5 * ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->qmom, mom, 3 * ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->qgstr, gstr, 8 * ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->qhourg, hourg, 5 * ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->qeint, eint, 2 * ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->qepsd, epsd, ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->qsig, sig, np * 5 * ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->qpla, pla, np * ne, 0.0))) return rc;
+  if (m->qepsdip && (rc = put_state(m, m->qepsdip, epsd_ip, np * ne, 0.0))) return rc;
+  m->have_q4 = true;
+  return OR_OK;
+}
+
+int or_model_upload_skyline(or_model *m, const int64_t *adsky, const int64_t *iads, const int64_t *iadc) {
+  if (!m || !adsky) return fail(OR_ERR_ARG, "null argument");
+  if ((m->neh > 0 && !iads) || (m->neq > 0 && !iadc)) return fail(OR_ERR_ARG, "missing iads / iadc");
+  OR_HIP(hipSetDevice(m->device));
+  const int64_t nn = m->nn, neh = m->neh, neq = m->neq;
+  const int64_t lsky = m->lsky_h + m->lsky_q;
+  if (adsky[0] != 0 || adsky[nn] != lsky) return fail(OR_ERR_ARG, "adsky must start at 0 and end at 8*ne_hex+4*ne_q4");
+  for (int64_t n = 0; n < nn; ++n)
+    if (adsky[n + 1] < adsky[n]) return fail(OR_ERR_ARG, "adsky must be non-decreasing");
+  // owner node of every global row, and the family of every row
+  std::vector<int32_t> owner((size_t)lsky);
+  for (int64_t n = 0; n < nn; ++n)
+    for (int64_t k = adsky[n]; k < adsky[n + 1]; ++k) owner[(size_t)k] = (int32_t)n;
+  std::vector<int8_t> fam((size_t)lsky, -1);
+  std::vector<int32_t> nh((size_t)nn, 0), nq((size_t)nn, 0);
+  auto claim = [&](int64_t K, int f) -> int {
+    if (K < 0 || K >= lsky) return fail(OR_ERR_ARG, "skyline row out of range");
+    if (fam[(size_t)K] != -1) return fail(OR_ERR_ARG, "skyline rows must be distinct");
+    fam[(size_t)K] = (int8_t)f;
+    return OR_OK;
+  };
+  int rc;
+  for (int64_t e = 0; e < neh; ++e)
     for (int k = 0; k < 8; ++k) {
       const int64_t K = iads[8 * e + k];
-      if (K < 0 || K >= m->lsky) return fail(OR_ERR_ARG, "iads row out of range");
-      if (seen[K]) return fail(OR_ERR_ARG, "iads rows must be distinct");
-      seen[K] = 1;
-      soa[(size_t)(k * ne + e)] = (int32_t)K;
+      if ((rc = claim(K, 0))) return rc;
+      // the hex node must own the row
+      std::vector<int32_t> dummy;
+      nh[(size_t)owner[(size_t)K]] += 1;
     }
-  OR_HIP(hipMemcpy(m->adsky, a32.data(), sizeof(int32_t) * (nn + 1), hipMemcpyHostToDevice));
-  if (ne) OR_HIP(hipMemcpy(m->iads, soa.data(), sizeof(int32_t) * 8 * ne, hipMemcpyHostToDevice));
+  for (int64_t e = 0; e < neq; ++e)
+    for (int k = 0; k < 4; ++k) {
+      const int64_t K = iadc[4 * e + k];
+      if ((rc = claim(K, 1))) return rc;
+      nq[(size_t)owner[(size_t)K]] += 1;
+    }
+  // FILLCNE order: the solid rows of a node come before its shell rows
+  for (int64_t n = 0; n < nn; ++n)
+    for (int64_t k = adsky[n]; k < adsky[n + 1]; ++k)
+      if ((k - adsky[n] < nh[(size_t)n]) != (fam[(size_t)k] == 0))
+        return fail(OR_ERR_ARG, "skyline rows of a node must list solid rows before shell rows (FILLCNE order)");
+  std::vector<int32_t> ah((size_t)(nn + 1), 0), aq((size_t)(nn + 1), 0);
+  for (int64_t n = 0; n < nn; ++n) {
+    ah[(size_t)(n + 1)] = ah[(size_t)n] + nh[(size_t)n];
+    aq[(size_t)(n + 1)] = aq[(size_t)n] + nq[(size_t)n];
+  }
+  auto local = [&](int64_t K) -> int32_t {
+    const int32_t n = owner[(size_t)K];
+    const int64_t j = K - adsky[n];
+    return fam[(size_t)K] == 0 ? (int32_t)(ah[(size_t)n] + j) : (int32_t)(aq[(size_t)n] + (j - nh[(size_t)n]));
+  };
+  if (neh > 0) {
+    std::vector<int32_t> soa((size_t)(8 * neh));
+    for (int64_t e = 0; e < neh; ++e)
+      for (int k = 0; k < 8; ++k) soa[(size_t)(k * neh + e)] = local(iads[8 * e + k]);
+    OR_HIP(hipMemcpyAsync(m->adsky_h, ah.data(), sizeof(int32_t) * (nn + 1), hipMemcpyHostToDevice, m->stream));
+    OR_HIP(hipMemcpyAsync(m->hiads, soa.data(), sizeof(int32_t) * 8 * neh, hipMemcpyHostToDevice, m->stream));
+    OR_HIP(hipStreamSynchronize(m->stream));
+  }
+  if (neq > 0) {
+    std::vector<int32_t> soa((size_t)(4 * neq));
+    for (int64_t e = 0; e < neq; ++e)
+      for (int k = 0; k < 4; ++k) soa[(size_t)(k * neq + e)] = local(iadc[4 * e + k]);
+    OR_HIP(hipMemcpyAsync(m->adsky_q, aq.data(), sizeof(int32_t) * (nn + 1), hipMemcpyHostToDevice, m->stream));
+    OR_HIP(hipMemcpyAsync(m->qiadc, soa.data(), sizeof(int32_t) * 4 * neq, hipMemcpyHostToDevice, m->stream));
+    OR_HIP(hipStreamSynchronize(m->stream));
+  }
   m->have_sky = true;
   return OR_OK;
 }
@@ -647,34 +1039,49 @@ int or_model_get_time(or_model *m, or_time_state *t) {
 
 static int flush_timing(or_model *m) {
   if (m->ring_used == 0) return OR_OK;
-  OR_HIP(hipEventSynchronize(m->ev[3 * (m->ring_used - 1) + 2]));
+  OR_HIP(hipEventSynchronize(m->ev[4 * (m->ring_used - 1) + 3]));
   for (int j = 0; j < m->ring_used; ++j) {
-    float a = 0.f, b = 0.f;
-    OR_HIP(hipEventElapsedTime(&a, m->ev[3 * j], m->ev[3 * j + 1]));
-    OR_HIP(hipEventElapsedTime(&b, m->ev[3 * j + 1], m->ev[3 * j + 2]));
-    m->t_elem += a;
-    m->t_node += b;
+    float a = 0.f, b = 0.f, c = 0.f;
+    OR_HIP(hipEventElapsedTime(&a, m->ev[4 * j], m->ev[4 * j + 1]));
+    OR_HIP(hipEventElapsedTime(&b, m->ev[4 * j + 1], m->ev[4 * j + 2]));
+    OR_HIP(hipEventElapsedTime(&c, m->ev[4 * j + 2], m->ev[4 * j + 3]));
+    m->t_hex += a;
+    m->t_q4 += b;
+    m->t_node += c;
     m->t_cycles += 1;
   }
   m->ring_used = 0;
   return OR_OK;
 }
 
-template <typename R, typename S, typename Row> static int launch_cycle(or_model *m) {
+template <typename R, typename S> static int launch_cycle(or_model *m) {
   const int p = (int)(m->parity & 1);
-  NodePtrs nd{m->xv, m->d, m->ms, m->code, m->vimp, m->stifn, m->adsky, m->nn};
-  if (m->ne > 0) {
-    HexPtrs<S> e{m->conn, m->iads, (S *)m->sig, (S *)m->eint, (S *)m->rho, (S *)m->pla, (S *)m->epsd,
-                 (S *)m->offg, (S *)m->qvis, (const S *)m->vol0, m->ne};
-    const unsigned gb = (unsigned)((m->ne + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((k_hex_forces<R, S, Row>), dim3(gb), dim3(kBlock), 0, m->stream, e, m->xv, (Row *)m->sky,
-                       m->consts, m->ts, p, m->dtmin, m->flags, (S *)nullptr);
+  if (m->neh > 0) {
+    HexPtrs<S> e{m->hconn, m->hiads, (S *)m->hsig, (S *)m->heint, (S *)m->hrho, (S *)m->hpla, (S *)m->hepsd,
+                 (S *)m->hoffg, (S *)m->hqvis, (const S *)m->hvol0, m->neh};
+    const unsigned gb = (unsigned)((m->neh + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((k_hex_forces<R, S>), dim3(gb), dim3(kBlock), 0, m->stream, e, m->xv, (Row4<S> *)m->skh,
+                       m->hc, m->ts, p, m->dtmin, m->flags);
     OR_HIP(hipGetLastError());
   }
-  if (m->timing) OR_HIP(hipEventRecord(m->ev[3 * m->ring_used + 1], m->stream));
+  if (m->timing) OR_HIP(hipEventRecord(m->ev[4 * m->ring_used + 1], m->stream));
+  if (m->neq > 0) {
+    Q4Ptrs<S> q{m->qconn, m->qiadc, (S *)m->qoffg, (S *)m->qthk, (const S *)m->qthke, (S *)m->qforc, (S *)m->qmom,
+                (S *)m->qgstr, (S *)m->qhourg, (S *)m->qeint, (S *)m->qepsd, (S *)m->qsig, (S *)m->qpla,
+                (S *)m->qepsdip, m->neq};
+    const unsigned gq = (unsigned)((m->neq + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((k_q4_forces<R, S>), dim3(gq), dim3(kBlock), 0, m->stream, q, m->xv, m->vr,
+                       (Row8<S> *)m->skq, m->qc, m->ts, p, m->dtmin);
+    OR_HIP(hipGetLastError());
+  }
+  if (m->timing) OR_HIP(hipEventRecord(m->ev[4 * m->ring_used + 2], m->stream));
+  NodePtrs nd{m->xv, m->vr, m->d, m->ms, m->in, m->code, m->vimp, m->adsky_h, m->adsky_q, m->nn};
   const unsigned gn = (unsigned)((m->nn + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL((k_node_update<Row>), dim3(gn), dim3(kBlock), 0, m->stream, nd, (const Row *)m->sky, m->ts, p,
-                     m->dtmin, m->flags, m->consts.dt_growth, m->consts.dtmx);
+  const double dt_growth = (double)1.1f;
+  const double dtmx = m->neh > 0 ? m->hc.dtmx : 1.0e20;
+  hipLaunchKernelGGL((k_node_update<S>), dim3(gn), dim3(kBlock), 0, m->stream, nd,
+                     (const Row4<S> *)(m->neh > 0 ? m->skh : nullptr), (const Row8<S> *)(m->neq > 0 ? m->skq : nullptr),
+                     m->ts, p, m->dtmin, m->flags, dt_growth, dtmx);
   OR_HIP(hipGetLastError());
   m->parity ^= 1;
   return OR_OK;
@@ -683,21 +1090,22 @@ template <typename R, typename S, typename Row> static int launch_cycle(or_model
 int or_model_run(or_model *m, int64_t ncycles) {
   if (!m) return fail(OR_ERR_ARG, "null model");
   if (ncycles < 0) return fail(OR_ERR_ARG, "ncycles < 0");
-  if (!m->have_params || !m->have_nodes || !m->have_hex || !m->have_sky)
-    return fail(OR_ERR_STATE, "params, nodes, hex state and skyline must be uploaded before run");
+  if (!m->have_nodes || !m->have_sky || (m->neh > 0 && (!m->have_hparams || !m->have_hex)) ||
+      (m->neq > 0 && (!m->have_qparams || !m->have_q4)))
+    return fail(OR_ERR_STATE, "params, nodes, element state and skyline must be uploaded before run");
   OR_HIP(hipSetDevice(m->device));
   for (int64_t c = 0; c < ncycles; ++c) {
     if (m->timing) {
-      if (m->ring_used * 3 + 3 > (int)m->ev.size()) {
+      if (m->ring_used * 4 + 4 > (int)m->ev.size()) {
         int rc = flush_timing(m);
         if (rc) return rc;
       }
-      OR_HIP(hipEventRecord(m->ev[3 * m->ring_used], m->stream));
+      OR_HIP(hipEventRecord(m->ev[4 * m->ring_used], m->stream));
     }
-    int rc = (m->prec == OR_PREC_FP64) ? launch_cycle<double, double, RowD>(m) : launch_cycle<float, float, RowF>(m);
+    int rc = (m->prec == OR_PREC_FP64) ? launch_cycle<double, double>(m) : launch_cycle<float, float>(m);
     if (rc) return rc;
     if (m->timing) {
-      OR_HIP(hipEventRecord(m->ev[3 * m->ring_used + 2], m->stream));
+      OR_HIP(hipEventRecord(m->ev[4 * m->ring_used + 3], m->stream));
       m->ring_used += 1;
     }
   }
@@ -718,27 +1126,28 @@ int or_model_set_timing(or_model *m, int enable) {
   if (!m) return fail(OR_ERR_ARG, "null model");
   OR_HIP(hipSetDevice(m->device));
   if (enable && m->ev.empty()) {
-    m->ev.resize(3 * 512);
+    m->ev.resize(4 * 256);
     for (auto &e : m->ev) OR_HIP(hipEventCreate(&e));
   }
   m->timing = enable != 0;
   return OR_OK;
 }
 
-int or_model_kernel_times(or_model *m, int reset, double *elem_ms, double *node_ms, int64_t *cycles) {
+int or_model_kernel_times(or_model *m, int reset, double *hex_ms, double *q4_ms, double *node_ms, int64_t *cycles) {
   if (!m) return fail(OR_ERR_ARG, "null model");
   OR_HIP(hipSetDevice(m->device));
   int rc = flush_timing(m);
   if (rc) return rc;
   const double nc = m->t_cycles > 0 ? (double)m->t_cycles : 1.0;
-  if (elem_ms) *elem_ms = m->t_elem / nc;
+  if (hex_ms) *hex_ms = m->t_hex / nc;
+  if (q4_ms) *q4_ms = m->t_q4 / nc;
   if (node_ms) *node_ms = m->t_node / nc;
   if (cycles) *cycles = m->t_cycles;
-  if (reset) { m->t_elem = 0.0; m->t_node = 0.0; m->t_cycles = 0; }
+  if (reset) { m->t_hex = 0.0; m->t_q4 = 0.0; m->t_node = 0.0; m->t_cycles = 0; }
   return OR_OK;
 }
 
-int or_model_download_nodes(or_model *m, double *x, double *v, double *d, double *stifn) {
+int or_model_download_nodes(or_model *m, double *x, double *v, double *vr, double *d) {
   if (!m) return fail(OR_ERR_ARG, "null model");
   OR_HIP(hipSetDevice(m->device));
   OR_HIP(hipStreamSynchronize(m->stream));
@@ -749,11 +1158,16 @@ int or_model_download_nodes(or_model *m, double *x, double *v, double *d, double
     if (x) { x[3 * n] = xv[6 * n]; x[3 * n + 1] = xv[6 * n + 1]; x[3 * n + 2] = xv[6 * n + 2]; }
     if (v) { v[3 * n] = xv[6 * n + 3]; v[3 * n + 1] = xv[6 * n + 4]; v[3 * n + 2] = xv[6 * n + 5]; }
   }
-  if (d) OR_HIP(hipMemcpy(d, m->d, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost));
-  if (stifn) {
-    if (!m->stifn) return fail(OR_ERR_STATE, "STIFN assembly is not enabled on this model");
-    OR_HIP(hipMemcpy(stifn, m->stifn, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  if (vr) {
+    if (m->vr) {
+      std::vector<double> w((size_t)(4 * nn));
+      OR_HIP(hipMemcpy(w.data(), m->vr, sizeof(double) * 4 * nn, hipMemcpyDeviceToHost));
+      for (int64_t n = 0; n < nn; ++n) { vr[3 * n] = w[4 * n]; vr[3 * n + 1] = w[4 * n + 1]; vr[3 * n + 2] = w[4 * n + 2]; }
+    } else {
+      std::memset(vr, 0, sizeof(double) * 3 * nn);
+    }
   }
+  if (d) OR_HIP(hipMemcpy(d, m->d, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost));
   return OR_OK;
 }
 
@@ -762,47 +1176,71 @@ int or_model_download_hex(or_model *m, double *sig, double *eint, double *rho, d
   if (!m) return fail(OR_ERR_ARG, "null model");
   OR_HIP(hipSetDevice(m->device));
   OR_HIP(hipStreamSynchronize(m->stream));
-  const int64_t ne = m->ne;
+  const int64_t ne = m->neh;
   int rc;
-  if (m->prec == OR_PREC_FP64) {
-    if ((rc = download_state<double>(m, sig, m->sig, 6 * ne))) return rc;
-    if ((rc = download_state<double>(m, eint, m->eint, ne))) return rc;
-    if ((rc = download_state<double>(m, rho, m->rho, ne))) return rc;
-    if ((rc = download_state<double>(m, pla, m->pla, ne))) return rc;
-    if ((rc = download_state<double>(m, epsd, m->epsd, ne))) return rc;
-    if ((rc = download_state<double>(m, offg, m->offg, ne))) return rc;
-    if ((rc = download_state<double>(m, qvis, m->qvis, ne))) return rc;
-  } else {
-    if ((rc = download_state<float>(m, sig, m->sig, 6 * ne))) return rc;
-    if ((rc = download_state<float>(m, eint, m->eint, ne))) return rc;
-    if ((rc = download_state<float>(m, rho, m->rho, ne))) return rc;
-    if ((rc = download_state<float>(m, pla, m->pla, ne))) return rc;
-    if ((rc = download_state<float>(m, epsd, m->epsd, ne))) return rc;
-    if ((rc = download_state<float>(m, offg, m->offg, ne))) return rc;
-    if ((rc = download_state<float>(m, qvis, m->qvis, ne))) return rc;
+  if ((rc = get_state(m, sig, m->hsig, 6 * ne))) return rc;
+  if ((rc = get_state(m, eint, m->heint, ne))) return rc;
+  if ((rc = get_state(m, rho, m->hrho, ne))) return rc;
+  if ((rc = get_state(m, pla, m->hpla, ne))) return rc;
+  if ((rc = get_state(m, epsd, m->hepsd, ne))) return rc;
+  if ((rc = get_state(m, offg, m->hoffg, ne))) return rc;
+  if ((rc = get_state(m, qvis, m->hqvis, ne))) return rc;
+  return OR_OK;
+}
+
+int or_model_download_q4(or_model *m, double *offg, double *thk, double *forc, double *mom, double *gstr,
+                         double *hourg, double *eint, double *epsd, double *sig, double *pla, double *epsd_ip) {
+  if (!m) return fail(OR_ERR_ARG, "null model");
+  OR_HIP(hipSetDevice(m->device));
+  OR_HIP(hipStreamSynchronize(m->stream));
+  const int64_t ne = m->neq, np = m->npt;
+  int rc;
+  if ((rc = get_state(m, offg, m->qoffg, ne))) return rc;
+  if ((rc = get_state(m, thk, m->qthk, ne))) return rc;
+  if ((rc = get_state(m, forc, m->qforc, 5 * ne))) return rc;
+  if ((rc = get_state(m, mom, m->qmom, 3 * ne))) return rc;
+  if ((rc = get_state(m, gstr, m->qgstr, 8 * ne))) return rc;
+  if ((rc = get_state(m, hourg, m->qhourg, 5 * ne))) return rc;
+  if ((rc = get_state(m, eint, m->qeint, 2 * ne))) return rc;
+  if ((rc = get_state(m, epsd, m->qepsd, ne))) return rc;
+  if ((rc = get_state(m, sig, m->qsig, np * 5 * ne))) return rc;
+  if ((rc = get_state(m, pla, m->qpla, np * ne))) return rc;
+  if (epsd_ip) {
+    if (m->qepsdip) {
+      if ((rc = get_state(m, epsd_ip, m->qepsdip, np * ne))) return rc;
+    } else {
+      // ASRATE = 1: LBUF%EPSD = 1*EPSD_PG + 0*LBUF%EPSD equals GBUF%EPSD at every point
+      std::vector<double> g((size_t)ne);
+      if ((rc = get_state(m, g.data(), m->qepsd, ne))) return rc;
+      for (int64_t ip = 0; ip < np; ++ip) std::memcpy(epsd_ip + ip * ne, g.data(), sizeof(double) * ne);
+    }
   }
   return OR_OK;
 }
 
-int or_model_energy(or_model *m, double out[2]) {
+int or_model_energy(or_model *m, double out[3]) {
   if (!m || !out) return fail(OR_ERR_ARG, "null argument");
   OR_HIP(hipSetDevice(m->device));
-  const int64_t n = m->nn > m->ne ? m->nn : m->ne;
+  int64_t n = m->nn;
+  if (m->neh > n) n = m->neh;
+  if (m->neq > n) n = m->neq;
   const int nb = 1024;
   const int64_t per = (n + nb - 1) / nb;
   const int64_t per_block = ((per + kBlock - 1) / kBlock) * kBlock;
-  double *partial = m->energy + 2;
+  double *partial = m->energy + 3;
   if (m->prec == OR_PREC_FP64)
-    hipLaunchKernelGGL((k_energy_partial<double>), dim3(nb), dim3(kBlock), 0, m->stream, (const double *)m->eint,
-                       (const double *)m->vol0, m->ne, m->xv, m->ms, m->nn, per_block, partial);
+    hipLaunchKernelGGL((k_energy_partial<double>), dim3(nb), dim3(kBlock), 0, m->stream, (const double *)m->heint,
+                       (const double *)m->hvol0, m->neh, (const double *)m->qeint, m->neq, m->xv, m->vr, m->ms, m->in,
+                       m->nn, per_block, partial);
   else
-    hipLaunchKernelGGL((k_energy_partial<float>), dim3(nb), dim3(kBlock), 0, m->stream, (const float *)m->eint,
-                       (const float *)m->vol0, m->ne, m->xv, m->ms, m->nn, per_block, partial);
+    hipLaunchKernelGGL((k_energy_partial<float>), dim3(nb), dim3(kBlock), 0, m->stream, (const float *)m->heint,
+                       (const float *)m->hvol0, m->neh, (const float *)m->qeint, m->neq, m->xv, m->vr, m->ms, m->in,
+                       m->nn, per_block, partial);
   OR_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, m->stream, (const double *)partial, nb, m->energy);
   OR_HIP(hipGetLastError());
   OR_HIP(hipStreamSynchronize(m->stream));
-  OR_HIP(hipMemcpy(out, m->energy, 2 * sizeof(double), hipMemcpyDeviceToHost));
+  OR_HIP(hipMemcpy(out, m->energy, 3 * sizeof(double), hipMemcpyDeviceToHost));
   return OR_OK;
 }
 
@@ -811,6 +1249,29 @@ int or_model_footprint(const or_model *m, int64_t *bytes) {
   *bytes = m->bytes;
   return OR_OK;
 }
+
+// ============================================================== group drop-ins
+namespace {
+// Device staging of one group call: every buffer allocated, copied, freed here.
+struct Stage {
+  std::vector<void *> ptrs;
+  hipError_t e = hipSuccess;
+  ~Stage() {
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+  }
+  template <typename T> T *up(const T *h, size_t n) {
+    T *d = nullptr;
+    if (e == hipSuccess) e = hipMalloc((void **)&d, sizeof(T) * (n ? n : 1));
+    ptrs.push_back(d);
+    if (e == hipSuccess && h && n) e = hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice);
+    return d;
+  }
+  template <typename T> void down(T *h, const T *d, size_t n) {
+    if (e == hipSuccess && h && n) e = hipMemcpy(h, d, sizeof(T) * n, hipMemcpyDeviceToHost);
+  }
+};
+} // namespace
 
 int or_sforc3_group(int device, const or_hex_params *p, int64_t numnod, int32_t nel, int64_t nft, const double *x,
                     const double *v, const int32_t *ixs, double *sig, double *eint, double *rho, const double *vol,
@@ -821,74 +1282,38 @@ int or_sforc3_group(int device, const or_hex_params *p, int64_t numnod, int32_t 
     return fail(OR_ERR_ARG, "null argument");
   if (nel <= 0 || nel > 128) return fail(OR_ERR_ARG, "nel must be in 1..128 (MVSIZ-1)");
   if (numnod <= 0 || lsky <= 0 || nft < 0) return fail(OR_ERR_ARG, "bad sizes");
-  int rc = check_params(*p);
+  int rc = check_hex_params(*p);
   if (rc) return rc;
-  for (int i = 0; i < nel; ++i) {
+  for (int i = 0; i < nel; ++i)
     for (int k = 0; k < 8; ++k) {
       const int32_t nid = ixs[11 * (nft + i) + 1 + k];
       if (nid < 1 || nid > numnod) return fail(OR_ERR_ARG, "IXS node id out of range");
       const int32_t K = iads[8 * (nft + i) + k];
       if (K < 1 || K > lsky) return fail(OR_ERR_ARG, "IADS row out of range");
     }
-  }
   OR_HIP(hipSetDevice(device));
-  HexConsts c = make_consts(*p);
+  HexConsts c = make_hex_consts(*p);
   const int64_t ncol = nft + nel;
-  double *dx = nullptr, *dv = nullptr, *dbuf = nullptr, *dfsky = nullptr, *ddtx = nullptr;
-  int32_t *dixs = nullptr, *diads = nullptr, *dms = nullptr;
-  int ok = 1;
-  ok &= hipMalloc(&dx, sizeof(double) * 3 * numnod) == hipSuccess;
-  ok &= hipMalloc(&dv, sizeof(double) * 3 * numnod) == hipSuccess;
-  ok &= hipMalloc(&dixs, sizeof(int32_t) * 11 * ncol) == hipSuccess;
-  ok &= hipMalloc(&diads, sizeof(int32_t) * 8 * ncol) == hipSuccess;
-  ok &= hipMalloc(&dbuf, sizeof(double) * 13 * nel) == hipSuccess;
-  ok &= hipMalloc(&dfsky, sizeof(double) * 8 * lsky) == hipSuccess;
-  ok &= hipMalloc(&ddtx, sizeof(double) * nel) == hipSuccess;
-  ok &= hipMalloc(&dms, sizeof(int32_t) * nel) == hipSuccess;
-  auto cleanup = [&]() {
-    void *ps[] = {dx, dv, dixs, diads, dbuf, dfsky, ddtx, dms};
-    for (void *q : ps)
-      if (q) hipFree(q);
-  };
-  if (!ok) { cleanup(); return fail(OR_ERR_HIP, "hipMalloc in or_sforc3_group"); }
-  double *dsig = dbuf, *deint = dbuf + 6 * nel, *drho = dbuf + 7 * nel, *dvol = dbuf + 8 * nel,
-         *dpla = dbuf + 9 * nel, *depsd = dbuf + 10 * nel, *doff = dbuf + 11 * nel, *dq = dbuf + 12 * nel;
-  hipError_t e = hipSuccess;
-  auto H2D = [&](void *d, const void *h, size_t b) { if (e == hipSuccess) e = hipMemcpy(d, h, b, hipMemcpyHostToDevice); };
-  auto D2H = [&](void *h, const void *d, size_t b) { if (e == hipSuccess) e = hipMemcpy(h, d, b, hipMemcpyDeviceToHost); };
-  H2D(dx, x, sizeof(double) * 3 * numnod);
-  H2D(dv, v, sizeof(double) * 3 * numnod);
-  H2D(dixs, ixs, sizeof(int32_t) * 11 * ncol);
-  H2D(diads, iads, sizeof(int32_t) * 8 * ncol);
-  H2D(dsig, sig, sizeof(double) * 6 * nel);
-  H2D(deint, eint, sizeof(double) * nel);
-  H2D(drho, rho, sizeof(double) * nel);
-  H2D(dvol, vol, sizeof(double) * nel);
-  H2D(dpla, pla, sizeof(double) * nel);
-  H2D(depsd, epsd, sizeof(double) * nel);
-  H2D(doff, off, sizeof(double) * nel);
-  H2D(dq, qvis, sizeof(double) * nel);
-  H2D(dfsky, fsky, sizeof(double) * 8 * lsky);
-  if (e == hipSuccess) {
+  Stage s;
+  double *dx = s.up(x, 3 * numnod), *dv = s.up(v, 3 * numnod);
+  int32_t *dixs = s.up(ixs, 11 * ncol), *diads = s.up(iads, 8 * ncol);
+  double *dsig = s.up(sig, 6 * nel), *deint = s.up(eint, nel), *drho = s.up(rho, nel), *dvol = s.up(vol, nel),
+         *dpla = s.up(pla, nel), *depsd = s.up(epsd, nel), *doff = s.up(off, nel), *dq = s.up(qvis, nel),
+         *dfsky = s.up(fsky, 8 * lsky);
+  double *ddtx = s.up<double>(nullptr, nel);
+  int *dms = s.up<int>(nullptr, nel);
+  if (s.e == hipSuccess) {
     hipLaunchKernelGGL(k_sforc3_group, dim3((nel + kBlock - 1) / kBlock), dim3(kBlock), 0, 0, nel, nft, dx, dv, dixs,
                        dsig, deint, drho, dvol, dpla, depsd, doff, dq, dfsky, diads, c, dt1, ddtx, dms);
-    e = hipGetLastError();
+    s.e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (s.e == hipSuccess) s.e = hipDeviceSynchronize();
   std::vector<double> dtx((size_t)nel);
-  std::vector<int32_t> msv((size_t)nel);
-  D2H(sig, dsig, sizeof(double) * 6 * nel);
-  D2H(eint, deint, sizeof(double) * nel);
-  D2H(rho, drho, sizeof(double) * nel);
-  D2H(pla, dpla, sizeof(double) * nel);
-  D2H(epsd, depsd, sizeof(double) * nel);
-  D2H(off, doff, sizeof(double) * nel);
-  D2H(qvis, dq, sizeof(double) * nel);
-  D2H(fsky, dfsky, sizeof(double) * 8 * lsky);
-  D2H(dtx.data(), ddtx, sizeof(double) * nel);
-  D2H(msv.data(), dms, sizeof(int32_t) * nel);
-  cleanup();
-  if (e != hipSuccess) return fail(OR_ERR_HIP, std::string("or_sforc3_group: ") + hipGetErrorString(e));
+  std::vector<int> msv((size_t)nel);
+  s.down(sig, dsig, 6 * nel); s.down(eint, deint, nel); s.down(rho, drho, nel); s.down(pla, dpla, nel);
+  s.down(epsd, depsd, nel); s.down(off, doff, nel); s.down(qvis, dq, nel); s.down(fsky, dfsky, 8 * lsky);
+  s.down(dtx.data(), ddtx, nel); s.down(msv.data(), dms, nel);
+  if (s.e != hipSuccess) return fail(OR_ERR_HIP, std::string("or_sforc3_group: ") + hipGetErrorString(s.e));
   for (int i = 0; i < nel; ++i) {
     if (dtx[i] < *dt2t) *dt2t = dtx[i];
     if (msv[i]) *mstop = 1;
@@ -896,3 +1321,50 @@ int or_sforc3_group(int device, const or_hex_params *p, int64_t numnod, int32_t 
   return OR_OK;
 }
 
+int or_cforc3_group(int device, const or_q4_params *p, int64_t numnod, int32_t nel, int64_t nft, const double *x,
+                    const double *v, const double *vr, const int32_t *ixc, double *offg, double *thk,
+                    const double *thke, double *forc, double *mom, double *gstr, double *hourg, double *eint,
+                    double *epsd, double *sig, double *pla, double *epsd_ip, int64_t lsky, double *fsky,
+                    const int32_t *iadc, double dt1, double *dt2t) {
+  if (!p || !x || !v || !vr || !ixc || !offg || !thk || !thke || !forc || !mom || !gstr || !hourg || !eint ||
+      !epsd || !sig || !pla || !epsd_ip || !fsky || !iadc || !dt2t)
+    return fail(OR_ERR_ARG, "null argument");
+  if (nel <= 0 || nel > 128) return fail(OR_ERR_ARG, "nel must be in 1..128 (MVSIZ-1)");
+  if (numnod <= 0 || lsky <= 0 || nft < 0) return fail(OR_ERR_ARG, "bad sizes");
+  Q4Consts c;
+  int rc = make_q4_consts(*p, c);
+  if (rc) return rc;
+  for (int i = 0; i < nel; ++i)
+    for (int k = 0; k < 4; ++k) {
+      const int32_t nid = ixc[7 * (nft + i) + 1 + k];
+      if (nid < 1 || nid > numnod) return fail(OR_ERR_ARG, "IXC node id out of range");
+      const int32_t K = iadc[4 * (nft + i) + k];
+      if (K < 1 || K > lsky) return fail(OR_ERR_ARG, "IADC row out of range");
+    }
+  OR_HIP(hipSetDevice(device));
+  const int64_t ncol = nft + nel, np = p->npt;
+  Stage s;
+  double *dx = s.up(x, 3 * numnod), *dv = s.up(v, 3 * numnod), *dvr = s.up(vr, 3 * numnod);
+  int32_t *dixc = s.up(ixc, 7 * ncol), *diadc = s.up(iadc, 4 * ncol);
+  double *doffg = s.up(offg, nel), *dthk = s.up(thk, nel), *dthke = s.up(thke, nel), *dforc = s.up(forc, 5 * nel),
+         *dmom = s.up(mom, 3 * nel), *dgstr = s.up(gstr, 8 * nel), *dhourg = s.up(hourg, 5 * nel),
+         *deint = s.up(eint, 2 * nel), *depsd = s.up(epsd, nel), *dsig = s.up(sig, np * 5 * nel),
+         *dpla = s.up(pla, np * nel), *depi = s.up(epsd_ip, np * nel), *dfsky = s.up(fsky, 8 * lsky);
+  double *ddt = s.up<double>(nullptr, nel);
+  if (s.e == hipSuccess) {
+    hipLaunchKernelGGL(k_cforc3_group, dim3((nel + kBlock - 1) / kBlock), dim3(kBlock), 0, 0, nel, nft, dx, dv, dvr,
+                       dixc, doffg, dthk, dthke, dforc, dmom, dgstr, dhourg, deint, depsd, dsig, dpla, depi, dfsky,
+                       diadc, c, dt1, ddt);
+    s.e = hipGetLastError();
+  }
+  if (s.e == hipSuccess) s.e = hipDeviceSynchronize();
+  std::vector<double> dtv((size_t)nel);
+  s.down(offg, doffg, nel); s.down(thk, dthk, nel); s.down(forc, dforc, 5 * nel); s.down(mom, dmom, 3 * nel);
+  s.down(gstr, dgstr, 8 * nel); s.down(hourg, dhourg, 5 * nel); s.down(eint, deint, 2 * nel);
+  s.down(epsd, depsd, nel); s.down(sig, dsig, np * 5 * nel); s.down(pla, dpla, np * nel);
+  s.down(epsd_ip, depi, np * nel); s.down(fsky, dfsky, 8 * lsky); s.down(dtv.data(), ddt, nel);
+  if (s.e != hipSuccess) return fail(OR_ERR_HIP, std::string("or_cforc3_group: ") + hipGetErrorString(s.e));
+  for (int i = 0; i < nel; ++i)
+    if (dtv[i] < *dt2t) *dt2t = dtv[i];
+  return OR_OK;
+}

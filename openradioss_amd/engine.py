@@ -1,10 +1,11 @@
 """Host side of the GPU path: a resident model driven through the C ABI.
 
-``GpuHexModel`` uploads a :class:`openradioss_amd.model.HexModel` once, then
-advances it cycle by cycle on the GPU with all state resident in HBM, in the
-call order of RESOL for this path (resol.F:2721 DT1=DT2 ... 8605 TT+=DT2).
-``sforc3_group`` is the group-level drop-in of SFORC3 with the reference's
-host array layouts.
+``GpuModel`` uploads an :class:`openradioss_amd.model.Model` (hexes and/or Q4
+shells) once, then advances it cycle by cycle on the GPU with all state
+resident in HBM, in the call order of RESOL for this path (resol.F:2721
+DT1=DT2 ... 8605 TT+=DT2).  ``GpuHexModel`` keeps the hex-only interface.
+``sforc3_group`` / ``cforc3_group`` are the group-level drop-ins of SFORC3 and
+CFORC3 with the reference's host array layouts.
 """
 from __future__ import annotations
 
@@ -13,60 +14,77 @@ import ctypes as C
 import numpy as np
 
 from . import lib as L
-from .model import HexModel
+from .model import HexModel, Model, hex_model_to_model
 
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-def hex_params(model: HexModel) -> L.OrHexParams:
-    p = model.params()
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def hex_params(model) -> L.OrHexParams:
+    """or_hex_params of a HexModel or of the hex part of a Model."""
+    p = model.params() if isinstance(model, HexModel) else model.hexp.params(model.dtmx)
     hp = L.OrHexParams()
     for name, _ in L.OrHexParams._fields_:
         setattr(hp, name, p[name])
     return hp
 
 
-def _f64(a):
-    return np.ascontiguousarray(a, dtype=np.float64)
+def q4_params(model: Model) -> L.OrQ4Params:
+    p = model.q4.params()
+    qp = L.OrQ4Params()
+    for name, _ in L.OrQ4Params._fields_:
+        setattr(qp, name, p[name])
+    return qp
 
 
-class GpuHexModel:
-    """Resident hex model on one GPU (precision 'fp64' or 'split')."""
+PRECISIONS = {"fp64": L.OR_PREC_FP64, "split": L.OR_PREC_SPLIT}
 
-    def __init__(self, model: HexModel, precision: str = "fp64", device: int = 0, state: dict | None = None):
+
+class GpuModel:
+    """Resident hex / Q4 model on one GPU (precision 'fp64' or 'split')."""
+
+    def __init__(self, model: Model, precision: str = "fp64", device: int = 0, hex_state: dict | None = None,
+                 q4_state: dict | None = None):
         lib = L.load()
         self.model = model
-        self.prec = {"fp64": L.OR_PREC_FP64, "split": L.OR_PREC_SPLIT}[precision]
+        self.prec = PRECISIONS[precision]
+        npt = model.q4.prop.npt if model.q4 is not None else 0
         h = C.c_void_p()
-        L.check(lib.or_model_create(C.byref(h), device, model.nn, model.ne, self.prec))
+        L.check(lib.or_model_create(C.byref(h), device, model.nn, model.ne_hex, model.ne_q4, npt, self.prec))
         self._h = h
         self._lib = lib
         try:
-            self._params = hex_params(model)
-            L.check(lib.or_model_set_hex_params(h, C.byref(self._params)))
-            x = _f64(model.x)
-            v = _f64(model.v)
-            imp_code = None
-            vimp = None
-            if model.imp_node.shape[0]:
-                imp_code = np.zeros(model.nn, np.int32)
-                vimp = np.zeros((model.nn, 3))
-                bit = np.array([4, 2, 1], np.int32)
-                for n, dof, val in zip(model.imp_node, model.imp_dof, model.imp_val):
-                    imp_code[n] |= bit[dof]
-                    vimp[n, dof] = val
-            L.check(lib.or_model_upload_nodes(h, _p(x), _p(v), _p(_f64(model.ms)),
-                                              _p(np.ascontiguousarray(model.bcs, np.int32)), _p(imp_code),
-                                              _p(vimp)))
-            st = model.initial_state() if state is None else state
-            L.check(lib.or_model_upload_hex(h, _p(np.ascontiguousarray(model.conn, np.int32)), _p(_f64(st["sig"])),
-                                            _p(_f64(st["eint"])), _p(_f64(st["rho"])), _p(_f64(model.vol0)),
-                                            _p(_f64(st["pla"])), _p(_f64(st["epsd"])), _p(_f64(st["offg"])),
-                                            _p(_f64(st["qvis"]))))
-            L.check(lib.or_model_upload_skyline(h, _p(np.ascontiguousarray(model.adsky, np.int64)),
-                                                _p(np.ascontiguousarray(model.iads, np.int64))))
+            if model.hexp is not None:
+                self._hp = hex_params(model)
+                L.check(lib.or_model_set_hex_params(h, C.byref(self._hp)))
+            if model.q4 is not None:
+                self._qp = q4_params(model)
+                L.check(lib.or_model_set_q4_params(h, C.byref(self._qp)))
+            L.check(lib.or_model_upload_nodes(h, _p(_f64(model.x)), _p(_f64(model.v)), _p(_f64(model.vr)),
+                                              _p(_f64(model.ms)), _p(_f64(model.inr)),
+                                              _p(np.ascontiguousarray(model.code, np.int32)),
+                                              _p(_f64(model.vimp))))
+            if model.hexp is not None:
+                st = model.hexp.initial_state() if hex_state is None else hex_state
+                L.check(lib.or_model_upload_hex(h, _p(np.ascontiguousarray(model.hexp.conn, np.int32)),
+                                                _p(_f64(st["sig"])), _p(_f64(st["eint"])), _p(_f64(st["rho"])),
+                                                _p(_f64(model.hexp.vol0)), _p(_f64(st["pla"])), _p(_f64(st["epsd"])),
+                                                _p(_f64(st["offg"])), _p(_f64(st["qvis"]))))
+            if model.q4 is not None:
+                st = model.q4.initial_state() if q4_state is None else q4_state
+                L.check(lib.or_model_upload_q4(h, _p(np.ascontiguousarray(model.q4.conn, np.int32)),
+                                               _p(_f64(model.q4.thke)), *[_p(_f64(st[k])) for k in (
+                                                   "offg", "thk", "forc", "mom", "gstr", "hourg", "eint", "epsd",
+                                                   "sig", "pla", "epsd_ip")]))
+            iads = None if model.hexp is None else np.ascontiguousarray(model.hexp.iads, np.int64)
+            iadc = None if model.q4 is None else np.ascontiguousarray(model.q4.iadc, np.int64)
+            L.check(lib.or_model_upload_skyline(h, _p(np.ascontiguousarray(model.adsky, np.int64)), _p(iads),
+                                                _p(iadc)))
         except Exception:
             self.close()
             raise
@@ -107,16 +125,19 @@ class GpuHexModel:
         L.check(self._lib.or_model_set_time(self._h, C.byref(ts)))
 
     # -- results ---------------------------------------------------------
-    def nodes(self):
+    def nodes6(self):
+        """(x, v, vr, d) as (nn,3) arrays."""
         nn = self.model.nn
-        x = np.empty((nn, 3))
-        v = np.empty((nn, 3))
-        d = np.empty((nn, 3))
-        L.check(self._lib.or_model_download_nodes(self._h, _p(x), _p(v), _p(d), None))
+        x, v, vr, d = (np.empty((nn, 3)) for _ in range(4))
+        L.check(self._lib.or_model_download_nodes(self._h, _p(x), _p(v), _p(vr), _p(d)))
+        return x, v, vr, d
+
+    def nodes(self):
+        x, v, _, d = self.nodes6()
         return x, v, d
 
     def hex_state(self) -> dict:
-        ne = self.model.ne
+        ne = self.model.ne_hex
         out = dict(sig=np.empty((6, ne)), eint=np.empty(ne), rho=np.empty(ne), pla=np.empty(ne),
                    epsd=np.empty(ne), offg=np.empty(ne), qvis=np.empty(ne))
         L.check(self._lib.or_model_download_hex(self._h, _p(out["sig"]), _p(out["eint"]), _p(out["rho"]),
@@ -124,8 +145,18 @@ class GpuHexModel:
                                                 _p(out["qvis"])))
         return out
 
+    def q4_state(self) -> dict:
+        ne, npt = self.model.ne_q4, self.model.q4.prop.npt
+        out = dict(offg=np.empty(ne), thk=np.empty(ne), forc=np.empty((5, ne)), mom=np.empty((3, ne)),
+                   gstr=np.empty((8, ne)), hourg=np.empty((5, ne)), eint=np.empty((2, ne)), epsd=np.empty(ne),
+                   sig=np.empty((npt, 5, ne)), pla=np.empty((npt, ne)), epsd_ip=np.empty((npt, ne)))
+        L.check(self._lib.or_model_download_q4(self._h, *[_p(out[k]) for k in (
+            "offg", "thk", "forc", "mom", "gstr", "hourg", "eint", "epsd", "sig", "pla", "epsd_ip")]))
+        return out
+
     def energy(self) -> np.ndarray:
-        out = np.zeros(2)
+        """[internal, translational kinetic, rotational kinetic]."""
+        out = np.zeros(3)
         L.check(self._lib.or_model_energy(self._h, _p(out)))
         return out
 
@@ -133,17 +164,28 @@ class GpuHexModel:
         L.check(self._lib.or_model_set_timing(self._h, int(bool(enable))))
 
     def kernel_times(self, reset: bool = True):
-        """(element ms, node ms, cycles) averaged per launch since the last reset."""
-        e = C.c_double()
-        n = C.c_double()
-        c = C.c_int64()
-        L.check(self._lib.or_model_kernel_times(self._h, int(reset), C.byref(e), C.byref(n), C.byref(c)))
-        return e.value, n.value, c.value
+        """(hex ms, q4 ms, node ms, cycles) averaged per launch since the last reset."""
+        h, q, n, c = C.c_double(), C.c_double(), C.c_double(), C.c_int64()
+        L.check(self._lib.or_model_kernel_times(self._h, int(reset), C.byref(h), C.byref(q), C.byref(n),
+                                                C.byref(c)))
+        return h.value, q.value, n.value, c.value
 
     def footprint(self) -> int:
         b = C.c_int64()
         L.check(self._lib.or_model_footprint(self._h, C.byref(b)))
         return b.value
+
+
+class GpuHexModel(GpuModel):
+    """Resident hex-only model (a HexModel, with its BCS / imposed-velocity lists)."""
+
+    def __init__(self, model: HexModel, precision: str = "fp64", device: int = 0, state: dict | None = None):
+        self.hex_model = model
+        super().__init__(hex_model_to_model(model), precision=precision, device=device, hex_state=state)
+
+    def energy(self) -> np.ndarray:
+        """[internal, kinetic] as the hex path reports them."""
+        return super().energy()[:2]
 
 
 def sforc3_group(params: L.OrHexParams, x, v, ixs, gbuf: dict, fsky, iads, nft: int, nel: int, dt1: float,
@@ -166,3 +208,25 @@ def sforc3_group(params: L.OrHexParams, x, v, ixs, gbuf: dict, fsky, iads, nft: 
                                 _p(fsky), _p(np.ascontiguousarray(iads, np.int32)), float(dt1), C.byref(d),
                                 C.byref(ms)))
     return d.value, ms.value
+
+
+def cforc3_group(params: L.OrQ4Params, x, v, vr, ixc, gbuf: dict, lbuf: dict, fsky, iadc, nft: int, nel: int,
+                 dt1: float, dt2t: float, device: int = 0) -> float:
+    """CFORC3 on one group (reference layouts; see or_cforc3_group in include/or_gpu.h).
+
+    x, v, vr: (numnod,3); ixc: IXC(7,*) as (ncol,7) int32 1-based; gbuf: GBUF arrays of the group
+    (offg, thk, thke, forc (5,nel), mom (3,nel), gstr (8,nel), hourg (5,nel), eint (2,nel), epsd);
+    lbuf: sig (npt,5,nel), pla (npt,nel), epsd (npt,nel); fsky (lsky,8); iadc (ncol,4) 1-based.
+    Updates gbuf, lbuf and fsky in place; returns dt2t.
+    """
+    lib = L.load()
+    d = C.c_double(dt2t)
+    g = [gbuf[k] for k in ("offg", "thk", "thke", "forc", "mom", "gstr", "hourg", "eint", "epsd")]
+    lb = [lbuf[k] for k in ("sig", "pla", "epsd")]
+    for a in g + lb + [fsky]:
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    L.check(lib.or_cforc3_group(device, C.byref(params), x.shape[0], nel, nft, _p(_f64(x)), _p(_f64(v)),
+                                _p(_f64(vr)), _p(np.ascontiguousarray(ixc, np.int32)), *[_p(a) for a in g],
+                                *[_p(a) for a in lb], fsky.shape[0], _p(fsky),
+                                _p(np.ascontiguousarray(iadc, np.int32)), float(dt1), C.byref(d)))
+    return d.value

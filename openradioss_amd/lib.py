@@ -24,10 +24,11 @@ OR_PREC_SPLIT = 1
 # Every symbol declared in include/or_gpu.h.
 EXPORTS = (
     "or_version", "or_last_error", "or_device_count", "or_model_create", "or_model_destroy",
-    "or_model_set_hex_params", "or_model_upload_nodes", "or_model_upload_hex", "or_model_upload_skyline",
-    "or_model_set_time", "or_model_get_time", "or_model_run", "or_model_sync", "or_model_download_nodes",
-    "or_model_download_hex", "or_model_energy", "or_model_kernel_times", "or_model_set_timing",
-    "or_model_footprint", "or_sforc3_group",
+    "or_model_set_hex_params", "or_model_set_q4_params", "or_model_upload_nodes", "or_model_upload_hex",
+    "or_model_upload_q4", "or_model_upload_skyline", "or_model_set_time", "or_model_get_time", "or_model_run",
+    "or_model_sync", "or_model_download_nodes", "or_model_download_hex", "or_model_download_q4",
+    "or_model_energy", "or_model_kernel_times", "or_model_set_timing", "or_model_footprint", "or_sforc3_group",
+    "or_cforc3_group",
 )
 
 
@@ -36,6 +37,13 @@ class OrHexParams(C.Structure):
         "rho0", "bulk", "shear", "ca", "cb", "cn", "epmx", "sigmx", "cc", "epdr", "fisokin", "asrate",
         "m_exp")] + [(n, C.c_int32) for n in ("icc", "vp", "israte", "iform")] + [
         (n, C.c_double) for n in ("qa", "qb", "qh", "vns1", "vns2", "dtfac", "dtmx")]
+
+
+class OrQ4Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "rho", "young", "nu", "shear", "ssp", "ca", "cb", "cn", "epmx", "ymax", "cc", "epdr", "m_exp",
+        "asrate")] + [(n, C.c_int32) for n in ("icc", "vp", "israte", "iform", "npt", "ipla", "ithk", "ishear")] + [
+        (n, C.c_double) for n in ("hm", "hf", "hr", "fsh", "dtfac", "dtmin")]
 
 
 class OrTimeState(C.Structure):
@@ -56,24 +64,29 @@ _SIGS = {
     "or_version": ([], C.c_int),
     "or_last_error": ([], C.c_char_p),
     "or_device_count": ([_P], C.c_int),
-    "or_model_create": ([_P, C.c_int, C.c_int64, C.c_int64, C.c_int], C.c_int),
+    "or_model_create": ([_P, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_int], C.c_int),
     "or_model_destroy": ([_P], C.c_int),
     "or_model_set_hex_params": ([_P, _P], C.c_int),
-    "or_model_upload_nodes": ([_P, _P, _P, _P, _P, _P, _P], C.c_int),
+    "or_model_set_q4_params": ([_P, _P], C.c_int),
+    "or_model_upload_nodes": ([_P] * 8, C.c_int),
     "or_model_upload_hex": ([_P] * 10, C.c_int),
-    "or_model_upload_skyline": ([_P, _P, _P], C.c_int),
+    "or_model_upload_q4": ([_P] * 14, C.c_int),
+    "or_model_upload_skyline": ([_P] * 4, C.c_int),
     "or_model_set_time": ([_P, _P], C.c_int),
     "or_model_get_time": ([_P, _P], C.c_int),
     "or_model_run": ([_P, C.c_int64], C.c_int),
     "or_model_sync": ([_P], C.c_int),
-    "or_model_download_nodes": ([_P, _P, _P, _P, _P], C.c_int),
+    "or_model_download_nodes": ([_P] * 5, C.c_int),
     "or_model_download_hex": ([_P] * 8, C.c_int),
+    "or_model_download_q4": ([_P] * 12, C.c_int),
     "or_model_energy": ([_P, _P], C.c_int),
-    "or_model_kernel_times": ([_P, C.c_int, _P, _P, _P], C.c_int),
+    "or_model_kernel_times": ([_P, C.c_int, _P, _P, _P, _P], C.c_int),
     "or_model_set_timing": ([_P, C.c_int], C.c_int),
     "or_model_footprint": ([_P, _P], C.c_int),
     "or_sforc3_group": ([C.c_int, _P, C.c_int64, C.c_int32, C.c_int64] + [_P] * 11 + [C.c_int64, _P, _P,
                                                                                       C.c_double, _P, _P], C.c_int),
+    "or_cforc3_group": ([C.c_int, _P, C.c_int64, C.c_int32, C.c_int64] + [_P] * 16 + [C.c_int64, _P, _P,
+                                                                                      C.c_double, _P], C.c_int),
 }
 
 

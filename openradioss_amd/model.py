@@ -580,3 +580,36 @@ def tie_q4_on_hex(xh, ch, xp, cp, rng, v0: float = -5.0e3, noise: float = 300.0,
     code = np.where(x[:, 2] == 0.0, BC_ALL, 0).astype(np.int32)
     return build_model(x, hex_conn=ch, hex_mat=STEEL, q4_conn=cq, q4_mat=STEEL, q4_prop=ShellProp(thick=thick),
                        v0=v, code=code)
+
+
+def block_with_plate(n: int = 160, plate: int = 1000, h: float = 1.0, thick: float = 2.0, seed: int = 20261203,
+                     v0: float = 1.0e4, sigma: float = 500.0, npt: int = 5) -> Model:
+    """The GPU-filling roofline model: an n^3 hex block (edge h) with an plate x plate Q4 plate
+    (edge h, thickness thick) lying on its top face, the block's top nodes shared with the plate.
+    Law-2 steel for both; initial velocity v0 along z plus N(0, sigma) per component (seeded)."""
+    xh, ch = hex_block(n, n, n, h)
+    nhex_nodes = xh.shape[0]
+    off = (plate - n) // 2
+    i, j = np.meshgrid(np.arange(plate + 1), np.arange(plate + 1), indexing="xy")
+    i, j = i.reshape(-1), j.reshape(-1)
+    bi, bj = i - off, j - off
+    shared = (bi >= 0) & (bi <= n) & (bj >= 0) & (bj <= n)
+    pid = np.empty(i.shape[0], dtype=np.int64)
+    # block node (ix,iy,iz) -> ix + (n+1)*(iy + (n+1)*iz); top face iz = n
+    pid[shared] = bi[shared] + (n + 1) * (bj[shared] + (n + 1) * n)
+    nnew = int((~shared).sum())
+    pid[~shared] = nhex_nodes + np.arange(nnew)
+    xp = np.zeros((nnew, 3))
+    xp[:, 0] = (i[~shared] - off) * h
+    xp[:, 1] = (j[~shared] - off) * h
+    xp[:, 2] = n * h
+    x = np.vstack([xh, xp])
+    ei, ej = np.meshgrid(np.arange(plate), np.arange(plate), indexing="xy")
+    ei, ej = ei.reshape(-1), ej.reshape(-1)
+    g = lambda a, b: pid[a + (plate + 1) * b]  # noqa: E731
+    cq = np.stack([g(ei, ej), g(ei + 1, ej), g(ei + 1, ej + 1), g(ei, ej + 1)], axis=1).astype(np.int32)
+    rng = np.random.default_rng(seed)
+    v = sigma * rng.standard_normal(x.shape)
+    v[:, 2] += v0
+    return build_model(x, hex_conn=ch, hex_mat=STEEL, q4_conn=cq, q4_mat=STEEL,
+                       q4_prop=ShellProp(thick=thick, npt=npt), v0=v)

@@ -1,11 +1,11 @@
-"""Algorithmic HBM traffic of the path, implemented once (used by bench.py and DESIGN.md).
+"""Algorithmic HBM traffic of the path, implemented once (used by bench.py).
 
 Word counts per element-update follow the compulsory-traffic model of the
-explicit hex cycle: element state read once and written once, connectivity
-read once, node data gathered once per node per cycle, and the nodal update
-reading and writing each node once.  The deterministic two-phase assembly
-adds the skyline rows (written by the element kernel, read by the node
-kernel); it is reported separately as the "as implemented" extra.
+explicit cycle: element state read once and written once, connectivity read
+once, node data gathered once per node per cycle, and the nodal update reading
+and writing each node once.  The deterministic two-phase assembly adds the
+skyline rows (written by the element kernels, read by the node kernel) and the
+row indices; "as_implemented" includes them.
 """
 from __future__ import annotations
 
@@ -13,34 +13,74 @@ HEX_STATE_READ = 13   # SIG(6), EINT, RHO, VOL0, PLA, EPSD, OFF, QVIS
 HEX_STATE_WRITE = 12  # the same minus VOL0
 HEX_CONN_BYTES = 8 * 4
 HEX_IADS_BYTES = 8 * 4
-NODE_GATHER_BYTES = 6 * 8    # X(3), V(3) in FP64
-NODE_UPDATE_READ = 6 * 8 + 3 * 8 + 8 + 4 + 4  # X,V + D + MS + BC code + ADSKY
-NODE_UPDATE_WRITE = 6 * 8 + 3 * 8             # X,V + D
+
+# Q4: GBUF OFF, THK, THKE, FOR(5), MOM(3), STRA(8), HOURG(5), EINT(2), EPSD + per point SIG(5), PLA
+Q4_GBUF_READ = 1 + 1 + 1 + 5 + 3 + 8 + 5 + 2 + 1
+Q4_GBUF_WRITE = Q4_GBUF_READ - 1      # THKE is read only
+Q4_LBUF_WORDS = 5 + 1                 # per point, read and written
+Q4_CONN_BYTES = 4 * 4
+Q4_IADC_BYTES = 4 * 4
+
+NODE_GATHER_BYTES = 6 * 8      # X(3), V(3) in FP64
+NODE_GATHER_ROT_BYTES = 3 * 8  # VR(3)
 
 
 def state_word(precision: str) -> int:
     return 8 if precision == "fp64" else 4
 
 
-def row_bytes(precision: str) -> int:
-    return 4 * state_word(precision)  # (FX, FY, FZ, STI/4)
-
-
 def hex_element_kernel_bytes(ne: int, nn: int, precision: str) -> dict:
-    """Bytes one launch of the element kernel must move (per launch, for ne elements)."""
+    """Bytes one launch of the hex element kernel must move (ne elements, nn nodes gathered)."""
     w = state_word(precision)
     compulsory = ne * ((HEX_STATE_READ + HEX_STATE_WRITE) * w + HEX_CONN_BYTES) + nn * NODE_GATHER_BYTES
-    skyline = ne * (HEX_IADS_BYTES + 8 * row_bytes(precision))
-    return {"compulsory": compulsory, "as_implemented": compulsory + skyline}
+    rows = ne * (HEX_IADS_BYTES + 8 * 4 * w)
+    return {"compulsory": compulsory, "as_implemented": compulsory + rows}
 
 
-def node_kernel_bytes(ne: int, nn: int, precision: str) -> dict:
-    compulsory = nn * (NODE_UPDATE_READ + NODE_UPDATE_WRITE)
-    skyline = 8 * ne * row_bytes(precision)
-    return {"compulsory": compulsory, "as_implemented": compulsory + skyline}
+def q4_element_kernel_bytes(ne: int, nn: int, npt: int, precision: str) -> dict:
+    """Bytes one launch of the Q4 element kernel must move (ne elements, nn nodes gathered)."""
+    w = state_word(precision)
+    words = Q4_GBUF_READ + Q4_GBUF_WRITE + 2 * npt * Q4_LBUF_WORDS
+    compulsory = ne * (words * w + Q4_CONN_BYTES) + nn * (NODE_GATHER_BYTES + NODE_GATHER_ROT_BYTES)
+    rows = ne * (Q4_IADC_BYTES + 4 * 8 * w)
+    return {"compulsory": compulsory, "as_implemented": compulsory + rows}
 
 
-def cycle_bytes_per_element_update(ne: int, nn: int, precision: str) -> dict:
+def node_kernel_bytes(nn: int, rows_hex: int, rows_q4: int, rotational: bool, precision: str) -> dict:
+    """Bytes of one launch of the node kernel: X,V,D read and written, MS, code, and with
+    rotations VR read and written and IN read; as implemented, the skyline rows and pointers."""
+    per = (6 + 3) * 8 * 2 + 8 + 4
+    if rotational:
+        per += 3 * 8 * 2 + 8
+    compulsory = nn * per
+    w = state_word(precision)
+    rows = rows_hex * 4 * w + rows_q4 * 8 * w + nn * 4 * ((rows_hex > 0) + (rows_q4 > 0) + 1)
+    return {"compulsory": compulsory, "as_implemented": compulsory + rows}
+
+
+def cycle_bytes(model, precision: str) -> dict:
+    """Per-cycle bytes of every kernel of a Model, and the total."""
+    out = {}
+    nn = model.nn
+    if model.hexp is not None:
+        nh = nn_touched(model.hexp.conn, nn)
+        out["hex"] = hex_element_kernel_bytes(model.ne_hex, nh, precision)
+    if model.q4 is not None:
+        nq = nn_touched(model.q4.conn, nn)
+        out["q4"] = q4_element_kernel_bytes(model.ne_q4, nq, model.q4.prop.npt, precision)
+    out["node"] = node_kernel_bytes(nn, 8 * model.ne_hex, 4 * model.ne_q4, model.q4 is not None, precision)
+    out["total"] = {k: sum(v[k] for v in out.values()) for k in ("compulsory", "as_implemented")}
+    return out
+
+
+def nn_touched(conn, nn: int) -> int:
+    import numpy as np
+    seen = np.zeros(nn, dtype=bool)
+    seen[conn.reshape(-1)] = True
+    return int(seen.sum())
+
+
+def hex_bytes_per_element_update(ne: int, nn: int, precision: str) -> dict:
     e = hex_element_kernel_bytes(ne, nn, precision)
-    n = node_kernel_bytes(ne, nn, precision)
+    n = node_kernel_bytes(nn, 8 * ne, 0, False, precision)
     return {k: (e[k] + n[k]) / ne for k in ("compulsory", "as_implemented")}

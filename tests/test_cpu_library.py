@@ -29,7 +29,7 @@ def test_library_exports_every_symbol():
 
 def test_null_arguments_are_rejected():
     lib = L.load()
-    assert lib.or_model_create(None, 0, 10, 1, 0) == L.OR_ERR_ARG
+    assert lib.or_model_create(None, 0, 10, 1, 0, 0, 0) == L.OR_ERR_ARG
     assert lib.or_device_count(None) == L.OR_ERR_ARG
     assert b"null" in lib.or_last_error()
 
@@ -37,9 +37,11 @@ def test_null_arguments_are_rejected():
 def test_bad_sizes_are_rejected_before_any_device_call():
     lib = L.load()
     h = C.c_void_p()
-    assert lib.or_model_create(C.byref(h), 0, 0, 1, 0) == L.OR_ERR_ARG
-    assert lib.or_model_create(C.byref(h), 0, 10, 1, 7) == L.OR_ERR_ARG
-    assert lib.or_model_create(C.byref(h), 0, 10, 1 << 29, 0) == L.OR_ERR_ARG
+    assert lib.or_model_create(C.byref(h), 0, 0, 1, 0, 0, 0) == L.OR_ERR_ARG        # no nodes
+    assert lib.or_model_create(C.byref(h), 0, 10, 1, 0, 0, 7) == L.OR_ERR_ARG       # bad precision
+    assert lib.or_model_create(C.byref(h), 0, 10, 1 << 29, 0, 0, 0) == L.OR_ERR_ARG  # rows beyond int32
+    assert lib.or_model_create(C.byref(h), 0, 10, 0, 5, 1, 0) == L.OR_ERR_ARG       # NPT=1 not on this path
+    assert lib.or_model_create(C.byref(h), 0, 10, 0, 5, 11, 0) == L.OR_ERR_ARG      # NPT > 10
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
@@ -47,3 +49,27 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(L, "_lib", None)
     with pytest.raises(ImportError):
         L.load()
+
+
+def test_group_dropins_reject_bad_input_before_any_device_call():
+    import numpy as np
+    from openradioss_amd import engine as E
+    from openradioss_amd import model as M
+    lib = L.load()
+    m = M.square_tube(width=4.0, length=2.0)
+    qp = E.q4_params(m)
+    dt = C.c_double(1.0)
+    z = np.zeros(8)
+    # nel out of 1..128
+    assert lib.or_cforc3_group(0, C.byref(qp), 10, 0, 0, *([z.ctypes.data_as(C.c_void_p)] * 16), 8,
+                               z.ctypes.data_as(C.c_void_p), z.ctypes.data_as(C.c_void_p), 1.0,
+                               C.byref(dt)) == L.OR_ERR_ARG
+    qp.npt = 1
+    ixc = np.array([[0, 1, 2, 3, 4, 1, 1]], np.int32)
+    iadc = np.array([[1, 2, 3, 4]], np.int32)
+    arr = [np.zeros(64) for _ in range(16)]
+    arr[3] = ixc
+    args = [a.ctypes.data_as(C.c_void_p) for a in arr]
+    assert lib.or_cforc3_group(0, C.byref(qp), 4, 1, 0, *args, 4, np.zeros(32).ctypes.data_as(C.c_void_p),
+                               iadc.ctypes.data_as(C.c_void_p), 1.0, C.byref(dt)) == L.OR_ERR_ARG
+    assert b"NPT" in lib.or_last_error()
