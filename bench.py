@@ -85,6 +85,18 @@ def cpu_baseline(model, budget_s: float = 12.0):
                        f"128) on the same {ne}-element model, {n} cycles after 1 warm-up cycle, {el:.1f} s")
 
 
+def copy_probe(mib: int):
+    """Three plain device-to-device copies of a known size: each reads and writes mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 copies of {mib} MiB")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,7 +106,12 @@ def main():
     ap.add_argument("--plate", type=int, default=1000, help="Q4 plate edge in elements (1000 = roofline model)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision form")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; runs on several GPUs are not part of it",
                           "n_gpus": args.gpus}))

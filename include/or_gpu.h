@@ -174,6 +174,12 @@ int or_cforc3_group(int device, const or_q4_params *p, int64_t numnod, int32_t n
                     double *eint, double *epsd, double *sig, double *pla, double *epsd_ip, int64_t lsky,
                     double *fsky, const int32_t *iadc, double dt1, double *dt2t);
 
+/* Diagnostic, no counterpart in the reference: evaluates on the device the FP64
+ * elementary functions the element kernels use in place of the host libm's
+ * (fn 0: log(x), 1: exp(x), 2: pow(x, y)), for n arguments, 0 <= n <= 2^30.
+ * They are correctly rounded except within ~2^-17 ulp of a rounding boundary. */
+int or_math_eval(int device, int fn, int64_t n, const double *x, const double *y, double *out);
+
 #ifdef __cplusplus
 }
 #endif

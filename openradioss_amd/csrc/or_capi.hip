@@ -1368,3 +1368,34 @@ int or_cforc3_group(int device, const or_q4_params *p, int64_t numnod, int32_t n
     if (dtv[i] < *dt2t) *dt2t = dtv[i];
   return OR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Device evaluation of the FP64 elementary functions (diagnostic).
+namespace {
+__global__ void k_math_eval(int fn, int64_t n, const double *__restrict__ x, const double *__restrict__ y,
+                            double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = x[i];
+  out[i] = fn == 0 ? ormath::or_log(a) : fn == 1 ? ormath::or_exp(a) : ormath::or_pow(a, y[i]);
+}
+} // namespace
+
+int or_math_eval(int device, int fn, int64_t n, const double *x, const double *y, double *out) {
+  if (fn < 0 || fn > 2) return fail(OR_ERR_ARG, "fn must be 0 (log), 1 (exp) or 2 (pow)");
+  if (n < 0 || n > (int64_t(1) << 30)) return fail(OR_ERR_ARG, "n must be in 0..2^30");
+  if (n == 0) return OR_OK;
+  if (!x || !out || (fn == 2 && !y)) return fail(OR_ERR_ARG, "null argument");
+  OR_HIP(hipSetDevice(device));
+  Stage s;
+  double *dx = s.up(x, n), *dy = fn == 2 ? s.up(y, n) : nullptr, *dout = s.up<double>(nullptr, n);
+  if (s.e == hipSuccess) {
+    hipLaunchKernelGGL(k_math_eval, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0, fn, n, dx, dy,
+                       dout);
+    s.e = hipGetLastError();
+  }
+  if (s.e == hipSuccess) s.e = hipDeviceSynchronize();
+  s.down(out, dout, n);
+  if (s.e != hipSuccess) return fail(OR_ERR_HIP, std::string("or_math_eval: ") + hipGetErrorString(s.e));
+  return OR_OK;
+}

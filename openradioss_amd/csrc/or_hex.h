@@ -3,12 +3,15 @@
 // One thread owns one element.  The arithmetic of every step follows the
 // reference routine it names, operation for operation, so that with
 // -ffp-contract=off the FP64 instantiation rounds exactly like the reference
-// CPU engine (pow() is the only libm routine whose last bit can differ).
+// CPU engine (pow() is evaluated in double-double and correctly rounded except in
+// rare near-tie cases, see or_math.h).
 // The split instantiation (R = float) keeps coordinates and their differences
 // in FP64 and runs strain, stress and hourglass arithmetic in FP32.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "or_math.h"
 
 namespace orgpu {
 
@@ -36,11 +39,13 @@ struct DevTime {
 template <typename T> __device__ __forceinline__ T dmaxT(T a, T b) { return (a > b) ? a : b; }
 template <typename T> __device__ __forceinline__ T dminT(T a, T b) { return (a < b) ? a : b; }
 
-__device__ __forceinline__ double powR(double a, double b) { return pow(a, b); }
+// FP64: the double-double routines of or_math.h, which round like the host C library
+// the reference engine calls; FP32 (split form): the device library.
+__device__ __forceinline__ double powR(double a, double b) { return ormath::or_pow(a, b); }
 __device__ __forceinline__ float powR(float a, float b) { return powf(a, b); }
 __device__ __forceinline__ double sqrtR(double a) { return sqrt(a); }
 __device__ __forceinline__ float sqrtR(float a) { return sqrtf(a); }
-__device__ __forceinline__ double logR(double a) { return log(a); }
+__device__ __forceinline__ double logR(double a) { return ormath::or_log(a); }
 __device__ __forceinline__ float logR(float a) { return logf(a); }
 __device__ __forceinline__ double fabsR(double a) { return fabs(a); }
 __device__ __forceinline__ float fabsR(float a) { return fabsf(a); }

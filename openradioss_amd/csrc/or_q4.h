@@ -2,8 +2,9 @@
 //
 // One thread owns one element.  Each step follows the reference routine it
 // names operation for operation, so that with -ffp-contract=off the FP64
-// instantiation rounds like the reference CPU engine (exp/log of the hardening
-// law are the only library routines whose last bit can differ).  The split
+// instantiation rounds like the reference CPU engine (exp/log of the hardening law
+// are evaluated in double-double and correctly rounded except in rare near-tie
+// cases, see or_math.h).  The split
 // instantiation (R = float) computes the convected frame and local coordinates
 // from FP64 coordinates, then runs strain, material and hourglass arithmetic in
 // FP32 with FP32 element state.
@@ -38,7 +39,7 @@ template <typename R> struct Q4Out {
   double dt;    // element time step, or +EP06 when excluded from the minimum
 };
 
-__device__ __forceinline__ double expR(double a) { return exp(a); }
+__device__ __forceinline__ double expR(double a) { return ormath::or_exp(a); }
 __device__ __forceinline__ float expR(float a) { return expf(a); }
 template <typename T> __device__ __forceinline__ T fsignT(T a, T b) { return copysign(fabs(a), b); }
 

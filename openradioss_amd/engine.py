@@ -230,3 +230,14 @@ def cforc3_group(params: L.OrQ4Params, x, v, vr, ixc, gbuf: dict, lbuf: dict, fs
                                 *[_p(a) for a in lb], fsky.shape[0], _p(fsky),
                                 _p(np.ascontiguousarray(iadc, np.int32)), float(dt1), C.byref(d)))
     return d.value
+
+
+def math_eval(fn: str, x, y=None, device: int = 0) -> np.ndarray:
+    """Device values of the FP64 elementary functions of the element kernels (or_math_eval):
+    fn 'log' / 'exp' of x, or 'pow' of (x, y)."""
+    code = {"log": 0, "exp": 1, "pow": 2}[fn]
+    x = _f64(x).ravel()
+    yy = None if y is None else _f64(np.broadcast_to(y, x.shape)).ravel()
+    out = np.empty_like(x)
+    L.check(L.load().or_math_eval(device, code, x.size, _p(x), _p(yy), _p(out)))
+    return out

@@ -28,7 +28,7 @@ EXPORTS = (
     "or_model_upload_q4", "or_model_upload_skyline", "or_model_set_time", "or_model_get_time", "or_model_run",
     "or_model_sync", "or_model_download_nodes", "or_model_download_hex", "or_model_download_q4",
     "or_model_energy", "or_model_kernel_times", "or_model_set_timing", "or_model_footprint", "or_sforc3_group",
-    "or_cforc3_group",
+    "or_cforc3_group", "or_math_eval",
 )
 
 
@@ -87,6 +87,7 @@ _SIGS = {
                                                                                       C.c_double, _P, _P], C.c_int),
     "or_cforc3_group": ([C.c_int, _P, C.c_int64, C.c_int32, C.c_int64] + [_P] * 16 + [C.c_int64, _P, _P,
                                                                                       C.c_double, _P], C.c_int),
+    "or_math_eval": ([C.c_int, C.c_int, C.c_int64, _P, _P, _P], C.c_int),
 }
 
 

@@ -20,6 +20,26 @@
 extern "C" {
 #endif
 
+/* Elementary functions of the hardening laws and the bulk viscosity.
+ * Default build: the host C library, as the reference engine calls it (X**Y,
+ * LOG, EXP of the Fortran sources).  Built with -DOR_CR_MATH: correctly
+ * rounded pow / log / exp (or_crmath.cpp), i.e. the reference algorithm under
+ * a correctly rounded libm.  The two builds differ only where the host
+ * library's result is not the correctly rounded one (about 1 call in 1000 for
+ * glibc 2.35 exp and pow). */
+#ifdef OR_CR_MATH
+double or_cr_pow(double x, double y);
+double or_cr_log(double x);
+double or_cr_exp(double x);
+#define OR_POW or_cr_pow
+#define OR_LOG or_cr_log
+#define OR_EXP or_cr_exp
+#else
+#define OR_POW pow
+#define OR_LOG log
+#define OR_EXP exp
+#endif
+
 /* Material (law 2, Johnson-Cook form) + solid property + run control. */
 typedef struct {
   double rho0;     /* mat_param%rho        (hm_read_mat02_jc.F90: pm(1)=rhor) */

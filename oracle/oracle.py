@@ -12,18 +12,38 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
 _SRCS = [os.path.join(_HERE, "or_oracle.c"), os.path.join(_HERE, "or_shell.c")]
 _HDR = os.path.join(_HERE, "or_oracle.h")
+_CR_SRC = os.path.join(_HERE, "or_crmath.cpp")
+_CR_DEPS = [os.path.join(_HERE, "..", "openradioss_amd", "csrc", f) for f in ("or_math.h", "or_math_tables.h")]
+# "libm": pow/log/exp of the host C library, as the reference engine calls them (pinned bitwise to
+# the reference's compiled routines).  "cr": the same restatement under correctly rounded
+# pow/log/exp, the elementary functions of the FP64 device path.
+MATH_MODES = ("libm", "cr")
+_LIBS = {"libm": os.path.join(_HERE, "liboracle.so"), "cr": os.path.join(_HERE, "liboracle_cr.so")}
+_LIB = _LIBS["libm"]
 
 
-def build(force: bool = False) -> str:
+def _stale(out, deps):
+    return not os.path.exists(out) or any(os.path.getmtime(out) < os.path.getmtime(f) for f in deps)
+
+
+def build(force: bool = False, math: str = "libm") -> str:
     """Compile the oracle with contraction off (matching the reference release flags)."""
-    stale = not os.path.exists(_LIB) or any(os.path.getmtime(_LIB) < os.path.getmtime(f) for f in _SRCS + [_HDR])
-    if force or stale:
-        subprocess.check_call(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-                               "-fopenmp", "-Wall", *_SRCS, "-o", _LIB, "-lm"])
-    return _LIB
+    out = _LIBS[math]
+    cflags = ["-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-Wall"]
+    if math == "libm":
+        if force or _stale(out, _SRCS + [_HDR]):
+            subprocess.check_call(["gcc", *cflags, "-shared", *_SRCS, "-o", out, "-lm"])
+        return out
+    if force or _stale(out, _SRCS + [_HDR, _CR_SRC] + _CR_DEPS):
+        tmp = out + ".tmp.o"
+        subprocess.check_call(["g++", *cflags, "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                               "-c", _CR_SRC, "-o", tmp])
+        subprocess.check_call(["gcc", *cflags, "-DOR_CR_MATH", "-shared", *_SRCS, tmp, "-o", out, "-lm",
+                               "-lstdc++"])
+        os.remove(tmp)
+    return out
 
 
 class HexParams(C.Structure):
@@ -38,21 +58,20 @@ class TimeState(C.Structure):
                 ("dt2old", C.c_double), ("ncycle", C.c_int64), ("mstop", C.c_int)]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        build()
-        _lib = C.CDLL(_LIB)
-        _lib.or_const.restype = C.c_double
-        _lib.or_const.argtypes = [C.c_int]
-        _lib.or_shell_const.restype = C.c_double
-        _lib.or_shell_const.argtypes = [C.c_int]
-        _lib.or_shell_table.restype = C.c_double
-        _lib.or_shell_table.argtypes = [C.c_int, C.c_int, C.c_int]
-    return _lib
+def lib(math: str = "libm"):
+    if math not in _libs:
+        L = C.CDLL(build(math=math))
+        L.or_const.restype = C.c_double
+        L.or_const.argtypes = [C.c_int]
+        L.or_shell_const.restype = C.c_double
+        L.or_shell_const.argtypes = [C.c_int]
+        L.or_shell_table.restype = C.c_double
+        L.or_shell_table.argtypes = [C.c_int, C.c_int, C.c_int]
+        _libs[math] = L
+    return _libs[math]
 
 
 def _p(a):
@@ -83,8 +102,9 @@ def initial_time_state() -> TimeState:
 class HexOracle:
     """Runs the hex path on the CPU with the reference's array layouts."""
 
-    def __init__(self, model, nthreads: int = 1, track_hourglass: bool = False):
+    def __init__(self, model, nthreads: int = 1, track_hourglass: bool = False, math: str = "libm"):
         self.m = model
+        self.math = math
         st = model.initial_state()
         self.x = np.ascontiguousarray(model.x, dtype=np.float64).copy()
         self.v = np.ascontiguousarray(model.v, dtype=np.float64).copy()
@@ -114,7 +134,7 @@ class HexOracle:
         self.nthreads = nthreads
 
     def cycle(self, n: int = 1):
-        L = lib()
+        L = lib(self.math)
         for _ in range(n):
             L.or_cycle_hex(C.c_int64(self.m.nn), C.c_int64(self.m.ne), C.c_int64(self.m.lsky),
                            _p(self.conn), _p(self.x), _p(self.v), _p(self.a), _p(self.d), _p(self.ms),
@@ -126,7 +146,7 @@ class HexOracle:
 
     def energy(self):
         out = np.zeros(2)
-        lib().or_energy(C.c_int64(self.m.nn), C.c_int64(self.m.ne), _p(self.eint), _p(self.vol0), _p(self.ms),
+        lib(self.math).or_energy(C.c_int64(self.m.nn), C.c_int64(self.m.ne), _p(self.eint), _p(self.vol0), _p(self.ms),
                         _p(self.v), _p(out))
         return out
 
@@ -177,8 +197,9 @@ def _pd(a):
 class ModelOracle:
     """Runs a general openradioss_amd.model.Model on the CPU (hex and Q4 chains)."""
 
-    def __init__(self, model, nthreads: int = 1):
+    def __init__(self, model, nthreads: int = 1, math: str = "libm"):
         m = self.m = model
+        self.math = math
         nn = m.nn
         self.x = m.x.copy()
         self.v = m.v.copy()
@@ -242,11 +263,11 @@ class ModelOracle:
         self.ts = initial_time_state()
 
     def cycle(self, n: int = 1):
-        L = lib()
+        L = lib(self.math)
         for _ in range(n):
             L.or_cycle_model(C.byref(self.c), C.byref(self.ts), C.c_int(self.nthreads))
 
     def energy(self):
         out = np.zeros(3)
-        lib().or_energy_model(C.byref(self.c), _pd(out))
+        lib(self.math).or_energy_model(C.byref(self.c), _pd(out))
         return out

@@ -1,12 +1,28 @@
-"""GPU parity of the hex path: HIP kernels (through the C ABI) against the C oracle."""
+"""GPU parity of the hex path: HIP kernels (through the C ABI) against the C oracle.
+
+FP64: the kernels follow the reference's operation order with contraction off and
+evaluate pow / log / exp correctly rounded (openradioss_amd/csrc/or_math.h).  They
+are compared bit for bit with the oracle built under the same correctly rounded
+functions (math="cr"), and against the reference's own golden vectors (glibc
+libm) within the spread that the choice of libm alone causes in the reference
+algorithm (measured on these cases between the two oracle builds: up to 6.6e-12
+relative in stress, 2.4e-12 in velocity).
+"""
+import os
+
 import numpy as np
 import pytest
 
 import oracle as O
+from golden_cases import CASES, build_case
 from openradioss_amd import engine as E
 from openradioss_amd import model as M
 
 pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+# libm spread of the reference algorithm on the golden cases (glibc 2.35 vs correctly rounded)
+LIBM_SPREAD_TOL = 1e-11
 
 
 def rel_err(a, b):
@@ -14,48 +30,74 @@ def rel_err(a, b):
     return np.abs(a - b).max() / scale
 
 
-def run_pair(model, ncycles, precision="fp64"):
-    o = O.HexOracle(model, nthreads=8)
+def run_pair(model, ncycles, precision="fp64", math="cr"):
+    o = O.HexOracle(model, nthreads=8, math=math)
     g = E.GpuHexModel(model, precision=precision)
     o.cycle(ncycles)
     g.run(ncycles)
     return o, g
 
 
-def test_fp64_bar_matches_oracle(gpu_lib):
-    print("case: cantilever bar 6x5x12, 300 cycles, fp64", flush=True)
+def report(tag, a, b):
+    print(f"{tag}: bitwise-equal fraction {np.mean(a == b):.4f}, max rel err {rel_err(a, b):.3e}", flush=True)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_matches_reference_golden(name, gpu_lib):
+    """GPU FP64 against the vectors written by the reference's compiled SFORC3 chain."""
+    c = CASES[name]
+    print(f"case: {name}, {c['cycles']} cycles, fp64 vs reference golden", flush=True)
+    gold = np.load(os.path.join(HERE, "golden", name + ".npz"))
+    m = build_case(c)
+    g = E.GpuHexModel(m)
+    g.run(c["cycles"])
+    ts = g.time_state()
+    assert ts.ncycle == int(gold["ncycle"][0]) and ts.mstop == 0
+    assert ts.tt == pytest.approx(gold["time"][0], rel=1e-13)
+    x, v, d = g.nodes()
+    st = g.hex_state()
+    report("d", d, gold["d"])
+    assert rel_err(d, gold["d"]) <= LIBM_SPREAD_TOL
+    assert rel_err(v, gold["v"]) <= LIBM_SPREAD_TOL
+    assert rel_err(st["sig"], gold["sig"]) <= LIBM_SPREAD_TOL
+    assert rel_err(st["pla"], gold["pla"]) <= LIBM_SPREAD_TOL
+    assert np.array_equal(st["offg"], gold["offg"])
+    g.close()
+
+
+def test_fp64_bar_bitwise_vs_cr_oracle(gpu_lib):
+    print("case: cantilever bar 6x5x12, 300 cycles, fp64 vs oracle(cr)", flush=True)
     m = M.cantilever_bar(6, 5, 12)
     o, g = run_pair(m, 300)
     ts = g.time_state()
     assert ts.ncycle == o.ts.ncycle == 300
     assert ts.mstop == 0
-    assert ts.dt2 == pytest.approx(o.ts.dt2, rel=1e-13)
-    assert ts.tt == pytest.approx(o.ts.tt, rel=1e-13)
     x, v, d = g.nodes()
     st = g.hex_state()
-    # tolerance of the plain FP64 method: 1e-12 relative (pow() last-ulp differences only)
-    assert rel_err(d, o.d) <= 1e-12
-    assert rel_err(v, o.v) <= 1e-12
-    assert rel_err(st["sig"], o.sig) <= 1e-12
-    assert rel_err(st["pla"], o.pla) <= 1e-12
-    e_gpu = g.energy()
-    e_ora = o.energy()
-    assert abs(e_gpu.sum() - e_ora.sum()) / e_ora.sum() <= 1e-12
-    frac_exact = np.mean(d == o.d)
-    print(f"bitwise-equal displacement fraction {frac_exact:.4f}; max rel err {rel_err(d, o.d):.3e}")
+    report("d", d, o.d)
+    report("sig", st["sig"], o.sig)
+    assert (ts.tt, ts.dt2) == (o.ts.tt, o.ts.dt2)
+    assert np.array_equal(d, o.d)
+    assert np.array_equal(v, o.v)
+    assert np.array_equal(st["sig"], o.sig)
+    assert np.array_equal(st["pla"], o.pla)
+    # energies are sums over all elements / nodes in a different order: not bitwise
+    assert np.abs(g.energy() - o.energy()).max() <= 1e-13 * np.abs(o.energy()).max()
     g.close()
 
 
-def test_fp64_plastic_impact_bitwise_or_ulp(gpu_lib):
+def test_fp64_plastic_impact_bitwise_vs_cr_oracle(gpu_lib):
     """Plastic regime (high impact velocity) exercises the pow() branch of M2LAW every cycle."""
-    print("case: bar 4x4x16 at 60 m/s, 400 cycles, fp64", flush=True)
+    print("case: bar 4x4x16 at 60 m/s, 400 cycles, fp64 vs oracle(cr)", flush=True)
     m = M.cantilever_bar(4, 4, 16, v0=6.0e4)
     o, g = run_pair(m, 400)
     x, v, d = g.nodes()
     st = g.hex_state()
     assert o.pla.max() > 1e-3
-    assert rel_err(d, o.d) <= 1e-12
-    assert rel_err(st["pla"], o.pla) <= 1e-11
+    report("d", d, o.d)
+    report("pla", st["pla"], o.pla)
+    assert np.array_equal(d, o.d)
+    assert np.array_equal(st["pla"], o.pla)
     assert np.array_equal(st["offg"], o.offg)
     g.close()
 
@@ -63,7 +105,7 @@ def test_fp64_plastic_impact_bitwise_or_ulp(gpu_lib):
 def test_split_precision_error_bound(gpu_lib):
     print("case: cantilever bar 10x10x40, 500 cycles, split", flush=True)
     m = M.cantilever_bar(10, 10, 40)
-    o, g = run_pair(m, 500, precision="split")
+    o, g = run_pair(m, 500, precision="split", math="libm")
     x, v, d = g.nodes()
     err = rel_err(d, o.d)
     e_gpu, e_ora = g.energy(), o.energy()
@@ -85,7 +127,8 @@ def test_imposed_velocity_and_fixed_end(gpu_lib):
     x1, v1, d1 = g.nodes()
     assert np.all(v1[top, 2] == -1.5e4)
     assert np.all(v1[bcs == 7] == 0.0)
-    assert rel_err(d1, o.d) <= 1e-12
+    report("d", d1, o.d)
+    assert np.array_equal(d1, o.d)
     g.close()
 
 
@@ -99,14 +142,14 @@ def test_sforc3_group_dropin_matches_oracle(gpu_lib):
     v = 3e3 * rng.standard_normal((m.nn, 3))
     sig = 200.0 * rng.standard_normal((6, m.ne))
     pla = np.abs(1e-3 * rng.standard_normal(m.ne))
-    o = O.HexOracle(m)
+    o = O.HexOracle(m, math="cr")
     o.v[:] = v
     o.sig[:] = sig
     o.pla[:] = pla
     dt1 = 1.2e-7
     dt2t = np.array([1.0e6])
     ms = np.zeros(1, np.int32)
-    O.lib().or_hex_forces(O.C.c_int64(0), O.C.c_int64(m.ne), O.C.c_int64(m.ne), O._p(o.conn), O._p(o.x),
+    O.lib("cr").or_hex_forces(O.C.c_int64(0), O.C.c_int64(m.ne), O.C.c_int64(m.ne), O._p(o.conn), O._p(o.x),
                           O._p(o.v), O._p(o.sig), O._p(o.eint), O._p(o.rho), O._p(o.vol0), O._p(o.pla),
                           O._p(o.epsd), O._p(o.offg), O._p(o.qvis), None, O._p(o.iads), O._p(o.fsky),
                           O.C.byref(o.params), O.C.c_double(dt1), O._p(dt2t), O._p(ms))
@@ -119,8 +162,8 @@ def test_sforc3_group_dropin_matches_oracle(gpu_lib):
     fsky = np.zeros((m.lsky, 8))
     dt2t_g, mstop = E.sforc3_group(E.hex_params(m), x, v, ixs, gb, fsky, iads, nft=0, nel=m.ne, dt1=dt1, dt2t=1e6)
     assert mstop == 0
-    assert dt2t_g == pytest.approx(dt2t[0], rel=1e-15)
-    assert rel_err(fsky, o.fsky) <= 1e-12
-    assert rel_err(gb["sig"], o.sig) <= 1e-12
-    assert rel_err(gb["eint"], o.eint) <= 1e-12
-    print(f"group: fsky bitwise fraction {np.mean(fsky == o.fsky):.4f}")
+    report("fsky", fsky, o.fsky)
+    assert dt2t_g == dt2t[0]
+    assert np.array_equal(fsky, o.fsky)
+    assert np.array_equal(gb["sig"], o.sig)
+    assert np.array_equal(gb["eint"], o.eint)
