@@ -1,0 +1,260 @@
+! or_gpu_mod.F90 -- Fortran ISO_C_BINDING interface of the MI355X path (include/or_gpu.h).
+!
+! The Engine side of the boundary: RESOL / FORINT / FORINTC call these in place
+! of SFORC3 (forint.F:1159), CFORC3 (forintc.F:452), or, for a whole resident
+! model, in place of the element and nodal sequence of one cycle.  Every
+! function returns an integer status (OR_OK = 0); OR_ERROR_MESSAGE() returns the
+! text of the last failure of the calling thread.  Arrays are the Engine's own:
+! X(3,NUMNOD), V(3,NUMNOD), VR(3,NUMNOD), IXS(NIXS,*), IXC(NIXC,*), GBUF / LBUF
+! component-major arrays of one group, FSKY(8,LSKY), IADS(8,*), IADC(4,*).
+module or_gpu_mod
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+  public :: or_hex_params_t, or_q4_params_t, or_time_state_t
+  public :: or_version, or_device_count, or_error_message
+  public :: or_model_create, or_model_destroy, or_model_set_hex_params, or_model_set_q4_params
+  public :: or_model_upload_nodes, or_model_upload_hex, or_model_upload_q4, or_model_upload_skyline
+  public :: or_model_set_time, or_model_get_time, or_model_run, or_model_sync
+  public :: or_model_download_nodes, or_model_download_hex, or_model_download_q4
+  public :: or_model_energy, or_model_kernel_times, or_model_set_timing, or_model_footprint
+  public :: or_sforc3_group, or_cforc3_group, or_math_eval
+
+  integer(c_int), parameter, public :: OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_HIP = 2, OR_ERR_NODEV = 3, &
+                                       OR_ERR_STATE = 4, OR_ERR_MSTOP = 5
+  integer(c_int), parameter, public :: OR_PREC_FP64 = 0, OR_PREC_SPLIT = 1
+  integer(c_int32_t), parameter, public :: OR_BC_TX = 4, OR_BC_TY = 2, OR_BC_TZ = 1, OR_BC_RX = 32, &
+                                           OR_BC_RY = 16, OR_BC_RZ = 8, OR_IMP_X = 256, OR_IMP_Y = 128, &
+                                           OR_IMP_Z = 64
+
+  ! law 2 + /PROP/SOLID Isolid=1 + run control (or_hex_params)
+  type, bind(c) :: or_hex_params_t
+    real(c_double) :: rho0, bulk, shear
+    real(c_double) :: ca, cb, cn, epmx, sigmx, cc, epdr, fisokin, asrate, m_exp
+    integer(c_int32_t) :: icc, vp, israte, iform
+    real(c_double) :: qa, qb, qh, vns1, vns2
+    real(c_double) :: dtfac, dtmx
+  end type
+
+  ! law 2 for shells + /PROP/SHELL Ishell=1 (or_q4_params)
+  type, bind(c) :: or_q4_params_t
+    real(c_double) :: rho, young, nu, shear, ssp
+    real(c_double) :: ca, cb, cn, epmx, ymax, cc, epdr, m_exp
+    real(c_double) :: asrate
+    integer(c_int32_t) :: icc, vp, israte, iform
+    integer(c_int32_t) :: npt, ipla, ithk, ishear
+    real(c_double) :: hm, hf, hr, fsh, dtfac, dtmin
+  end type
+
+  ! TT, DT1, DT2, DT12, DT2OLD, NCYCLE, MSTOP (or_time_state)
+  type, bind(c) :: or_time_state_t
+    real(c_double) :: tt, dt1, dt2, dt12, dt2old
+    integer(c_int64_t) :: ncycle
+    integer(c_int32_t) :: mstop
+  end type
+
+  interface
+    integer(c_int) function or_version() bind(c, name="or_version")
+      import :: c_int
+    end function
+
+    type(c_ptr) function or_last_error_c() bind(c, name="or_last_error")
+      import :: c_ptr
+    end function
+
+    integer(c_int) function or_device_count(count) bind(c, name="or_device_count")
+      import :: c_int
+      integer(c_int), intent(out) :: count
+    end function
+
+    integer(c_int) function or_model_create(handle, device, nn, ne_hex, ne_q4, npt, precision) &
+        bind(c, name="or_model_create")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), intent(out) :: handle
+      integer(c_int), value :: device, precision
+      integer(c_int64_t), value :: nn, ne_hex, ne_q4
+      integer(c_int32_t), value :: npt
+    end function
+
+    integer(c_int) function or_model_destroy(handle) bind(c, name="or_model_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+    end function
+
+    integer(c_int) function or_model_set_hex_params(handle, p) bind(c, name="or_model_set_hex_params")
+      import :: c_int, c_ptr, or_hex_params_t
+      type(c_ptr), value :: handle
+      type(or_hex_params_t), intent(in) :: p
+    end function
+
+    integer(c_int) function or_model_set_q4_params(handle, p) bind(c, name="or_model_set_q4_params")
+      import :: c_int, c_ptr, or_q4_params_t
+      type(c_ptr), value :: handle
+      type(or_q4_params_t), intent(in) :: p
+    end function
+
+    ! optional arrays (vr, in, code, vimp) may be passed as c_null_ptr through the c_ptr forms
+    integer(c_int) function or_model_upload_nodes(handle, x, v, vr, ms, in, code, vimp) &
+        bind(c, name="or_model_upload_nodes")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle, x, v, vr, ms, in, code, vimp
+    end function
+
+    integer(c_int) function or_model_upload_hex(handle, conn, sig, eint, rho, vol0, pla, epsd, offg, qvis) &
+        bind(c, name="or_model_upload_hex")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle, conn, sig, eint, rho, vol0, pla, epsd, offg, qvis
+    end function
+
+    integer(c_int) function or_model_upload_q4(handle, conn, thke, offg, thk, forc, mom, gstr, hourg, eint, &
+        epsd, sig, pla, epsd_ip) bind(c, name="or_model_upload_q4")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle, conn, thke, offg, thk, forc, mom, gstr, hourg, eint, epsd, sig, pla, epsd_ip
+    end function
+
+    integer(c_int) function or_model_upload_skyline(handle, adsky, iads, iadc) &
+        bind(c, name="or_model_upload_skyline")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle, adsky, iads, iadc
+    end function
+
+    integer(c_int) function or_model_set_time(handle, ts) bind(c, name="or_model_set_time")
+      import :: c_int, c_ptr, or_time_state_t
+      type(c_ptr), value :: handle
+      type(or_time_state_t), intent(in) :: ts
+    end function
+
+    integer(c_int) function or_model_get_time(handle, ts) bind(c, name="or_model_get_time")
+      import :: c_int, c_ptr, or_time_state_t
+      type(c_ptr), value :: handle
+      type(or_time_state_t), intent(out) :: ts
+    end function
+
+    integer(c_int) function or_model_run(handle, ncycles) bind(c, name="or_model_run")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int64_t), value :: ncycles
+    end function
+
+    integer(c_int) function or_model_sync(handle) bind(c, name="or_model_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+    end function
+
+    integer(c_int) function or_model_download_nodes(handle, x, v, vr, d) bind(c, name="or_model_download_nodes")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle, x, v, vr, d
+    end function
+
+    integer(c_int) function or_model_download_hex(handle, sig, eint, rho, pla, epsd, offg, qvis) &
+        bind(c, name="or_model_download_hex")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle, sig, eint, rho, pla, epsd, offg, qvis
+    end function
+
+    integer(c_int) function or_model_download_q4(handle, offg, thk, forc, mom, gstr, hourg, eint, epsd, sig, &
+        pla, epsd_ip) bind(c, name="or_model_download_q4")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle, offg, thk, forc, mom, gstr, hourg, eint, epsd, sig, pla, epsd_ip
+    end function
+
+    integer(c_int) function or_model_energy(handle, e) bind(c, name="or_model_energy")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: e(3)
+    end function
+
+    integer(c_int) function or_model_kernel_times(handle, reset, hex_ms, q4_ms, node_ms, cycles) &
+        bind(c, name="or_model_kernel_times")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: handle
+      integer(c_int), value :: reset
+      real(c_double), intent(out) :: hex_ms, q4_ms, node_ms
+      integer(c_int64_t), intent(out) :: cycles
+    end function
+
+    integer(c_int) function or_model_set_timing(handle, enable) bind(c, name="or_model_set_timing")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: enable
+    end function
+
+    integer(c_int) function or_model_footprint(handle, nbytes) bind(c, name="or_model_footprint")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: handle
+      integer(c_int64_t), intent(out) :: nbytes
+    end function
+
+    ! SFORC3 on one group (sforc3.F:131-166 argument meaning): GBUF%SIG, EINT, RHO, VOL, PLA,
+    ! EPSD, OFF, QVIS of the group; FSKY rows written like SCUMU3P; DT2T min-folded.
+    integer(c_int) function or_sforc3_group(device, p, numnod, nel, nft, x, v, ixs, sig, eint, rho, vol, pla, &
+        epsd, off, qvis, lsky, fsky, iads, dt1, dt2t, mstop) bind(c, name="or_sforc3_group")
+      import :: c_int, c_int32_t, c_int64_t, c_double, or_hex_params_t
+      integer(c_int), value :: device
+      type(or_hex_params_t), intent(in) :: p
+      integer(c_int64_t), value :: numnod, nft, lsky
+      integer(c_int32_t), value :: nel
+      real(c_double), intent(in) :: x(3, *), v(3, *), vol(*)
+      integer(c_int32_t), intent(in) :: ixs(11, *), iads(8, *)
+      real(c_double), intent(inout) :: sig(*), eint(*), rho(*), pla(*), epsd(*), off(*), qvis(*)
+      real(c_double), intent(inout) :: fsky(8, *)
+      real(c_double), value :: dt1
+      real(c_double), intent(inout) :: dt2t
+      integer(c_int32_t), intent(inout) :: mstop
+    end function
+
+    ! CFORC3 on one group (cforc3.F:85-113 argument meaning): GBUF%OFF, THK, FOR, MOM, STRA,
+    ! HOURG, EINT, EPSD; LBUF SIG / PLA / EPSD per point; FSKY rows written like CUPDT3P.
+    integer(c_int) function or_cforc3_group(device, p, numnod, nel, nft, x, v, vr, ixc, offg, thk, thke, forc, &
+        mom, gstr, hourg, eint, epsd, sig, pla, epsd_ip, lsky, fsky, iadc, dt1, dt2t) &
+        bind(c, name="or_cforc3_group")
+      import :: c_int, c_int32_t, c_int64_t, c_double, or_q4_params_t
+      integer(c_int), value :: device
+      type(or_q4_params_t), intent(in) :: p
+      integer(c_int64_t), value :: numnod, nft, lsky
+      integer(c_int32_t), value :: nel
+      real(c_double), intent(in) :: x(3, *), v(3, *), vr(3, *), thke(*)
+      integer(c_int32_t), intent(in) :: ixc(7, *), iadc(4, *)
+      real(c_double), intent(inout) :: offg(*), thk(*), forc(*), mom(*), gstr(*), hourg(*), eint(*), epsd(*)
+      real(c_double), intent(inout) :: sig(*), pla(*), epsd_ip(*)
+      real(c_double), intent(inout) :: fsky(8, *)
+      real(c_double), value :: dt1
+      real(c_double), intent(inout) :: dt2t
+    end function
+
+    integer(c_int) function or_math_eval(device, fn, n, x, y, out) bind(c, name="or_math_eval")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: device, fn
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: x(*), y(*)
+      real(c_double), intent(out) :: out(*)
+    end function
+
+    integer(c_size_t) function c_strlen(s) bind(c, name="strlen")
+      import :: c_size_t, c_ptr
+      type(c_ptr), value :: s
+    end function
+  end interface
+
+contains
+
+  ! Text of the last failure of the calling thread.
+  function or_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: i, n
+    p = or_last_error_c()
+    if (.not. c_associated(p)) then
+      msg = ""
+      return
+    end if
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate (character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = chars(i)
+    end do
+  end function
+
+end module or_gpu_mod
