@@ -11,6 +11,7 @@
 //                  DEPLA; thread 0 advances the cycle scalars (double-buffered by parity)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -42,11 +43,56 @@ int fail(int code, const std::string &msg) {
 
 constexpr int kBlock = 256;
 
+// Minimum waves per SIMD the element kernels are compiled for (register budget
+// 512 / waves per lane, VGPRs and AGPRs together).
+#ifndef OR_HEX_WAVES_FP64
+#define OR_HEX_WAVES_FP64 1
+#endif
+#ifndef OR_HEX_WAVES_FP32
+#define OR_HEX_WAVES_FP32 2
+#endif
+#ifndef OR_Q4_WAVES_FP64
+#define OR_Q4_WAVES_FP64 1
+#endif
+#ifndef OR_Q4_WAVES_FP32
+#define OR_Q4_WAVES_FP32 2
+#endif
+template <typename R> constexpr int hex_waves() { return sizeof(R) == 8 ? OR_HEX_WAVES_FP64 : OR_HEX_WAVES_FP32; }
+template <typename R> constexpr int q4_waves() { return sizeof(R) == 8 ? OR_Q4_WAVES_FP64 : OR_Q4_WAVES_FP32; }
+
 // node code bits (include/or_gpu.h)
 constexpr int32_t kTX = 4, kTY = 2, kTZ = 1, kRX = 32, kRY = 16, kRZ = 8, kIX = 256, kIY = 128, kIZ = 64;
 
-template <typename S> struct alignas(4 * sizeof(S)) Row4 { S fx, fy, fz, sti; };
-template <typename S> struct alignas(4 * sizeof(S)) Row8 { S fx, fy, fz, mx, my, mz, sti, stir; };
+// Skyline rows of the resident model hold the force (and moment) words of FSKY.
+// The stiffness words (FSKY 7, 8) have no reader on this path (element time
+// step, NODADT=0, no interfaces), so the resident rows leave them out; the group
+// drop-ins write them.
+//
+// Layout (sliced ELL with component planes): nodes are taken in slices of kBlock,
+// the node kernel's block.  Slice b owns slots slab[b] .. slab[b] + D_b - 1, D_b the
+// largest row count of its nodes; row j of node n is slot slab[n / kBlock] + j in
+// lane n % kBlock, and its component c is word (slot * W + c) * kBlock + lane
+// (W = 3 words for hex rows, 6 for shell rows).  The node kernel then reads
+// row j of 64 consecutive nodes as 64 consecutive words per component, and
+// neighbouring elements write neighbouring words.  Element kernels hold each of
+// their rows as the position slot * kBlock + lane.
+constexpr int kHexRowWords = 3; // fx, fy, fz
+constexpr int kQ4RowWords = 6;  // fx, fy, fz, mx, my, mz
+__device__ __forceinline__ int64_t row_word(int32_t pos, int W) {
+  return ((int64_t)(pos / kBlock) * W) * kBlock + (pos % kBlock);
+}
+
+// Workgroups of a launch are dispatched to the 8 XCDs in turn (workgroup b runs on
+// XCD b mod 8), each XCD with its own L2.  Element kernels renumber their workgroups
+// so that each XCD works through one contiguous eighth of the element range:
+// neighbouring elements, which share nodes, then gather through the same L2.
+// The map is a bijection of [0, nb).
+__device__ __forceinline__ int64_t xcd_contiguous_block(int64_t b, int64_t nb) {
+  constexpr int64_t kXcd = 8;
+  const int64_t q = nb / kXcd, r = nb % kXcd;
+  const int64_t x = b % kXcd, j = b / kXcd;
+  return x * q + (x < r ? x : r) + j;
+}
 
 template <typename S> struct HexPtrs {
   const int32_t *conn;  // [8][ne]
@@ -75,7 +121,8 @@ struct NodePtrs {
   const double *in;      // [nn] or null
   const int32_t *code;   // [nn]
   const double *vimp;    // [nn][3]
-  const int32_t *adsky_h, *adsky_q; // [nn+1] each (null when the family is absent)
+  const int32_t *deg_h, *deg_q;    // [nn] rows of each node per family (null when the family is absent)
+  const int32_t *slab_h, *slab_q;  // [ceil(nn / kBlock)] first slot of each node slice
   int64_t nn;
 };
 
@@ -107,10 +154,10 @@ __device__ __forceinline__ void load_node(const double *xv, int64_t n, double &x
 
 // ---------------------------------------------------------------- hex element kernel
 template <typename R, typename S>
-__global__ __launch_bounds__(kBlock) void k_hex_forces(HexPtrs<S> e, const double *__restrict__ xv,
-                                                       Row4<S> *__restrict__ sky, HexConsts c, const DevTime *ts,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(hex_waves<R>()))) void k_hex_forces(HexPtrs<S> e, const double *__restrict__ xv,
+                                                       S *__restrict__ sky, HexConsts c, const DevTime *ts,
                                                        int parity, unsigned long long *dtmin, int *flags) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = xcd_contiguous_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   double dtx = kEP06;
   int mstop = 0;
   if (i < e.ne) {
@@ -137,9 +184,8 @@ __global__ __launch_bounds__(kBlock) void k_hex_forces(HexPtrs<S> e, const doubl
     e.epsd[i] = st.epsd; e.offg[i] = st.offg; e.qvis[i] = st.qvis;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      Row4<S> r;
-      r.fx = S(o.f1[k]); r.fy = S(o.f2[k]); r.fz = S(o.f3[k]); r.sti = S(o.sti4);
-      sky[e.iads[(int64_t)k * e.ne + i]] = r;
+      S *r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
+      r[0] = S(o.f1[k]); r[kBlock] = S(o.f2[k]); r[2 * kBlock] = S(o.f3[k]);
     }
     dtx = o.dtx;
     mstop = o.mstop;
@@ -170,11 +216,11 @@ template <typename R, typename S> struct Q4IpIO {
 };
 
 template <typename R, typename S>
-__global__ __launch_bounds__(kBlock) void k_q4_forces(Q4Ptrs<S> q, const double *__restrict__ xv,
-                                                      const double *__restrict__ vr, Row8<S> *__restrict__ sky,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(q4_waves<R>()))) void k_q4_forces(Q4Ptrs<S> q, const double *__restrict__ xv,
+                                                      const double *__restrict__ vr, S *__restrict__ sky,
                                                       Q4Consts c, const DevTime *ts, int parity,
                                                       unsigned long long *dtmin) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = xcd_contiguous_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   double dt = kEP06;
   if (i < q.ne) {
     const R DT1 = R(ts[parity].dt2);
@@ -218,11 +264,9 @@ __global__ __launch_bounds__(kBlock) void k_q4_forces(Q4Ptrs<S> q, const double 
     q.eint[i] = S(st.eint[0]); q.eint[q.ne + i] = S(st.eint[1]);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      Row8<S> r;
-      r.fx = S(-o.f[0][k]); r.fy = S(-o.f[1][k]); r.fz = S(-o.f[2][k]);
-      r.mx = S(-o.m[0][k]); r.my = S(-o.m[1][k]); r.mz = S(-o.m[2][k]);
-      r.sti = S(o.sti); r.stir = S(0);
-      sky[q.iadc[(int64_t)k * q.ne + i]] = r;
+      S *r = sky + row_word(q.iadc[(int64_t)k * q.ne + i], kQ4RowWords);
+      r[0] = S(-o.f[0][k]); r[kBlock] = S(-o.f[1][k]); r[2 * kBlock] = S(-o.f[2][k]);
+      r[3 * kBlock] = S(-o.m[0][k]); r[4 * kBlock] = S(-o.m[1][k]); r[5 * kBlock] = S(-o.m[2][k]);
     }
     dt = o.dt;
   }
@@ -232,8 +276,8 @@ __global__ __launch_bounds__(kBlock) void k_q4_forces(Q4Ptrs<S> q, const double 
 
 // ---------------------------------------------------------------- node kernel
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_node_update(NodePtrs nd, const Row4<S> *__restrict__ skh,
-                                                        const Row8<S> *__restrict__ skq, DevTime *ts, int parity,
+__global__ __launch_bounds__(kBlock) void k_node_update(NodePtrs nd, const S *__restrict__ skh,
+                                                        const S *__restrict__ skq, DevTime *ts, int parity,
                                                         unsigned long long *dtmin, const int *flags,
                                                         double dt_growth, double dtmx) {
   const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -265,19 +309,22 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodePtrs nd, const Row4<
   if (n >= nd.nn) return;
   // ASSPAR4: rows of this node in skyline order (solids, then shells), from zero
   double ax = 0.0, ay = 0.0, az = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
+  const int lane = (int)(n % kBlock);
   if (skh) {
-    const int32_t k0 = nd.adsky_h[n], k1 = nd.adsky_h[n + 1];
-    for (int32_t k = k0; k < k1; ++k) {
-      const Row4<S> r = skh[k];
-      ax = ax + double(r.fx); ay = ay + double(r.fy); az = az + double(r.fz);
+    const int32_t deg = nd.deg_h[n];
+    const S *r = skh + (int64_t)nd.slab_h[n / kBlock] * kHexRowWords * kBlock + lane;
+    for (int32_t j = 0; j < deg; ++j, r += kHexRowWords * kBlock) {
+      ax = ax + double(r[0]); ay = ay + double(r[kBlock]); az = az + double(r[2 * kBlock]);
     }
   }
+  bool rot = false; // node carries rotational DOF work this cycle
   if (skq) {
-    const int32_t k0 = nd.adsky_q[n], k1 = nd.adsky_q[n + 1];
-    for (int32_t k = k0; k < k1; ++k) {
-      const Row8<S> r = skq[k];
-      ax = ax + double(r.fx); ay = ay + double(r.fy); az = az + double(r.fz);
-      rx = rx + double(r.mx); ry = ry + double(r.my); rz = rz + double(r.mz);
+    const int32_t deg = nd.deg_q[n];
+    rot = deg > 0;
+    const S *r = skq + (int64_t)nd.slab_q[n / kBlock] * kQ4RowWords * kBlock + lane;
+    for (int32_t j = 0; j < deg; ++j, r += kQ4RowWords * kBlock) {
+      ax = ax + double(r[0]); ay = ay + double(r[kBlock]); az = az + double(r[2 * kBlock]);
+      rx = rx + double(r[3 * kBlock]); ry = ry + double(r[4 * kBlock]); rz = rz + double(r[5 * kBlock]);
     }
   }
   // ACCELE (accele.F:65-80, 97-109)
@@ -288,9 +335,13 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodePtrs nd, const Row4<
   } else {
     ax = 0.0; ay = 0.0; az = 0.0;
   }
+  const int32_t code = nd.code[n];
+  // A node without shell rows and without rotational BCs has AR = 0, so VELOCITY leaves
+  // its VR as it is: its VR and IN are not touched.
+  rot = rot || (code & (kRX | kRY | kRZ));
   double wx = 0.0, wy = 0.0, wz = 0.0;
   double2 *pr = nullptr;
-  if (nd.vr) {
+  if (nd.vr && rot) {
     const double inn = nd.in[n];
     if (inn > 0.0) {
       const double rtmp = 1.0 / inn;
@@ -305,7 +356,6 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodePtrs nd, const Row4<
   double2 *p = reinterpret_cast<double2 *>(nd.xv + 6 * n);
   double2 pa = p[0], pb = p[1], pc = p[2];
   double x = pa.x, y = pa.y, z = pb.x, vx = pb.y, vy = pc.x, vz = pc.y;
-  const int32_t code = nd.code[n];
   if (code) {
     // BCS1V global frame (bcs1.F)
     if (code & kTX) { ax = 0.0; vx = 0.0; }
@@ -631,7 +681,8 @@ struct or_model {
   Q4Consts qc{};
   // nodes
   double *xv = nullptr, *vr = nullptr, *d = nullptr, *ms = nullptr, *in = nullptr, *vimp = nullptr;
-  int32_t *code = nullptr, *adsky_h = nullptr, *adsky_q = nullptr;
+  int32_t *code = nullptr, *deg_h = nullptr, *deg_q = nullptr, *slab_h = nullptr, *slab_q = nullptr;
+  int64_t sky_bytes = 0;               // skyline rows, sized at upload_skyline
   // hex (storage type depends on precision)
   int32_t *hconn = nullptr, *hiads = nullptr;
   void *hsig = nullptr, *heint = nullptr, *hrho = nullptr, *hvol0 = nullptr, *hpla = nullptr, *hepsd = nullptr,
@@ -712,16 +763,17 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int6
     A(&m->in, nn);
   }
   if (ne_hex > 0) {
-    A(&m->adsky_h, nn + 1);
+    A(&m->deg_h, nn);
+    A(&m->slab_h, (nn + kBlock - 1) / kBlock);
     A(&m->hconn, 8 * ne_hex);
     A(&m->hiads, 8 * ne_hex);
     AS(&m->hsig, 6 * ne_hex);
     AS(&m->heint, ne_hex); AS(&m->hrho, ne_hex); AS(&m->hvol0, ne_hex); AS(&m->hpla, ne_hex);
     AS(&m->hepsd, ne_hex); AS(&m->hoffg, ne_hex); AS(&m->hqvis, ne_hex);
-    AS(&m->skh, 4 * m->lsky_h);
   }
   if (ne_q4 > 0) {
-    A(&m->adsky_q, nn + 1);
+    A(&m->deg_q, nn);
+    A(&m->slab_q, (nn + kBlock - 1) / kBlock);
     A(&m->qconn, 4 * ne_q4);
     A(&m->qiadc, 4 * ne_q4);
     AS(&m->qoffg, ne_q4); AS(&m->qthk, ne_q4); AS(&m->qthke, ne_q4); AS(&m->qepsd, ne_q4);
@@ -729,7 +781,6 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int6
     AS(&m->qeint, 2 * ne_q4);
     AS(&m->qsig, (int64_t)npt * 5 * ne_q4);
     AS(&m->qpla, (int64_t)npt * ne_q4);
-    AS(&m->skq, 8 * m->lsky_q);
   }
   A(&m->ts, 2);
   A(&m->dtmin, 2);
@@ -771,7 +822,7 @@ int or_model_destroy(or_model *m) {
   if (!m) return OR_OK;
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
-  void *ptrs[] = {m->xv, m->vr, m->d, m->ms, m->in, m->vimp, m->code, m->adsky_h, m->adsky_q, m->hconn, m->hiads,
+  void *ptrs[] = {m->xv, m->vr, m->d, m->ms, m->in, m->vimp, m->code, m->deg_h, m->deg_q, m->slab_h, m->slab_q, m->hconn, m->hiads,
                   m->hsig, m->heint, m->hrho, m->hvol0, m->hpla, m->hepsd, m->hoffg, m->hqvis, m->skh, m->qconn,
                   m->qiadc, m->qoffg, m->qthk, m->qthke, m->qforc, m->qmom, m->qgstr, m->qhourg, m->qeint, m->qepsd,
                   m->qsig, m->qpla, m->qepsdip, m->skq, m->ts, m->dtmin, m->flags, m->energy};
@@ -964,8 +1015,6 @@ int or_model_upload_skyline(or_model *m, const int64_t *adsky, const int64_t *ia
     for (int k = 0; k < 8; ++k) {
       const int64_t K = iads[8 * e + k];
       if ((rc = claim(K, 0))) return rc;
-      // the hex node must own the row
-      std::vector<int32_t> dummy;
       nh[(size_t)owner[(size_t)K]] += 1;
     }
   for (int64_t e = 0; e < neq; ++e)
@@ -979,32 +1028,60 @@ int or_model_upload_skyline(or_model *m, const int64_t *adsky, const int64_t *ia
     for (int64_t k = adsky[n]; k < adsky[n + 1]; ++k)
       if ((k - adsky[n] < nh[(size_t)n]) != (fam[(size_t)k] == 0))
         return fail(OR_ERR_ARG, "skyline rows of a node must list solid rows before shell rows (FILLCNE order)");
-  std::vector<int32_t> ah((size_t)(nn + 1), 0), aq((size_t)(nn + 1), 0);
-  for (int64_t n = 0; n < nn; ++n) {
-    ah[(size_t)(n + 1)] = ah[(size_t)n] + nh[(size_t)n];
-    aq[(size_t)(n + 1)] = aq[(size_t)n] + nq[(size_t)n];
+  // sliced-ELL slots: slice b of kBlock nodes gets max(rows of its nodes) slots per family
+  const int64_t nslice = (nn + kBlock - 1) / kBlock;
+  std::vector<int32_t> slab_h((size_t)nslice, 0), slab_q((size_t)nslice, 0);
+  int64_t th = 0, tq = 0;
+  for (int64_t b = 0; b < nslice; ++b) {
+    int32_t mh = 0, mq = 0;
+    for (int64_t n = b * kBlock; n < std::min(nn, (b + 1) * kBlock); ++n) {
+      mh = std::max(mh, nh[(size_t)n]);
+      mq = std::max(mq, nq[(size_t)n]);
+    }
+    slab_h[(size_t)b] = (int32_t)th;
+    slab_q[(size_t)b] = (int32_t)tq;
+    th += mh;
+    tq += mq;
   }
-  auto local = [&](int64_t K) -> int32_t {
+  if (th * kBlock >= (int64_t(1) << 31) || tq * kBlock >= (int64_t(1) << 31))
+    return fail(OR_ERR_ARG, "skyline too large for 32-bit row positions");
+  auto position = [&](int64_t K) -> int32_t {
     const int32_t n = owner[(size_t)K];
-    const int64_t j = K - adsky[n];
-    return fam[(size_t)K] == 0 ? (int32_t)(ah[(size_t)n] + j) : (int32_t)(aq[(size_t)n] + (j - nh[(size_t)n]));
+    const int64_t j = fam[(size_t)K] == 0 ? K - adsky[n] : K - adsky[n] - nh[(size_t)n];
+    const int64_t slot = (fam[(size_t)K] == 0 ? slab_h : slab_q)[(size_t)(n / kBlock)] + j;
+    return (int32_t)(slot * kBlock + n % kBlock);
   };
+  // (re)allocate the rows
+  const size_t ss = ssize(m);
+  if (m->skh) { (void)hipFree(m->skh); m->skh = nullptr; }
+  if (m->skq) { (void)hipFree(m->skq); m->skq = nullptr; }
+  m->bytes -= m->sky_bytes;
+  m->sky_bytes = 0;
   if (neh > 0) {
+    const size_t nb = ss * (size_t)(th * kHexRowWords * kBlock);
+    OR_HIP(hipMalloc(&m->skh, nb ? nb : ss));
+    m->sky_bytes += (int64_t)nb;
     std::vector<int32_t> soa((size_t)(8 * neh));
     for (int64_t e = 0; e < neh; ++e)
-      for (int k = 0; k < 8; ++k) soa[(size_t)(k * neh + e)] = local(iads[8 * e + k]);
-    OR_HIP(hipMemcpyAsync(m->adsky_h, ah.data(), sizeof(int32_t) * (nn + 1), hipMemcpyHostToDevice, m->stream));
+      for (int k = 0; k < 8; ++k) soa[(size_t)(k * neh + e)] = position(iads[8 * e + k]);
+    OR_HIP(hipMemcpyAsync(m->deg_h, nh.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, m->stream));
+    OR_HIP(hipMemcpyAsync(m->slab_h, slab_h.data(), sizeof(int32_t) * nslice, hipMemcpyHostToDevice, m->stream));
     OR_HIP(hipMemcpyAsync(m->hiads, soa.data(), sizeof(int32_t) * 8 * neh, hipMemcpyHostToDevice, m->stream));
     OR_HIP(hipStreamSynchronize(m->stream));
   }
   if (neq > 0) {
+    const size_t nb = ss * (size_t)(tq * kQ4RowWords * kBlock);
+    OR_HIP(hipMalloc(&m->skq, nb ? nb : ss));
+    m->sky_bytes += (int64_t)nb;
     std::vector<int32_t> soa((size_t)(4 * neq));
     for (int64_t e = 0; e < neq; ++e)
-      for (int k = 0; k < 4; ++k) soa[(size_t)(k * neq + e)] = local(iadc[4 * e + k]);
-    OR_HIP(hipMemcpyAsync(m->adsky_q, aq.data(), sizeof(int32_t) * (nn + 1), hipMemcpyHostToDevice, m->stream));
+      for (int k = 0; k < 4; ++k) soa[(size_t)(k * neq + e)] = position(iadc[4 * e + k]);
+    OR_HIP(hipMemcpyAsync(m->deg_q, nq.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, m->stream));
+    OR_HIP(hipMemcpyAsync(m->slab_q, slab_q.data(), sizeof(int32_t) * nslice, hipMemcpyHostToDevice, m->stream));
     OR_HIP(hipMemcpyAsync(m->qiadc, soa.data(), sizeof(int32_t) * 4 * neq, hipMemcpyHostToDevice, m->stream));
     OR_HIP(hipStreamSynchronize(m->stream));
   }
+  m->bytes += m->sky_bytes;
   m->have_sky = true;
   return OR_OK;
 }
@@ -1060,7 +1137,7 @@ template <typename R, typename S> static int launch_cycle(or_model *m) {
     HexPtrs<S> e{m->hconn, m->hiads, (S *)m->hsig, (S *)m->heint, (S *)m->hrho, (S *)m->hpla, (S *)m->hepsd,
                  (S *)m->hoffg, (S *)m->hqvis, (const S *)m->hvol0, m->neh};
     const unsigned gb = (unsigned)((m->neh + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((k_hex_forces<R, S>), dim3(gb), dim3(kBlock), 0, m->stream, e, m->xv, (Row4<S> *)m->skh,
+    hipLaunchKernelGGL((k_hex_forces<R, S>), dim3(gb), dim3(kBlock), 0, m->stream, e, m->xv, (S *)m->skh,
                        m->hc, m->ts, p, m->dtmin, m->flags);
     OR_HIP(hipGetLastError());
   }
@@ -1071,16 +1148,16 @@ template <typename R, typename S> static int launch_cycle(or_model *m) {
                 (S *)m->qepsdip, m->neq};
     const unsigned gq = (unsigned)((m->neq + kBlock - 1) / kBlock);
     hipLaunchKernelGGL((k_q4_forces<R, S>), dim3(gq), dim3(kBlock), 0, m->stream, q, m->xv, m->vr,
-                       (Row8<S> *)m->skq, m->qc, m->ts, p, m->dtmin);
+                       (S *)m->skq, m->qc, m->ts, p, m->dtmin);
     OR_HIP(hipGetLastError());
   }
   if (m->timing) OR_HIP(hipEventRecord(m->ev[4 * m->ring_used + 2], m->stream));
-  NodePtrs nd{m->xv, m->vr, m->d, m->ms, m->in, m->code, m->vimp, m->adsky_h, m->adsky_q, m->nn};
+  NodePtrs nd{m->xv, m->vr, m->d, m->ms, m->in, m->code, m->vimp, m->deg_h, m->deg_q, m->slab_h, m->slab_q, m->nn};
   const unsigned gn = (unsigned)((m->nn + kBlock - 1) / kBlock);
   const double dt_growth = (double)1.1f;
   const double dtmx = m->neh > 0 ? m->hc.dtmx : 1.0e20;
   hipLaunchKernelGGL((k_node_update<S>), dim3(gn), dim3(kBlock), 0, m->stream, nd,
-                     (const Row4<S> *)(m->neh > 0 ? m->skh : nullptr), (const Row8<S> *)(m->neq > 0 ? m->skq : nullptr),
+                     (const S *)(m->neh > 0 ? m->skh : nullptr), (const S *)(m->neq > 0 ? m->skq : nullptr),
                      m->ts, p, m->dtmin, m->flags, dt_growth, dtmx);
   OR_HIP(hipGetLastError());
   m->parity ^= 1;

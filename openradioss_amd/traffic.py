@@ -4,8 +4,9 @@ Word counts per element-update follow the compulsory-traffic model of the
 explicit cycle: element state read once and written once, connectivity read
 once, node data gathered once per node per cycle, and the nodal update reading
 and writing each node once.  The deterministic two-phase assembly adds the
-skyline rows (written by the element kernels, read by the node kernel) and the
-row indices; "as_implemented" includes them.
+skyline rows (written by the element kernels, read by the node kernel: 3 force
+words per hex row, 3 force + 3 moment words per shell row) and the row indices;
+"as_implemented" includes them.
 """
 from __future__ import annotations
 
@@ -23,6 +24,8 @@ Q4_IADC_BYTES = 4 * 4
 
 NODE_GATHER_BYTES = 6 * 8      # X(3), V(3) in FP64
 NODE_GATHER_ROT_BYTES = 3 * 8  # VR(3)
+HEX_ROW_WORDS = 3              # fx, fy, fz
+Q4_ROW_WORDS = 6               # fx, fy, fz, mx, my, mz
 
 
 def state_word(precision: str) -> int:
@@ -33,7 +36,7 @@ def hex_element_kernel_bytes(ne: int, nn: int, precision: str) -> dict:
     """Bytes one launch of the hex element kernel must move (ne elements, nn nodes gathered)."""
     w = state_word(precision)
     compulsory = ne * ((HEX_STATE_READ + HEX_STATE_WRITE) * w + HEX_CONN_BYTES) + nn * NODE_GATHER_BYTES
-    rows = ne * (HEX_IADS_BYTES + 8 * 4 * w)
+    rows = ne * (HEX_IADS_BYTES + 8 * HEX_ROW_WORDS * w)
     return {"compulsory": compulsory, "as_implemented": compulsory + rows}
 
 
@@ -42,19 +45,19 @@ def q4_element_kernel_bytes(ne: int, nn: int, npt: int, precision: str) -> dict:
     w = state_word(precision)
     words = Q4_GBUF_READ + Q4_GBUF_WRITE + 2 * npt * Q4_LBUF_WORDS
     compulsory = ne * (words * w + Q4_CONN_BYTES) + nn * (NODE_GATHER_BYTES + NODE_GATHER_ROT_BYTES)
-    rows = ne * (Q4_IADC_BYTES + 4 * 8 * w)
+    rows = ne * (Q4_IADC_BYTES + 4 * Q4_ROW_WORDS * w)
     return {"compulsory": compulsory, "as_implemented": compulsory + rows}
 
 
-def node_kernel_bytes(nn: int, rows_hex: int, rows_q4: int, rotational: bool, precision: str) -> dict:
-    """Bytes of one launch of the node kernel: X,V,D read and written, MS, code, and with
-    rotations VR read and written and IN read; as implemented, the skyline rows and pointers."""
-    per = (6 + 3) * 8 * 2 + 8 + 4
-    if rotational:
-        per += 3 * 8 * 2 + 8
-    compulsory = nn * per
+def node_kernel_bytes(nn: int, rows_hex: int, rows_q4: int, nn_rot: int, precision: str) -> dict:
+    """Bytes of one launch of the node kernel: X, V, D read and written, MS and the node code
+    read; for the nn_rot nodes with rotational work (shell rows or rotational BCs) also VR read
+    and written and IN read.  As implemented: the skyline rows, one CSR pointer per node and
+    family, and VR padded to 4 words."""
+    compulsory = nn * ((6 + 3) * 8 * 2 + 8 + 4) + nn_rot * (3 * 8 * 2 + 8)
     w = state_word(precision)
-    rows = rows_hex * 4 * w + rows_q4 * 8 * w + nn * 4 * ((rows_hex > 0) + (rows_q4 > 0) + 1)
+    fams = (rows_hex > 0) + (rows_q4 > 0)
+    rows = rows_hex * HEX_ROW_WORDS * w + rows_q4 * Q4_ROW_WORDS * w + nn * 4 * fams + nn_rot * 2 * 8 * 2
     return {"compulsory": compulsory, "as_implemented": compulsory + rows}
 
 
@@ -68,7 +71,14 @@ def cycle_bytes(model, precision: str) -> dict:
     if model.q4 is not None:
         nq = nn_touched(model.q4.conn, nn)
         out["q4"] = q4_element_kernel_bytes(model.ne_q4, nq, model.q4.prop.npt, precision)
-    out["node"] = node_kernel_bytes(nn, 8 * model.ne_hex, 4 * model.ne_q4, model.q4 is not None, precision)
+    nn_rot = 0
+    if model.q4 is not None:
+        import numpy as np
+        rot = np.zeros(nn, dtype=bool)
+        rot[model.q4.conn.reshape(-1)] = True
+        rot |= (np.asarray(model.code) & (32 | 16 | 8)) != 0
+        nn_rot = int(rot.sum())
+    out["node"] = node_kernel_bytes(nn, 8 * model.ne_hex, 4 * model.ne_q4, nn_rot, precision)
     out["total"] = {k: sum(v[k] for v in out.values()) for k in ("compulsory", "as_implemented")}
     return out
 
@@ -82,5 +92,5 @@ def nn_touched(conn, nn: int) -> int:
 
 def hex_bytes_per_element_update(ne: int, nn: int, precision: str) -> dict:
     e = hex_element_kernel_bytes(ne, nn, precision)
-    n = node_kernel_bytes(nn, 8 * ne, 0, False, precision)
+    n = node_kernel_bytes(nn, 8 * ne, 0, 0, precision)
     return {k: (e[k] + n[k]) / ne for k in ("compulsory", "as_implemented")}
