@@ -85,6 +85,54 @@ def cpu_baseline(model, budget_s: float = 12.0):
                        f"128) on the same {ne}-element model, {n} cycles after 1 warm-up cycle, {el:.1f} s")
 
 
+def sweep_leg(precision: str, cycles: int, warmup: int, n: int = 256):
+    """The parametric tube sweep (n tubes, element size 1-4 mm, 4.5k-72k shells each) advanced
+    together in one batch; a bounded sample of `cycles` batch cycles after `warmup`."""
+    from openradioss_amd import engine as E
+    from openradioss_amd import model as M
+
+    t0 = time.perf_counter()
+    sw = M.tube_sweep(n, seed=SEED)
+    b = E.GpuBatch()
+    gms = []
+    for m, _ in sw:
+        g = E.GpuModel(m, precision=precision)
+        b.add(g, 5.0e-3)
+        gms.append(g)
+    setup = time.perf_counter() - t0
+    b.run(warmup)
+    t1 = time.perf_counter()
+    b.run(cycles)
+    wall = time.perf_counter() - t1
+    active = b.active()
+    ne = sum(m.ne_q4 for m, _ in sw)
+    tts = [g.time_state().tt for g in gms]
+    b.close()
+    for g in gms:
+        g.close()
+    return dict(models=n, shells=ne, precision=precision, cycles=cycles, ms_per_cycle=1e3 * wall / cycles,
+                element_updates_per_s=ne * cycles / wall, active_after=active, setup_s=setup,
+                sim_time_range_s=[min(tts), max(tts)])
+
+
+def serve_leg(nreq: int, rate: float, time_scale: float):
+    """Served crash solves: Poisson arrivals, bars and tubes of 5k-500k elements, FP64 / split,
+    end times U(1, 10) ms x time_scale, admitted by HBM footprint into one running batch."""
+    import torch
+
+    from openradioss_amd import serve as S
+
+    reqs = S.request_stream(nreq, rate=rate, seed=SEED, time_scale=time_scale)
+    free, total = torch.cuda.mem_get_info(0)
+    budget = int(0.8 * free)
+    srv = S.Server(hbm_budget=budget, cycles_per_step=50, keep_results=False)
+    st = srv.serve(reqs)
+    srv.close()
+    st.update(rate_per_s=rate, time_scale=time_scale, hbm_budget_gb=budget / 1e9, max_used_gb=srv.max_used / 1e9,
+              elements=[min(r.ne for r in reqs), max(r.ne for r in reqs)])
+    return st
+
+
 def copy_probe(mib: int):
     """Three plain device-to-device copies of a known size: each reads and writes mib MiB."""
     import torch
@@ -106,6 +154,8 @@ def main():
     ap.add_argument("--plate", type=int, default=1000, help="Q4 plate edge in elements (1000 = roofline model)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision form")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched tube sweep")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -179,6 +229,18 @@ def main():
             "disp_rel_err_vs_fp64_gpu": derr, "energy_rel_err_vs_fp64_gpu": eerr,
             "dtype": "f32 element / f64 node"}
         log(f"split: {eus:.3e} element-updates/s (x{eus / eu64:.2f}), disp err {derr:.2e}, energy err {eerr:.2e}")
+    if not args.no_sweep:
+        out["batch_sweep"] = {}
+        for prec in ("fp64", "split"):
+            r = sweep_leg(prec, cycles=100, warmup=10)
+            out["batch_sweep"][prec] = r
+            log(f"sweep {prec}: {r['element_updates_per_s']:.3e} element-updates/s over {r['models']} tubes "
+                f"({r['shells']} shells), {r['ms_per_cycle']:.3f} ms per batch cycle")
+    if not args.no_serve:
+        r = serve_leg(32, rate=40.0, time_scale=0.02)
+        out["serving"] = r
+        log(f"serve: {r['requests']} requests, p50 {r['p50_s']:.3f} s, p99 {r['p99_s']:.3f} s, "
+            f"{r['element_updates_per_s']:.3e} element-updates/s")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(model)
         log(f"cpu: {out['cpu_baseline']['value']:.3e} element-updates/s")

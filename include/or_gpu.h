@@ -152,6 +152,13 @@ int or_model_kernel_times(or_model *m, int reset, double *hex_ms, double *q4_ms,
 int or_model_set_timing(or_model *m, int enable);
 /* Bytes of device memory held by the model. */
 int or_model_footprint(const or_model *m, int64_t *bytes);
+/* Bytes a model will hold once created and uploaded, computed without allocating
+ * anything (admission by HBM footprint).  strain_rate_filter: the shells' law has
+ * the strain-rate filter on (ISRATE > 0); rows_hex / rows_q4: (nn) number of hex /
+ * shell skyline rows of each node (required when the family is present). */
+int or_model_footprint_estimate(int64_t nn, int64_t ne_hex, int64_t ne_q4, int32_t npt, int precision,
+                                int strain_rate_filter, const int32_t *rows_hex, const int32_t *rows_q4,
+                                int64_t *bytes);
 
 /* ---- group-level drop-in for SFORC3 (one group of nel <= 128 hexes) ----
  * Host arrays in the reference layouts: x X(3,numnod), v V(3,numnod), ixs IXS(11,*)
@@ -173,6 +180,26 @@ int or_cforc3_group(int device, const or_q4_params *p, int64_t numnod, int32_t n
                     double *thk, const double *thke, double *forc, double *mom, double *gstr, double *hourg,
                     double *eint, double *epsd, double *sig, double *pla, double *epsd_ip, int64_t lsky,
                     double *fsky, const int32_t *iadc, double dt1, double *dt2t);
+
+/* ---- batches: many resident models advanced together ----
+ * No counterpart in the reference, which runs one deck per process: a batch advances
+ * every member model by its own explicit cycles (its own DT2 pipeline, time step and
+ * state) in one launch per kernel and precision class per cycle, so that many small
+ * models fill one GPU.  A member stops after the cycle whose time reaches its t_end
+ * (or when an element raises MSTOP) and then keeps its state.  Members stay owned by
+ * the caller; their or_model_download_* / get_time / energy calls work between
+ * batch calls.  or_model_run is refused while a model belongs to a batch. */
+typedef struct or_batch or_batch;
+int or_batch_create(or_batch **out, int device);
+int or_batch_destroy(or_batch *b);   /* removes the members, does not destroy them */
+int or_batch_add(or_batch *b, or_model *m, double t_end);
+int or_batch_remove(or_batch *b, or_model *m);
+/* Launch ncycles cycles of every member still running (asynchronous). */
+int or_batch_run(or_batch *b, int64_t ncycles);
+int or_batch_sync(or_batch *b);
+/* Number of members still running; synchronises, and drops stopped members from the
+ * launches of later cycles. */
+int or_batch_active(or_batch *b, int32_t *nactive);
 
 /* Diagnostic, no counterpart in the reference: evaluates on the device the FP64
  * elementary functions the element kernels use in place of the host libm's

@@ -42,6 +42,7 @@ int fail(int code, const std::string &msg) {
   } while (0)
 
 constexpr int kBlock = 256;
+constexpr double kNoEnd = 1.0e300; // end time of a model advanced by or_model_run
 
 // Minimum waves per SIMD the element kernels are compiled for (register budget
 // 512 / waves per lane, VGPRs and AGPRs together).
@@ -126,6 +127,47 @@ struct NodePtrs {
   int64_t nn;
 };
 
+// One model as the cycle kernels see it: pointers into its resident arrays, its
+// parameters and its cycle scalars.  Held in device memory, one per model (and one
+// per slot of a batch); a launch covers any number of models through a tile list.
+struct ModelDev {
+  // hexes; element state in the model's storage type (double, or float in the split form)
+  const int32_t *hconn, *hiads;
+  void *hsig, *heint, *hrho, *hpla, *hepsd, *hoffg, *hqvis;
+  const void *hvol0;
+  int64_t neh;
+  // Q4 shells
+  const int32_t *qconn, *qiadc;
+  void *qoffg, *qthk, *qforc, *qmom, *qgstr, *qhourg, *qeint, *qepsd, *qsig, *qpla, *qepsdip;
+  const void *qthke;
+  int64_t neq;
+  NodePtrs nd;
+  void *skh, *skq;            // skyline rows (storage type)
+  DevTime *ts;                // [2] cycle scalars, by cycle parity
+  unsigned long long *dtmin;  // [2] element time-step minimum (FP64 bits), by parity
+  int *flags;                 // bit 0: an element reported MSTOP
+  double dtmx;                // DTMX
+  double tend;                // the model stops after the cycle whose time reaches tend
+  HexConsts hc;
+  Q4Consts qc;
+};
+
+// One block of a cycle launch: kBlock consecutive elements (or nodes) of one model.
+struct Tile {
+  int32_t model; // index in the launch's ModelDev array
+  int32_t first; // first element (node) of the block
+};
+
+template <typename S> __device__ __forceinline__ HexPtrs<S> hex_ptrs(const ModelDev &M) {
+  return HexPtrs<S>{M.hconn, M.hiads, (S *)M.hsig, (S *)M.heint, (S *)M.hrho, (S *)M.hpla, (S *)M.hepsd,
+                    (S *)M.hoffg, (S *)M.hqvis, (const S *)M.hvol0, M.neh};
+}
+template <typename S> __device__ __forceinline__ Q4Ptrs<S> q4_ptrs(const ModelDev &M) {
+  return Q4Ptrs<S>{M.qconn, M.qiadc, (S *)M.qoffg, (S *)M.qthk, (const S *)M.qthke, (S *)M.qforc, (S *)M.qmom,
+                   (S *)M.qgstr, (S *)M.qhourg, (S *)M.qeint, (S *)M.qepsd, (S *)M.qsig, (S *)M.qpla,
+                   (S *)M.qepsdip, M.neq};
+}
+
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
@@ -154,10 +196,17 @@ __device__ __forceinline__ void load_node(const double *xv, int64_t n, double &x
 
 // ---------------------------------------------------------------- hex element kernel
 template <typename R, typename S>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(hex_waves<R>()))) void k_hex_forces(HexPtrs<S> e, const double *__restrict__ xv,
-                                                       S *__restrict__ sky, HexConsts c, const DevTime *ts,
-                                                       int parity, unsigned long long *dtmin, int *flags) {
-  const int64_t i = xcd_contiguous_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(hex_waves<R>()))) void k_hex_forces(
+    const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity) {
+  const Tile tl = tiles[xcd_contiguous_block(blockIdx.x, gridDim.x)];
+  const ModelDev &M = md[tl.model];
+  const DevTime *ts = M.ts;
+  if (ts[parity].done) return;
+  const HexPtrs<S> e = hex_ptrs<S>(M);
+  const double *__restrict__ xv = M.nd.xv;
+  S *__restrict__ sky = (S *)M.skh;
+  const HexConsts &c = M.hc;
+  const int64_t i = (int64_t)tl.first + threadIdx.x;
   double dtx = kEP06;
   int mstop = 0;
   if (i < e.ne) {
@@ -190,9 +239,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(hex_wave
     dtx = o.dtx;
     mstop = o.mstop;
   }
-  if (mstop) atomicOr(flags, 1);
+  if (mstop) atomicOr(M.flags, 1);
   const double m = block_min(dtx);
-  if (threadIdx.x == 0) atomicMin(dtmin + parity, dbits(m));
+  if (threadIdx.x == 0) atomicMin(M.dtmin + parity, dbits(m));
 }
 
 // ---------------------------------------------------------------- Q4 element kernel
@@ -216,11 +265,18 @@ template <typename R, typename S> struct Q4IpIO {
 };
 
 template <typename R, typename S>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(q4_waves<R>()))) void k_q4_forces(Q4Ptrs<S> q, const double *__restrict__ xv,
-                                                      const double *__restrict__ vr, S *__restrict__ sky,
-                                                      Q4Consts c, const DevTime *ts, int parity,
-                                                      unsigned long long *dtmin) {
-  const int64_t i = xcd_contiguous_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(q4_waves<R>()))) void k_q4_forces(
+    const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity) {
+  const Tile tl = tiles[xcd_contiguous_block(blockIdx.x, gridDim.x)];
+  const ModelDev &M = md[tl.model];
+  const DevTime *ts = M.ts;
+  if (ts[parity].done) return;
+  const Q4Ptrs<S> q = q4_ptrs<S>(M);
+  const double *__restrict__ xv = M.nd.xv;
+  const double *__restrict__ vr = M.nd.vr;
+  S *__restrict__ sky = (S *)M.skq;
+  const Q4Consts &c = M.qc;
+  const int64_t i = (int64_t)tl.first + threadIdx.x;
   double dt = kEP06;
   if (i < q.ne) {
     const R DT1 = R(ts[parity].dt2);
@@ -271,18 +327,33 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(q4_waves
     dt = o.dt;
   }
   const double m = block_min(dt);
-  if (threadIdx.x == 0) atomicMin(dtmin + parity, dbits(m));
+  if (threadIdx.x == 0) atomicMin(M.dtmin + parity, dbits(m));
 }
 
 // ---------------------------------------------------------------- node kernel
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_node_update(NodePtrs nd, const S *__restrict__ skh,
-                                                        const S *__restrict__ skq, DevTime *ts, int parity,
-                                                        unsigned long long *dtmin, const int *flags,
-                                                        double dt_growth, double dtmx) {
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  // DT2 pipeline (resol.F:2721-2722, 2826, 6125-6129, 6352, 6494-6497)
+__global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restrict__ md,
+                                                        const Tile *__restrict__ tiles, int parity,
+                                                        double dt_growth) {
+  const Tile tl = tiles[blockIdx.x];
+  const ModelDev &M = md[tl.model];
+  const NodePtrs &nd = M.nd;
+  const S *__restrict__ skh = (const S *)M.skh;
+  const S *__restrict__ skq = (const S *)M.skq;
+  DevTime *ts = M.ts;
+  unsigned long long *dtmin = M.dtmin;
+  const double dtmx = M.dtmx;
+  const int64_t n = (int64_t)tl.first + threadIdx.x;
   const DevTime prev = ts[parity];
+  if (prev.done) {
+    // a stopped model keeps its state; its scalars are carried to the other parity
+    if (n == 0) {
+      ts[parity ^ 1] = prev;
+      dtmin[parity ^ 1] = dbits(kEP06);
+    }
+    return;
+  }
+  // DT2 pipeline (resol.F:2721-2722, 2826, 6125-6129, 6352, 6494-6497)
   const double DT1 = prev.dt2;
   const double dt2t = bitsd(dtmin[parity]);
   double DT2 = kEP06;
@@ -301,8 +372,8 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodePtrs nd, const S *__
     nx.dt12 = DT12;
     nx.dt2old = DT2;
     nx.ncycle = prev.ncycle + 1;
-    nx.mstop = prev.mstop | *flags;
-    nx.pad = 0;
+    nx.mstop = prev.mstop | *M.flags;
+    nx.done = (nx.mstop != 0 || nx.tt >= M.tend) ? 1 : 0;
     ts[parity ^ 1] = nx;
     dtmin[parity ^ 1] = dbits(kEP06);
   }
@@ -698,6 +769,13 @@ struct or_model {
   int *flags = nullptr;
   double *energy = nullptr;            // [3 + 3*nb]
   int64_t parity = 0;
+  // what the kernels see (host copy, device copy) and the model's own tile lists
+  ModelDev hdev{};
+  ModelDev *dev = nullptr;
+  bool dev_dirty = true;
+  Tile *tiles_h = nullptr, *tiles_q = nullptr, *tiles_n = nullptr;
+  int32_t nt_h = 0, nt_q = 0, nt_n = 0;
+  or_batch *batch = nullptr;           // set while the model belongs to a batch
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;          // ring of 4 events per cycle (hex | q4 | node)
@@ -705,6 +783,39 @@ struct or_model {
   double t_hex = 0.0, t_q4 = 0.0, t_node = 0.0;
   int64_t t_cycles = 0;
 };
+
+// A set of resident models advanced together: every cycle launches each kernel once
+// per precision class over the tile lists of all live members (or_batch_*).
+struct or_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<or_model *> members; // slot order
+  std::vector<double> tend;        // per slot
+  std::vector<char> live;          // per slot: still cycling (cleared by or_batch_active)
+  ModelDev *md = nullptr;          // device array of the slots' kernel views
+  int64_t md_cap = 0;
+  Tile *tiles[2][3] = {};          // [precision][hex, q4, node]
+  int32_t nt[2][3] = {};
+  int64_t tcap[2][3] = {};
+  bool dirty = true;
+  int64_t parity = 0;
+  int64_t cycles = 0;              // cycles launched
+};
+
+// Wait for every stream that may hold work of the model (its own, and its batch's).
+static hipError_t quiesce(or_model *m) {
+  if (m->batch) {
+    hipError_t e = hipStreamSynchronize(m->batch->stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipStreamSynchronize(m->stream);
+}
+
+// Pointers or parameters of the model changed: its kernel views must be rebuilt.
+static void mark_changed(or_model *m) {
+  m->dev_dirty = true;
+  if (m->batch) m->batch->dirty = true;
+}
 
 static size_t ssize(const or_model *m) { return m->prec == OR_PREC_FP64 ? sizeof(double) : sizeof(float); }
 
@@ -785,6 +896,13 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int6
   A(&m->ts, 2);
   A(&m->dtmin, 2);
   A(&m->flags, 1);
+  A(&m->dev, 1);
+  m->nt_h = (int32_t)((ne_hex + kBlock - 1) / kBlock);
+  m->nt_q = (int32_t)((ne_q4 + kBlock - 1) / kBlock);
+  m->nt_n = (int32_t)((nn + kBlock - 1) / kBlock);
+  A(&m->tiles_h, m->nt_h);
+  A(&m->tiles_q, m->nt_q);
+  A(&m->tiles_n, m->nt_n);
   const int64_t nb = 1024;
   A(&m->energy, 3 + 3 * nb);
   if (rc == OR_OK && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
@@ -803,6 +921,17 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int6
     double big = kEP06;
     std::memcpy(&dm[0], &big, 8);
     dm[1] = dm[0];
+    std::vector<Tile> tl;
+    auto put_tiles = [&](Tile *dst, int32_t nt) {
+      if (rc != OR_OK || nt == 0) return;
+      tl.resize((size_t)nt);
+      for (int32_t b = 0; b < nt; ++b) tl[(size_t)b] = Tile{0, b * kBlock};
+      if (hipMemcpy(dst, tl.data(), sizeof(Tile) * (size_t)nt, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(OR_ERR_HIP, "tile upload");
+    };
+    put_tiles(m->tiles_h, m->nt_h);
+    put_tiles(m->tiles_q, m->nt_q);
+    put_tiles(m->tiles_n, m->nt_n);
     if (rc == OR_OK && (hipMemcpyAsync(m->ts, tsh, sizeof(tsh), hipMemcpyHostToDevice, m->stream) != hipSuccess ||
                         hipMemcpyAsync(m->dtmin, dm, sizeof(dm), hipMemcpyHostToDevice, m->stream) != hipSuccess ||
                         hipStreamSynchronize(m->stream) != hipSuccess))
@@ -820,12 +949,14 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int6
 
 int or_model_destroy(or_model *m) {
   if (!m) return OR_OK;
+  if (m->batch) (void)or_batch_remove(m->batch, m);
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   void *ptrs[] = {m->xv, m->vr, m->d, m->ms, m->in, m->vimp, m->code, m->deg_h, m->deg_q, m->slab_h, m->slab_q, m->hconn, m->hiads,
                   m->hsig, m->heint, m->hrho, m->hvol0, m->hpla, m->hepsd, m->hoffg, m->hqvis, m->skh, m->qconn,
                   m->qiadc, m->qoffg, m->qthk, m->qthke, m->qforc, m->qmom, m->qgstr, m->qhourg, m->qeint, m->qepsd,
-                  m->qsig, m->qpla, m->qepsdip, m->skq, m->ts, m->dtmin, m->flags, m->energy};
+                  m->qsig, m->qpla, m->qepsdip, m->skq, m->ts, m->dtmin, m->flags, m->energy, m->dev,
+                  m->tiles_h, m->tiles_q, m->tiles_n};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
@@ -840,6 +971,7 @@ int or_model_set_hex_params(or_model *m, const or_hex_params *p) {
   if (rc) return rc;
   m->hc = make_hex_consts(*p);
   m->have_hparams = true;
+  mark_changed(m);
   return OR_OK;
 }
 
@@ -855,11 +987,12 @@ int or_model_set_q4_params(or_model *m, const or_q4_params *p) {
     OR_HIP(hipSetDevice(m->device));
     OR_HIP(hipMalloc(&m->qepsdip, ss * (size_t)(m->npt * m->neq)));
     OR_HIP(hipMemsetAsync(m->qepsdip, 0, ss * (size_t)(m->npt * m->neq), m->stream));
-    OR_HIP(hipStreamSynchronize(m->stream));
+    OR_HIP(quiesce(m));
     m->bytes += (int64_t)(ss * (size_t)(m->npt * m->neq));
   }
   m->qc = c;
   m->have_qparams = true;
+  mark_changed(m);
   return OR_OK;
 }
 
@@ -892,7 +1025,7 @@ int or_model_upload_nodes(or_model *m, const double *x, const double *v, const d
     OR_HIP(hipMemcpyAsync(m->in, in, sizeof(double) * nn, hipMemcpyHostToDevice, m->stream));
   }
   OR_HIP(hipMemsetAsync(m->d, 0, sizeof(double) * 3 * nn, m->stream));
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   m->have_nodes = true;
   return OR_OK;
 }
@@ -901,7 +1034,7 @@ template <typename S> static int put(or_model *m, void *dst, const double *src, 
   std::vector<S> t((size_t)n);
   for (int64_t i = 0; i < n; ++i) t[i] = (S)(src ? src[i] : dflt);
   OR_HIP(hipMemcpyAsync(dst, t.data(), sizeof(S) * n, hipMemcpyHostToDevice, m->stream));
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   return OR_OK;
 }
 static int put_state(or_model *m, void *dst, const double *src, int64_t n, double dflt) {
@@ -911,7 +1044,7 @@ static int put_state(or_model *m, void *dst, const double *src, int64_t n, doubl
 template <typename S> static int get(or_model *m, double *dst, const void *src, int64_t n) {
   std::vector<S> t((size_t)n);
   OR_HIP(hipMemcpyAsync(t.data(), src, sizeof(S) * n, hipMemcpyDeviceToHost, m->stream));
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   for (int64_t i = 0; i < n; ++i) dst[i] = (double)t[i];
   return OR_OK;
 }
@@ -937,7 +1070,7 @@ int or_model_upload_hex(or_model *m, const int32_t *conn, const double *sig, con
       soa[(size_t)(k * ne + e)] = n;
     }
   OR_HIP(hipMemcpyAsync(m->hconn, soa.data(), sizeof(int32_t) * 8 * ne, hipMemcpyHostToDevice, m->stream));
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   int rc;
   if ((rc = put_state(m, m->hsig, sig, 6 * ne, 0.0))) return rc;
   if ((rc = put_state(m, m->heint, eint, ne, 0.0))) return rc;
@@ -970,7 +1103,7 @@ int or_model_upload_q4(or_model *m, const int32_t *conn, const double *thke, con
     }
   }
   OR_HIP(hipMemcpyAsync(m->qconn, soa.data(), sizeof(int32_t) * 4 * ne, hipMemcpyHostToDevice, m->stream));
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   const int64_t np = m->npt;
   int rc;
   if ((rc = put_state(m, m->qthke, thke, ne, 0.0))) return rc;
@@ -1067,7 +1200,7 @@ int or_model_upload_skyline(or_model *m, const int64_t *adsky, const int64_t *ia
     OR_HIP(hipMemcpyAsync(m->deg_h, nh.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, m->stream));
     OR_HIP(hipMemcpyAsync(m->slab_h, slab_h.data(), sizeof(int32_t) * nslice, hipMemcpyHostToDevice, m->stream));
     OR_HIP(hipMemcpyAsync(m->hiads, soa.data(), sizeof(int32_t) * 8 * neh, hipMemcpyHostToDevice, m->stream));
-    OR_HIP(hipStreamSynchronize(m->stream));
+    OR_HIP(quiesce(m));
   }
   if (neq > 0) {
     const size_t nb = ss * (size_t)(tq * kQ4RowWords * kBlock);
@@ -1079,22 +1212,28 @@ int or_model_upload_skyline(or_model *m, const int64_t *adsky, const int64_t *ia
     OR_HIP(hipMemcpyAsync(m->deg_q, nq.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, m->stream));
     OR_HIP(hipMemcpyAsync(m->slab_q, slab_q.data(), sizeof(int32_t) * nslice, hipMemcpyHostToDevice, m->stream));
     OR_HIP(hipMemcpyAsync(m->qiadc, soa.data(), sizeof(int32_t) * 4 * neq, hipMemcpyHostToDevice, m->stream));
-    OR_HIP(hipStreamSynchronize(m->stream));
+    OR_HIP(quiesce(m));
   }
   m->bytes += m->sky_bytes;
   m->have_sky = true;
+  mark_changed(m);
   return OR_OK;
 }
 
 int or_model_set_time(or_model *m, const or_time_state *t) {
   if (!m || !t) return fail(OR_ERR_ARG, "null argument");
   OR_HIP(hipSetDevice(m->device));
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   DevTime dt{};
   dt.tt = t->tt; dt.dt1 = t->dt1; dt.dt2 = t->dt2; dt.dt12 = t->dt12; dt.dt2old = t->dt2old;
-  dt.ncycle = t->ncycle; dt.mstop = t->mstop;
+  dt.ncycle = t->ncycle; dt.mstop = t->mstop; dt.done = t->mstop ? 1 : 0;
   DevTime two[2] = {dt, dt};
   OR_HIP(hipMemcpy(m->ts, two, sizeof(two), hipMemcpyHostToDevice));
+  unsigned long long dm[2];
+  const double big = kEP06;
+  std::memcpy(&dm[0], &big, 8);
+  dm[1] = dm[0];
+  OR_HIP(hipMemcpy(m->dtmin, dm, sizeof(dm), hipMemcpyHostToDevice));
   int zero = 0;
   OR_HIP(hipMemcpy(m->flags, &zero, sizeof(int), hipMemcpyHostToDevice));
   m->parity = 0;
@@ -1104,7 +1243,7 @@ int or_model_set_time(or_model *m, const or_time_state *t) {
 int or_model_get_time(or_model *m, or_time_state *t) {
   if (!m || !t) return fail(OR_ERR_ARG, "null argument");
   OR_HIP(hipSetDevice(m->device));
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   DevTime dt;
   OR_HIP(hipMemcpy(&dt, m->ts + (m->parity & 1), sizeof(DevTime), hipMemcpyDeviceToHost));
   int flags = 0;
@@ -1131,36 +1270,60 @@ static int flush_timing(or_model *m) {
   return OR_OK;
 }
 
-template <typename R, typename S> static int launch_cycle(or_model *m) {
-  const int p = (int)(m->parity & 1);
-  if (m->neh > 0) {
-    HexPtrs<S> e{m->hconn, m->hiads, (S *)m->hsig, (S *)m->heint, (S *)m->hrho, (S *)m->hpla, (S *)m->hepsd,
-                 (S *)m->hoffg, (S *)m->hqvis, (const S *)m->hvol0, m->neh};
-    const unsigned gb = (unsigned)((m->neh + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((k_hex_forces<R, S>), dim3(gb), dim3(kBlock), 0, m->stream, e, m->xv, (S *)m->skh,
-                       m->hc, m->ts, p, m->dtmin, m->flags);
+// Refresh the device copy of a model's kernel view after uploads or parameter changes.
+static void fill_dev(or_model *m, ModelDev &d, double tend) {
+  d = ModelDev{};
+  d.hconn = m->hconn; d.hiads = m->hiads; d.hsig = m->hsig; d.heint = m->heint; d.hrho = m->hrho;
+  d.hpla = m->hpla; d.hepsd = m->hepsd; d.hoffg = m->hoffg; d.hqvis = m->hqvis; d.hvol0 = m->hvol0;
+  d.neh = m->neh;
+  d.qconn = m->qconn; d.qiadc = m->qiadc; d.qoffg = m->qoffg; d.qthk = m->qthk; d.qforc = m->qforc;
+  d.qmom = m->qmom; d.qgstr = m->qgstr; d.qhourg = m->qhourg; d.qeint = m->qeint; d.qepsd = m->qepsd;
+  d.qsig = m->qsig; d.qpla = m->qpla; d.qepsdip = m->qepsdip; d.qthke = m->qthke; d.neq = m->neq;
+  d.nd = NodePtrs{m->xv, m->vr, m->d, m->ms, m->in, m->code, m->vimp, m->deg_h, m->deg_q, m->slab_h, m->slab_q,
+                  m->nn};
+  d.skh = m->neh > 0 ? m->skh : nullptr;
+  d.skq = m->neq > 0 ? m->skq : nullptr;
+  d.ts = m->ts; d.dtmin = m->dtmin; d.flags = m->flags;
+  d.dtmx = m->neh > 0 ? m->hc.dtmx : 1.0e20;
+  d.tend = tend;
+  d.hc = m->hc;
+  d.qc = m->qc;
+}
+
+static int refresh_dev(or_model *m) {
+  if (!m->dev_dirty) return OR_OK;
+  fill_dev(m, m->hdev, kNoEnd);
+  OR_HIP(quiesce(m));
+  OR_HIP(hipMemcpy(m->dev, &m->hdev, sizeof(ModelDev), hipMemcpyHostToDevice));
+  m->dev_dirty = false;
+  return OR_OK;
+}
+
+// The three launches of one cycle for a set of models of one precision: element
+// forces (hexes, then shells) and the node update, each over the set's tile list.
+struct LaunchSet {
+  const ModelDev *md = nullptr;
+  const Tile *th = nullptr, *tq = nullptr, *tn = nullptr;
+  int32_t nth = 0, ntq = 0, ntn = 0;
+};
+
+template <typename R, typename S>
+static int launch_cycle(hipStream_t st, const LaunchSet &L, int parity, hipEvent_t after_hex, hipEvent_t after_q4) {
+  if (L.nth > 0) {
+    hipLaunchKernelGGL((k_hex_forces<R, S>), dim3((unsigned)L.nth), dim3(kBlock), 0, st, L.md, L.th, parity);
     OR_HIP(hipGetLastError());
   }
-  if (m->timing) OR_HIP(hipEventRecord(m->ev[4 * m->ring_used + 1], m->stream));
-  if (m->neq > 0) {
-    Q4Ptrs<S> q{m->qconn, m->qiadc, (S *)m->qoffg, (S *)m->qthk, (const S *)m->qthke, (S *)m->qforc, (S *)m->qmom,
-                (S *)m->qgstr, (S *)m->qhourg, (S *)m->qeint, (S *)m->qepsd, (S *)m->qsig, (S *)m->qpla,
-                (S *)m->qepsdip, m->neq};
-    const unsigned gq = (unsigned)((m->neq + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((k_q4_forces<R, S>), dim3(gq), dim3(kBlock), 0, m->stream, q, m->xv, m->vr,
-                       (S *)m->skq, m->qc, m->ts, p, m->dtmin);
+  if (after_hex) OR_HIP(hipEventRecord(after_hex, st));
+  if (L.ntq > 0) {
+    hipLaunchKernelGGL((k_q4_forces<R, S>), dim3((unsigned)L.ntq), dim3(kBlock), 0, st, L.md, L.tq, parity);
     OR_HIP(hipGetLastError());
   }
-  if (m->timing) OR_HIP(hipEventRecord(m->ev[4 * m->ring_used + 2], m->stream));
-  NodePtrs nd{m->xv, m->vr, m->d, m->ms, m->in, m->code, m->vimp, m->deg_h, m->deg_q, m->slab_h, m->slab_q, m->nn};
-  const unsigned gn = (unsigned)((m->nn + kBlock - 1) / kBlock);
-  const double dt_growth = (double)1.1f;
-  const double dtmx = m->neh > 0 ? m->hc.dtmx : 1.0e20;
-  hipLaunchKernelGGL((k_node_update<S>), dim3(gn), dim3(kBlock), 0, m->stream, nd,
-                     (const S *)(m->neh > 0 ? m->skh : nullptr), (const S *)(m->neq > 0 ? m->skq : nullptr),
-                     m->ts, p, m->dtmin, m->flags, dt_growth, dtmx);
-  OR_HIP(hipGetLastError());
-  m->parity ^= 1;
+  if (after_q4) OR_HIP(hipEventRecord(after_q4, st));
+  if (L.ntn > 0) {
+    hipLaunchKernelGGL((k_node_update<S>), dim3((unsigned)L.ntn), dim3(kBlock), 0, st, L.md, L.tn, parity,
+                       (double)1.1f);
+    OR_HIP(hipGetLastError());
+  }
   return OR_OK;
 }
 
@@ -1170,7 +1333,15 @@ int or_model_run(or_model *m, int64_t ncycles) {
   if (!m->have_nodes || !m->have_sky || (m->neh > 0 && (!m->have_hparams || !m->have_hex)) ||
       (m->neq > 0 && (!m->have_qparams || !m->have_q4)))
     return fail(OR_ERR_STATE, "params, nodes, element state and skyline must be uploaded before run");
+  if (m->batch) return fail(OR_ERR_STATE, "the model belongs to a batch: advance it with or_batch_run");
   OR_HIP(hipSetDevice(m->device));
+  int rc0 = refresh_dev(m);
+  if (rc0) return rc0;
+  LaunchSet L;
+  L.md = m->dev;
+  L.th = m->tiles_h; L.nth = m->nt_h;
+  L.tq = m->tiles_q; L.ntq = m->nt_q;
+  L.tn = m->tiles_n; L.ntn = m->nt_n;
   for (int64_t c = 0; c < ncycles; ++c) {
     if (m->timing) {
       if (m->ring_used * 4 + 4 > (int)m->ev.size()) {
@@ -1179,8 +1350,13 @@ int or_model_run(or_model *m, int64_t ncycles) {
       }
       OR_HIP(hipEventRecord(m->ev[4 * m->ring_used], m->stream));
     }
-    int rc = (m->prec == OR_PREC_FP64) ? launch_cycle<double, double>(m) : launch_cycle<float, float>(m);
+    hipEvent_t e1 = m->timing ? m->ev[4 * m->ring_used + 1] : nullptr;
+    hipEvent_t e2 = m->timing ? m->ev[4 * m->ring_used + 2] : nullptr;
+    const int p = (int)(m->parity & 1);
+    int rc = (m->prec == OR_PREC_FP64) ? launch_cycle<double, double>(m->stream, L, p, e1, e2)
+                                       : launch_cycle<float, float>(m->stream, L, p, e1, e2);
     if (rc) return rc;
+    m->parity ^= 1;
     if (m->timing) {
       OR_HIP(hipEventRecord(m->ev[4 * m->ring_used + 3], m->stream));
       m->ring_used += 1;
@@ -1192,7 +1368,7 @@ int or_model_run(or_model *m, int64_t ncycles) {
 int or_model_sync(or_model *m) {
   if (!m) return fail(OR_ERR_ARG, "null model");
   OR_HIP(hipSetDevice(m->device));
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   int flags = 0;
   OR_HIP(hipMemcpy(&flags, m->flags, sizeof(int), hipMemcpyDeviceToHost));
   if (flags & 1) return fail(OR_ERR_MSTOP, "non-positive Jacobian: element deleted and MSTOP raised");
@@ -1227,7 +1403,7 @@ int or_model_kernel_times(or_model *m, int reset, double *hex_ms, double *q4_ms,
 int or_model_download_nodes(or_model *m, double *x, double *v, double *vr, double *d) {
   if (!m) return fail(OR_ERR_ARG, "null model");
   OR_HIP(hipSetDevice(m->device));
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   const int64_t nn = m->nn;
   std::vector<double> xv((size_t)(6 * nn));
   OR_HIP(hipMemcpy(xv.data(), m->xv, sizeof(double) * 6 * nn, hipMemcpyDeviceToHost));
@@ -1252,7 +1428,7 @@ int or_model_download_hex(or_model *m, double *sig, double *eint, double *rho, d
                           double *offg, double *qvis) {
   if (!m) return fail(OR_ERR_ARG, "null model");
   OR_HIP(hipSetDevice(m->device));
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   const int64_t ne = m->neh;
   int rc;
   if ((rc = get_state(m, sig, m->hsig, 6 * ne))) return rc;
@@ -1269,7 +1445,7 @@ int or_model_download_q4(or_model *m, double *offg, double *thk, double *forc, d
                          double *hourg, double *eint, double *epsd, double *sig, double *pla, double *epsd_ip) {
   if (!m) return fail(OR_ERR_ARG, "null model");
   OR_HIP(hipSetDevice(m->device));
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   const int64_t ne = m->neq, np = m->npt;
   int rc;
   if ((rc = get_state(m, offg, m->qoffg, ne))) return rc;
@@ -1316,8 +1492,44 @@ int or_model_energy(or_model *m, double out[3]) {
   OR_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, m->stream, (const double *)partial, nb, m->energy);
   OR_HIP(hipGetLastError());
-  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(quiesce(m));
   OR_HIP(hipMemcpy(out, m->energy, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  return OR_OK;
+}
+
+int or_model_footprint_estimate(int64_t nn, int64_t ne_hex, int64_t ne_q4, int32_t npt, int precision,
+                                int strain_rate_filter, const int32_t *rows_hex, const int32_t *rows_q4,
+                                int64_t *bytes) {
+  if (!bytes) return fail(OR_ERR_ARG, "null argument");
+  if (nn <= 0 || ne_hex < 0 || ne_q4 < 0) return fail(OR_ERR_ARG, "nn must be > 0, ne_hex and ne_q4 >= 0");
+  if (ne_q4 > 0 && (npt < 2 || npt > 10)) return fail(OR_ERR_ARG, "shell NPT must be in 2..10");
+  if (precision != OR_PREC_FP64 && precision != OR_PREC_SPLIT) return fail(OR_ERR_ARG, "bad precision");
+  if ((ne_hex > 0 && !rows_hex) || (ne_q4 > 0 && !rows_q4)) return fail(OR_ERR_ARG, "missing row counts");
+  const int64_t ss = precision == OR_PREC_FP64 ? 8 : 4;
+  const int64_t nsl = (nn + kBlock - 1) / kBlock;
+  // the allocations of or_model_create, in the same order
+  int64_t b = 8 * (6 * nn + 3 * nn + nn + 3 * nn) + 4 * nn;           // xv, d, ms, vimp, code
+  if (ne_q4 > 0) b += 8 * (4 * nn + nn);                               // vr, in
+  if (ne_hex > 0) b += 4 * (nn + nsl + 16 * ne_hex) + ss * 13 * ne_hex; // deg, slab, conn, iads, state
+  if (ne_q4 > 0) {
+    b += 4 * (nn + nsl + 8 * ne_q4) + ss * (4 + 5 + 3 + 8 + 5 + 2) * ne_q4 + ss * 6 * (int64_t)npt * ne_q4;
+    if (strain_rate_filter) b += ss * (int64_t)npt * ne_q4;            // LBUF EPSD (or_model_set_q4_params)
+  }
+  b += (int64_t)sizeof(DevTime) * 2 + 8 * 2 + 4 + 8 * (3 + 3 * 1024) + (int64_t)sizeof(ModelDev);
+  b += (int64_t)sizeof(Tile) * ((ne_hex + kBlock - 1) / kBlock + (ne_q4 + kBlock - 1) / kBlock + nsl);
+  // skyline rows in sliced-ELL form (or_model_upload_skyline): per slice, its largest row count
+  for (int f = 0; f < 2; ++f) {
+    const int32_t *rows = f == 0 ? rows_hex : rows_q4;
+    if ((f == 0 ? ne_hex : ne_q4) == 0) continue;
+    int64_t slots = 0;
+    for (int64_t s0 = 0; s0 < nn; s0 += kBlock) {
+      int32_t mx = 0;
+      for (int64_t n = s0; n < std::min(nn, s0 + kBlock); ++n) mx = std::max(mx, rows[n]);
+      slots += mx;
+    }
+    b += ss * slots * kBlock * (f == 0 ? kHexRowWords : kQ4RowWords);
+  }
+  *bytes = b;
   return OR_OK;
 }
 
@@ -1474,5 +1686,195 @@ int or_math_eval(int device, int fn, int64_t n, const double *x, const double *y
   if (s.e == hipSuccess) s.e = hipDeviceSynchronize();
   s.down(out, dout, n);
   if (s.e != hipSuccess) return fail(OR_ERR_HIP, std::string("or_math_eval: ") + hipGetErrorString(s.e));
+  return OR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches: many resident models in one launch per kernel and precision class.
+namespace {
+
+int read_time(or_model *m, int slot, DevTime &t) {
+  OR_HIP(hipMemcpy(&t, m->ts + slot, sizeof(DevTime), hipMemcpyDeviceToHost));
+  return OR_OK;
+}
+
+// Move a model's cycle scalars to the slot of the given parity, and set its stop flag
+// for the end time tend.
+int align_model(or_model *m, int64_t parity, double tend) {
+  const int from = (int)(m->parity & 1), to = (int)(parity & 1);
+  DevTime t;
+  int rc = read_time(m, from, t);
+  if (rc) return rc;
+  t.done = (t.mstop != 0 || t.tt >= tend) ? 1 : 0;
+  OR_HIP(hipMemcpy(m->ts + to, &t, sizeof(DevTime), hipMemcpyHostToDevice));
+  unsigned long long dm;
+  const double big = kEP06;
+  std::memcpy(&dm, &big, 8);
+  OR_HIP(hipMemcpy(m->dtmin + to, &dm, sizeof(dm), hipMemcpyHostToDevice));
+  m->parity = parity & 1;
+  return OR_OK;
+}
+
+int batch_rebuild(or_batch *b) {
+  if (!b->dirty) return OR_OK;
+  OR_HIP(hipStreamSynchronize(b->stream));
+  const int64_t ns = (int64_t)b->members.size();
+  if (ns > b->md_cap) {
+    if (b->md) OR_HIP(hipFree(b->md));
+    b->md = nullptr;
+    b->md_cap = 0;
+    OR_HIP(hipMalloc((void **)&b->md, sizeof(ModelDev) * (size_t)ns));
+    b->md_cap = ns;
+  }
+  std::vector<ModelDev> hd((size_t)ns);
+  std::vector<Tile> tl[2][3];
+  for (int64_t i = 0; i < ns; ++i) {
+    or_model *m = b->members[(size_t)i];
+    fill_dev(m, hd[(size_t)i], b->tend[(size_t)i]);
+    if (!b->live[(size_t)i]) continue;
+    const int c = m->prec == OR_PREC_FP64 ? 0 : 1;
+    const int64_t cnt[3] = {m->neh, m->neq, m->nn};
+    for (int f = 0; f < 3; ++f)
+      for (int64_t first = 0; first < cnt[f]; first += kBlock) tl[c][f].push_back(Tile{(int32_t)i, (int32_t)first});
+  }
+  if (ns > 0) OR_HIP(hipMemcpy(b->md, hd.data(), sizeof(ModelDev) * (size_t)ns, hipMemcpyHostToDevice));
+  for (int c = 0; c < 2; ++c)
+    for (int f = 0; f < 3; ++f) {
+      const int64_t n = (int64_t)tl[c][f].size();
+      if (n > b->tcap[c][f]) {
+        if (b->tiles[c][f]) OR_HIP(hipFree(b->tiles[c][f]));
+        b->tiles[c][f] = nullptr;
+        b->tcap[c][f] = 0;
+        OR_HIP(hipMalloc((void **)&b->tiles[c][f], sizeof(Tile) * (size_t)n));
+        b->tcap[c][f] = n;
+      }
+      if (n > 0) OR_HIP(hipMemcpy(b->tiles[c][f], tl[c][f].data(), sizeof(Tile) * (size_t)n, hipMemcpyHostToDevice));
+      b->nt[c][f] = (int32_t)n;
+    }
+  b->dirty = false;
+  return OR_OK;
+}
+
+} // namespace
+
+int or_batch_create(or_batch **out, int device) {
+  if (!out) return fail(OR_ERR_ARG, "null out");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(OR_ERR_NODEV, "no HIP device");
+  if (device < 0 || device >= ndev) return fail(OR_ERR_ARG, "device index out of range");
+  OR_HIP(hipSetDevice(device));
+  or_batch *b = new or_batch();
+  b->device = device;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return fail(OR_ERR_HIP, "hipStreamCreate");
+  }
+  *out = b;
+  return OR_OK;
+}
+
+int or_batch_destroy(or_batch *b) {
+  if (!b) return OR_OK;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  while (!b->members.empty()) (void)or_batch_remove(b, b->members.back());
+  if (b->md) (void)hipFree(b->md);
+  for (auto &row : b->tiles)
+    for (Tile *t : row)
+      if (t) (void)hipFree(t);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return OR_OK;
+}
+
+int or_batch_add(or_batch *b, or_model *m, double t_end) {
+  if (!b || !m) return fail(OR_ERR_ARG, "null argument");
+  if (m->batch) return fail(OR_ERR_STATE, "the model already belongs to a batch");
+  if (m->device != b->device) return fail(OR_ERR_ARG, "the model lives on another device");
+  if (!(t_end > 0.0)) return fail(OR_ERR_ARG, "t_end must be > 0");
+  if (!m->have_nodes || !m->have_sky || (m->neh > 0 && (!m->have_hparams || !m->have_hex)) ||
+      (m->neq > 0 && (!m->have_qparams || !m->have_q4)))
+    return fail(OR_ERR_STATE, "params, nodes, element state and skyline must be uploaded before add");
+  OR_HIP(hipSetDevice(b->device));
+  OR_HIP(hipStreamSynchronize(m->stream));
+  OR_HIP(hipStreamSynchronize(b->stream));
+  int rc = align_model(m, b->parity, t_end);
+  if (rc) return rc;
+  m->batch = b;
+  b->members.push_back(m);
+  b->tend.push_back(t_end);
+  b->live.push_back(1);
+  b->dirty = true;
+  return OR_OK;
+}
+
+int or_batch_remove(or_batch *b, or_model *m) {
+  if (!b || !m) return fail(OR_ERR_ARG, "null argument");
+  size_t i = 0;
+  while (i < b->members.size() && b->members[i] != m) ++i;
+  if (i == b->members.size()) return fail(OR_ERR_ARG, "the model is not in this batch");
+  OR_HIP(hipSetDevice(b->device));
+  OR_HIP(hipStreamSynchronize(b->stream));
+  b->members.erase(b->members.begin() + (ptrdiff_t)i);
+  b->tend.erase(b->tend.begin() + (ptrdiff_t)i);
+  b->live.erase(b->live.begin() + (ptrdiff_t)i);
+  m->batch = nullptr;
+  m->dev_dirty = true;
+  b->dirty = true;
+  // alone again: the model runs without an end time unless MSTOP was raised
+  return align_model(m, m->parity, kNoEnd);
+}
+
+int or_batch_run(or_batch *b, int64_t ncycles) {
+  if (!b) return fail(OR_ERR_ARG, "null batch");
+  if (ncycles < 0) return fail(OR_ERR_ARG, "ncycles < 0");
+  OR_HIP(hipSetDevice(b->device));
+  int rc = batch_rebuild(b);
+  if (rc) return rc;
+  LaunchSet L[2];
+  for (int c = 0; c < 2; ++c) {
+    L[c].md = b->md;
+    L[c].th = b->tiles[c][0]; L[c].nth = b->nt[c][0];
+    L[c].tq = b->tiles[c][1]; L[c].ntq = b->nt[c][1];
+    L[c].tn = b->tiles[c][2]; L[c].ntn = b->nt[c][2];
+  }
+  for (int64_t k = 0; k < ncycles; ++k) {
+    const int p = (int)(b->parity & 1);
+    if ((rc = launch_cycle<double, double>(b->stream, L[0], p, nullptr, nullptr))) return rc;
+    if ((rc = launch_cycle<float, float>(b->stream, L[1], p, nullptr, nullptr))) return rc;
+    b->parity ^= 1;
+    b->cycles += 1;
+  }
+  // members still in the tile lists follow the batch parity; stopped ones keep their slot
+  for (size_t i = 0; i < b->members.size(); ++i)
+    if (b->live[i]) b->members[i]->parity = b->parity & 1;
+  return OR_OK;
+}
+
+int or_batch_sync(or_batch *b) {
+  if (!b) return fail(OR_ERR_ARG, "null batch");
+  OR_HIP(hipSetDevice(b->device));
+  OR_HIP(hipStreamSynchronize(b->stream));
+  return OR_OK;
+}
+
+int or_batch_active(or_batch *b, int32_t *nactive) {
+  if (!b || !nactive) return fail(OR_ERR_ARG, "null argument");
+  OR_HIP(hipSetDevice(b->device));
+  OR_HIP(hipStreamSynchronize(b->stream));
+  int32_t n = 0;
+  for (size_t i = 0; i < b->members.size(); ++i) {
+    DevTime t;
+    int rc = read_time(b->members[i], (int)(b->members[i]->parity & 1), t);
+    if (rc) return rc;
+    const char alive = t.done ? 0 : 1;
+    if (alive != b->live[i]) {
+      b->live[i] = alive;
+      b->dirty = true; // stopped models leave the tile lists
+    }
+    n += alive;
+  }
+  *nactive = n;
   return OR_OK;
 }

@@ -33,14 +33,14 @@ struct DevTime {
   double tt, dt1, dt2, dt12, dt2old;
   int64_t ncycle;
   int32_t mstop;
-  int32_t pad;
+  int32_t done; // the model has stopped: end time reached or MSTOP
 };
 
 template <typename T> __device__ __forceinline__ T dmaxT(T a, T b) { return (a > b) ? a : b; }
 template <typename T> __device__ __forceinline__ T dminT(T a, T b) { return (a < b) ? a : b; }
 
-// FP64: the double-double routines of or_math.h, which round like the host C library
-// the reference engine calls; FP32 (split form): the device library.
+// FP64: the correctly rounded double-double routines of or_math.h; FP32 (split
+// form): the device library.
 __device__ __forceinline__ double powR(double a, double b) { return ormath::or_pow(a, b); }
 __device__ __forceinline__ float powR(float a, float b) { return powf(a, b); }
 __device__ __forceinline__ double sqrtR(double a) { return sqrt(a); }

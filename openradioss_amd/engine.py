@@ -188,6 +188,52 @@ class GpuHexModel(GpuModel):
         return super().energy()[:2]
 
 
+class GpuBatch:
+    """Many resident models advanced together (or_batch_*): each member runs its own cycles
+    with its own time step until its time reaches its t_end."""
+
+    def __init__(self, device: int = 0):
+        lib = L.load()
+        h = C.c_void_p()
+        L.check(lib.or_batch_create(C.byref(h), device))
+        self._h = h
+        self._lib = lib
+        self.members = []
+
+    def add(self, gm: GpuModel, t_end: float):
+        L.check(self._lib.or_batch_add(self._h, gm._h, float(t_end)))
+        self.members.append(gm)
+
+    def remove(self, gm: GpuModel):
+        L.check(self._lib.or_batch_remove(self._h, gm._h))
+        self.members.remove(gm)
+
+    def run(self, ncycles: int, sync: bool = True):
+        L.check(self._lib.or_batch_run(self._h, int(ncycles)))
+        if sync:
+            self.sync()
+
+    def sync(self):
+        L.check(self._lib.or_batch_sync(self._h))
+
+    def active(self) -> int:
+        n = C.c_int32()
+        L.check(self._lib.or_batch_active(self._h, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.or_batch_destroy(self._h)
+            self._h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def sforc3_group(params: L.OrHexParams, x, v, ixs, gbuf: dict, fsky, iads, nft: int, nel: int, dt1: float,
                  dt2t: float, device: int = 0):
     """SFORC3 on one group (reference layouts; see or_sforc3_group in include/or_gpu.h).

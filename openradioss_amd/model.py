@@ -613,3 +613,27 @@ def block_with_plate(n: int = 160, plate: int = 1000, h: float = 1.0, thick: flo
     v[:, 2] += v0
     return build_model(x, hex_conn=ch, hex_mat=STEEL, q4_conn=cq, q4_mat=STEEL,
                        q4_prop=ShellProp(thick=thick, npt=npt), v0=v)
+
+
+# ---------------------------------------------------------------------------
+# Parametric sweep of tubes (many uneven models advanced together)
+# ---------------------------------------------------------------------------
+SWEEP_SIZES = (1.0, 1.5, 2.0, 3.0, 4.0)
+
+
+def tube_sweep(n: int = 256, seed: int = 20261203, width: float = 60.0, length: float = 300.0,
+               sizes=SWEEP_SIZES) -> list:
+    """n variants of the square-tube crush: element size uniform on `sizes` (mm), thickness
+    U(1.0, 2.5) mm, crush velocity U(5, 20) m/s, yield A U(150, 300) MPa (aluminium otherwise),
+    drawn in that order per variant from one PCG64 stream.  Returns [(Model, params dict)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        h = float(sizes[int(rng.integers(0, len(sizes)))])
+        thick = float(rng.uniform(1.0, 2.5))
+        vel = float(rng.uniform(5.0e3, 2.0e4))
+        a = float(rng.uniform(150.0, 300.0))
+        mat = dataclasses.replace(ALUMINIUM, a=a)
+        m = square_tube(width=width, length=length, h=h, thick=thick, v_top=-vel, mat=mat)
+        out.append((m, dict(h=h, thick=thick, v=vel, a=a)))
+    return out

@@ -73,3 +73,21 @@ def test_group_dropins_reject_bad_input_before_any_device_call():
     assert lib.or_cforc3_group(0, C.byref(qp), 4, 1, 0, *args, 4, np.zeros(32).ctypes.data_as(C.c_void_p),
                                iadc.ctypes.data_as(C.c_void_p), 1.0, C.byref(dt)) == L.OR_ERR_ARG
     assert b"NPT" in lib.or_last_error()
+
+
+def test_footprint_estimate_needs_no_device():
+    """Admission by footprint is computed on the host: no device call, exact row-slot count."""
+    from openradioss_amd import model as M
+    from openradioss_amd import serve as S
+    m = M.square_tube(width=12.0, length=24.0)
+    f64, f32 = S.footprint(m, "fp64"), S.footprint(m, "split")
+    assert 0 < f32 < f64
+    # element state and skyline rows halve in the split form, nodes stay FP64
+    nodes = m.nn * (8 * 13 + 4 + 8 * 5)
+    assert f64 - f32 > 0.3 * (f64 - nodes)
+    b = M.hex_model_to_model(M.cantilever_bar(4, 4, 16))
+    assert S.footprint(b, "fp64") > 0
+    L = __import__("openradioss_amd.lib", fromlist=["load"])
+    import ctypes as C
+    out = C.c_int64()
+    assert L.load().or_model_footprint_estimate(10, 1, 0, 0, 0, 0, None, None, C.byref(out)) == L.OR_ERR_ARG
