@@ -118,12 +118,11 @@ def sweep_leg(precision: str, cycles: int, warmup: int, n: int = 256):
 def serve_leg(nreq: int, rate: float, time_scale: float):
     """Served crash solves: Poisson arrivals, bars and tubes of 5k-500k elements, FP64 / split,
     end times U(1, 10) ms x time_scale, admitted by HBM footprint into one running batch."""
-    import torch
-
+    from openradioss_amd import lib as L
     from openradioss_amd import serve as S
 
     reqs = S.request_stream(nreq, rate=rate, seed=SEED, time_scale=time_scale)
-    free, total = torch.cuda.mem_get_info(0)
+    free, total = L.device_memory(0)
     budget = int(0.8 * free)
     srv = S.Server(hbm_budget=budget, cycles_per_step=50, keep_results=False)
     st = srv.serve(reqs)

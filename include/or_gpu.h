@@ -102,6 +102,9 @@ typedef struct or_model or_model;
 int or_version(void);
 const char *or_last_error(void);
 int or_device_count(int *count);
+/* Free and total device memory in bytes (no counterpart in the reference: the serving
+ * loop sizes its HBM budget with it). */
+int or_device_memory(int device, int64_t *free_bytes, int64_t *total_bytes);
 
 /* ---- resident model (upload once, step N cycles, state stays in HBM) ---- */
 /* nn nodes, ne_hex 8-node hexes, ne_q4 Q4 shells with npt through-thickness points,

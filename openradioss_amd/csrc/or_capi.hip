@@ -95,22 +95,30 @@ __device__ __forceinline__ int64_t xcd_contiguous_block(int64_t b, int64_t nb) {
   return x * q + (x < r ? x : r) + j;
 }
 
+// Pointers into HBM as the kernels use them: typed in the global address space, so
+// that their loads and stores compile to global (not flat) memory instructions even
+// though the kernels read them from a descriptor in memory.
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+typedef double v2d __attribute__((ext_vector_type(2))); // 16-byte load / store unit
+template <typename T> __device__ __forceinline__ gptr<T> to_global(T *p) { return (gptr<T>)p; }
+template <typename T> __device__ __forceinline__ gptr<T> to_global(const void *p) { return (gptr<T>)(T *)p; }
+
 template <typename S> struct HexPtrs {
-  const int32_t *conn;  // [8][ne]
-  const int32_t *iads;  // [8][ne] hex-row index
-  S *sig;               // [6][ne]
-  S *eint, *rho, *pla, *epsd, *offg, *qvis;
-  const S *vol0;
+  gptr<const int32_t> conn;  // [8][ne]
+  gptr<const int32_t> iads;  // [8][ne] row position
+  gptr<S> sig;               // [6][ne]
+  gptr<S> eint, rho, pla, epsd, offg, qvis;
+  gptr<const S> vol0;
   int64_t ne;
 };
 
 template <typename S> struct Q4Ptrs {
-  const int32_t *conn;  // [4][ne]
-  const int32_t *iadc;  // [4][ne] Q4-row index
-  S *offg, *thk;
-  const S *thke;
-  S *forc, *mom, *gstr, *hourg, *eint, *epsd; // [5][ne] [3][ne] [8][ne] [5][ne] [2][ne] [ne]
-  S *sig, *pla, *epsd_ip;                     // [npt][5][ne] [npt][ne] [npt][ne] (epsd_ip null when israte=0)
+  gptr<const int32_t> conn;  // [4][ne]
+  gptr<const int32_t> iadc;  // [4][ne] row position
+  gptr<S> offg, thk;
+  gptr<const S> thke;
+  gptr<S> forc, mom, gstr, hourg, eint, epsd; // [5][ne] [3][ne] [8][ne] [5][ne] [2][ne] [ne]
+  gptr<S> sig, pla, epsd_ip;                  // [npt][5][ne] [npt][ne] [npt][ne] (epsd_ip null when israte=0)
   int64_t ne;
 };
 
@@ -159,13 +167,15 @@ struct Tile {
 };
 
 template <typename S> __device__ __forceinline__ HexPtrs<S> hex_ptrs(const ModelDev &M) {
-  return HexPtrs<S>{M.hconn, M.hiads, (S *)M.hsig, (S *)M.heint, (S *)M.hrho, (S *)M.hpla, (S *)M.hepsd,
-                    (S *)M.hoffg, (S *)M.hqvis, (const S *)M.hvol0, M.neh};
+  return HexPtrs<S>{to_global(M.hconn), to_global(M.hiads), to_global<S>(M.hsig), to_global<S>(M.heint),
+                    to_global<S>(M.hrho), to_global<S>(M.hpla), to_global<S>(M.hepsd), to_global<S>(M.hoffg),
+                    to_global<S>(M.hqvis), to_global<const S>(M.hvol0), M.neh};
 }
 template <typename S> __device__ __forceinline__ Q4Ptrs<S> q4_ptrs(const ModelDev &M) {
-  return Q4Ptrs<S>{M.qconn, M.qiadc, (S *)M.qoffg, (S *)M.qthk, (const S *)M.qthke, (S *)M.qforc, (S *)M.qmom,
-                   (S *)M.qgstr, (S *)M.qhourg, (S *)M.qeint, (S *)M.qepsd, (S *)M.qsig, (S *)M.qpla,
-                   (S *)M.qepsdip, M.neq};
+  return Q4Ptrs<S>{to_global(M.qconn), to_global(M.qiadc), to_global<S>(M.qoffg), to_global<S>(M.qthk),
+                   to_global<const S>(M.qthke), to_global<S>(M.qforc), to_global<S>(M.qmom), to_global<S>(M.qgstr),
+                   to_global<S>(M.qhourg), to_global<S>(M.qeint), to_global<S>(M.qepsd), to_global<S>(M.qsig),
+                   to_global<S>(M.qpla), to_global<S>(M.qepsdip), M.neq};
 }
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
@@ -187,10 +197,10 @@ __device__ __forceinline__ double block_min(double v) {
   return v;
 }
 
-__device__ __forceinline__ void load_node(const double *xv, int64_t n, double &x, double &y, double &z, double &vx,
-                                          double &vy, double &vz) {
-  const double2 *p = reinterpret_cast<const double2 *>(xv + 6 * n);
-  double2 a = p[0], b = p[1], q = p[2];
+__device__ __forceinline__ void load_node(gptr<const double> xv, int64_t n, double &x, double &y, double &z,
+                                          double &vx, double &vy, double &vz) {
+  gptr<const v2d> p = (gptr<const v2d>)(xv + 6 * n);
+  const v2d a = p[0], b = p[1], q = p[2];
   x = a.x; y = a.y; z = b.x; vx = b.y; vy = q.x; vz = q.y;
 }
 
@@ -203,8 +213,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(hex_wave
   const DevTime *ts = M.ts;
   if (ts[parity].done) return;
   const HexPtrs<S> e = hex_ptrs<S>(M);
-  const double *__restrict__ xv = M.nd.xv;
-  S *__restrict__ sky = (S *)M.skh;
+  const gptr<const double> xv = to_global<const double>(M.nd.xv);
+  const gptr<S> sky = to_global<S>(M.skh);
   const HexConsts &c = M.hc;
   const int64_t i = (int64_t)tl.first + threadIdx.x;
   double dtx = kEP06;
@@ -233,7 +243,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(hex_wave
     e.epsd[i] = st.epsd; e.offg[i] = st.offg; e.qvis[i] = st.qvis;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      S *r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
+      const gptr<S> r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
       r[0] = S(o.f1[k]); r[kBlock] = S(o.f2[k]); r[2 * kBlock] = S(o.f3[k]);
     }
     dtx = o.dtx;
@@ -249,14 +259,14 @@ template <typename R, typename S> struct Q4IpIO {
   const Q4Ptrs<S> &q;
   int64_t i;
   __device__ __forceinline__ void load(int ip, R (&s)[5], R &pla, R &epsd_ip) const {
-    const S *sg = q.sig + (int64_t)ip * 5 * q.ne + i;
+    const gptr<const S> sg = q.sig + (int64_t)ip * 5 * q.ne + i;
 #pragma unroll
     for (int k = 0; k < 5; ++k) s[k] = R(sg[(int64_t)k * q.ne]);
     pla = R(q.pla[(int64_t)ip * q.ne + i]);
     epsd_ip = q.epsd_ip ? R(q.epsd_ip[(int64_t)ip * q.ne + i]) : R(0);
   }
   __device__ __forceinline__ void store(int ip, const R (&s)[5], R pla, R epsd_ip) const {
-    S *sg = q.sig + (int64_t)ip * 5 * q.ne + i;
+    const gptr<S> sg = q.sig + (int64_t)ip * 5 * q.ne + i;
 #pragma unroll
     for (int k = 0; k < 5; ++k) sg[(int64_t)k * q.ne] = S(s[k]);
     q.pla[(int64_t)ip * q.ne + i] = S(pla);
@@ -272,9 +282,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(q4_waves
   const DevTime *ts = M.ts;
   if (ts[parity].done) return;
   const Q4Ptrs<S> q = q4_ptrs<S>(M);
-  const double *__restrict__ xv = M.nd.xv;
-  const double *__restrict__ vr = M.nd.vr;
-  S *__restrict__ sky = (S *)M.skq;
+  const gptr<const double> xv = to_global<const double>(M.nd.xv);
+  const gptr<const double> vr = to_global<const double>(M.nd.vr);
+  const gptr<S> sky = to_global<S>(M.skq);
   const Q4Consts &c = M.qc;
   const int64_t i = (int64_t)tl.first + threadIdx.x;
   double dt = kEP06;
@@ -290,8 +300,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(q4_waves
       load_node(xv, nc[k], a, b, cc, u, v, w);
       xg[k] = a; yg[k] = b; zg[k] = cc;
       vl[k][0] = R(u); vl[k][1] = R(v); vl[k][2] = R(w);
-      const double2 *p = reinterpret_cast<const double2 *>(vr + 4 * (int64_t)nc[k]);
-      double2 r0 = p[0], r1 = p[1];
+      const gptr<const v2d> p = (gptr<const v2d>)(vr + 4 * (int64_t)nc[k]);
+      const v2d r0 = p[0], r1 = p[1];
       vrl[k][0] = R(r0.x); vrl[k][1] = R(r0.y); vrl[k][2] = R(r1.x);
     }
     Q4Elem<R> st;
@@ -320,7 +330,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(q4_waves
     q.eint[i] = S(st.eint[0]); q.eint[q.ne + i] = S(st.eint[1]);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      S *r = sky + row_word(q.iadc[(int64_t)k * q.ne + i], kQ4RowWords);
+      const gptr<S> r = sky + row_word(q.iadc[(int64_t)k * q.ne + i], kQ4RowWords);
       r[0] = S(-o.f[0][k]); r[kBlock] = S(-o.f[1][k]); r[2 * kBlock] = S(-o.f[2][k]);
       r[3 * kBlock] = S(-o.m[0][k]); r[4 * kBlock] = S(-o.m[1][k]); r[5 * kBlock] = S(-o.m[2][k]);
     }
@@ -338,8 +348,8 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
   const Tile tl = tiles[blockIdx.x];
   const ModelDev &M = md[tl.model];
   const NodePtrs &nd = M.nd;
-  const S *__restrict__ skh = (const S *)M.skh;
-  const S *__restrict__ skq = (const S *)M.skq;
+  const gptr<const S> skh = to_global<const S>(M.skh);
+  const gptr<const S> skq = to_global<const S>(M.skq);
   DevTime *ts = M.ts;
   unsigned long long *dtmin = M.dtmin;
   const double dtmx = M.dtmx;
@@ -382,50 +392,50 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
   double ax = 0.0, ay = 0.0, az = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
   const int lane = (int)(n % kBlock);
   if (skh) {
-    const int32_t deg = nd.deg_h[n];
-    const S *r = skh + (int64_t)nd.slab_h[n / kBlock] * kHexRowWords * kBlock + lane;
+    const int32_t deg = to_global(nd.deg_h)[n];
+    gptr<const S> r = skh + (int64_t)to_global(nd.slab_h)[n / kBlock] * kHexRowWords * kBlock + lane;
     for (int32_t j = 0; j < deg; ++j, r += kHexRowWords * kBlock) {
       ax = ax + double(r[0]); ay = ay + double(r[kBlock]); az = az + double(r[2 * kBlock]);
     }
   }
   bool rot = false; // node carries rotational DOF work this cycle
   if (skq) {
-    const int32_t deg = nd.deg_q[n];
+    const int32_t deg = to_global(nd.deg_q)[n];
     rot = deg > 0;
-    const S *r = skq + (int64_t)nd.slab_q[n / kBlock] * kQ4RowWords * kBlock + lane;
+    gptr<const S> r = skq + (int64_t)to_global(nd.slab_q)[n / kBlock] * kQ4RowWords * kBlock + lane;
     for (int32_t j = 0; j < deg; ++j, r += kQ4RowWords * kBlock) {
       ax = ax + double(r[0]); ay = ay + double(r[kBlock]); az = az + double(r[2 * kBlock]);
       rx = rx + double(r[3 * kBlock]); ry = ry + double(r[4 * kBlock]); rz = rz + double(r[5 * kBlock]);
     }
   }
   // ACCELE (accele.F:65-80, 97-109)
-  const double msn = nd.ms[n];
+  const double msn = to_global(nd.ms)[n];
   if (msn > 0.0) {
     const double rtmp = 1.0 / msn;
     ax = ax * rtmp; ay = ay * rtmp; az = az * rtmp;
   } else {
     ax = 0.0; ay = 0.0; az = 0.0;
   }
-  const int32_t code = nd.code[n];
+  const int32_t code = to_global(nd.code)[n];
   // A node without shell rows and without rotational BCs has AR = 0, so VELOCITY leaves
   // its VR as it is: its VR and IN are not touched.
   rot = rot || (code & (kRX | kRY | kRZ));
   double wx = 0.0, wy = 0.0, wz = 0.0;
-  double2 *pr = nullptr;
+  gptr<v2d> pr = nullptr;
   if (nd.vr && rot) {
-    const double inn = nd.in[n];
+    const double inn = to_global(nd.in)[n];
     if (inn > 0.0) {
       const double rtmp = 1.0 / inn;
       rx = rx * rtmp; ry = ry * rtmp; rz = rz * rtmp;
     } else {
       rx = 0.0; ry = 0.0; rz = 0.0;
     }
-    pr = reinterpret_cast<double2 *>(nd.vr + 4 * n);
-    double2 r0 = pr[0], r1 = pr[1];
+    pr = (gptr<v2d>)to_global(nd.vr + 4 * n);
+    const v2d r0 = pr[0], r1 = pr[1];
     wx = r0.x; wy = r0.y; wz = r1.x;
   }
-  double2 *p = reinterpret_cast<double2 *>(nd.xv + 6 * n);
-  double2 pa = p[0], pb = p[1], pc = p[2];
+  const gptr<v2d> p = (gptr<v2d>)to_global(nd.xv + 6 * n);
+  const v2d pa = p[0], pb = p[1], pc = p[2];
   double x = pa.x, y = pa.y, z = pb.x, vx = pb.y, vy = pc.x, vz = pc.y;
   if (code) {
     // BCS1V global frame (bcs1.F)
@@ -436,26 +446,27 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
     if (code & kRY) { ry = 0.0; wy = 0.0; }
     if (code & kRZ) { rz = 0.0; wz = 0.0; }
     // FIXVEL global frame (fixvel.F:366-379): A = (VIMP - V)/DT12
-    if (code & kIX) ax = (nd.vimp[3 * n + 0] - vx) / DT12;
-    if (code & kIY) ay = (nd.vimp[3 * n + 1] - vy) / DT12;
-    if (code & kIZ) az = (nd.vimp[3 * n + 2] - vz) / DT12;
+    const gptr<const double> vimp = to_global(nd.vimp) + 3 * n;
+    if (code & kIX) ax = (vimp[0] - vx) / DT12;
+    if (code & kIY) ay = (vimp[1] - vy) / DT12;
+    if (code & kIZ) az = (vimp[2] - vz) / DT12;
   }
   // VELOCITY (velocity.F:58-86)
   vx = vx + DT12 * ax; vy = vy + DT12 * ay; vz = vz + DT12 * az;
   if (pr) {
     wx = wx + DT12 * rx; wy = wy + DT12 * ry; wz = wz + DT12 * rz;
-    pr[0] = make_double2(wx, wy);
-    pr[1] = make_double2(wz, 0.0);
+    pr[0] = v2d{wx, wy};
+    pr[1] = v2d{wz, 0.0};
   }
   // DEPLA (displacement.F:108-118, IRESP=0 form in FP64)
-  double *dd = nd.d + 3 * n;
+  const gptr<double> dd = to_global(nd.d) + 3 * n;
   double vdt;
   vdt = DT2 * vx; dd[0] = dd[0] + vdt; x = x + vdt;
   vdt = DT2 * vy; dd[1] = dd[1] + vdt; y = y + vdt;
   vdt = DT2 * vz; dd[2] = dd[2] + vdt; z = z + vdt;
-  p[0] = make_double2(x, y);
-  p[1] = make_double2(z, vx);
-  p[2] = make_double2(vy, vz);
+  p[0] = v2d{x, y};
+  p[1] = v2d{z, vx};
+  p[2] = v2d{vy, vz};
 }
 
 // ---------------------------------------------------------------- energies (fixed-order)
@@ -831,6 +842,19 @@ int or_device_count(int *count) {
   hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess) { *count = 0; return fail(OR_ERR_NODEV, hipGetErrorString(e)); }
   *count = n;
+  return OR_OK;
+}
+
+int or_device_memory(int device, int64_t *free_bytes, int64_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(OR_ERR_ARG, "null argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(OR_ERR_NODEV, "no HIP device");
+  if (device < 0 || device >= ndev) return fail(OR_ERR_ARG, "device index out of range");
+  OR_HIP(hipSetDevice(device));
+  size_t f = 0, t = 0;
+  OR_HIP(hipMemGetInfo(&f, &t));
+  *free_bytes = (int64_t)f;
+  *total_bytes = (int64_t)t;
   return OR_OK;
 }
 

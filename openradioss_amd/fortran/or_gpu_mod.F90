@@ -12,12 +12,15 @@ module or_gpu_mod
   implicit none
   private
   public :: or_hex_params_t, or_q4_params_t, or_time_state_t
-  public :: or_version, or_device_count, or_error_message
+  public :: or_version, or_device_count, or_device_memory, or_error_message
   public :: or_model_create, or_model_destroy, or_model_set_hex_params, or_model_set_q4_params
   public :: or_model_upload_nodes, or_model_upload_hex, or_model_upload_q4, or_model_upload_skyline
   public :: or_model_set_time, or_model_get_time, or_model_run, or_model_sync
   public :: or_model_download_nodes, or_model_download_hex, or_model_download_q4
   public :: or_model_energy, or_model_kernel_times, or_model_set_timing, or_model_footprint
+  public :: or_model_footprint_estimate
+  public :: or_batch_create, or_batch_destroy, or_batch_add, or_batch_remove, or_batch_run, or_batch_sync
+  public :: or_batch_active
   public :: or_sforc3_group, or_cforc3_group, or_math_eval
 
   integer(c_int), parameter, public :: OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_HIP = 2, OR_ERR_NODEV = 3, &
@@ -65,6 +68,12 @@ module or_gpu_mod
     integer(c_int) function or_device_count(count) bind(c, name="or_device_count")
       import :: c_int
       integer(c_int), intent(out) :: count
+    end function
+
+    integer(c_int) function or_device_memory(device, free_bytes, total_bytes) bind(c, name="or_device_memory")
+      import :: c_int, c_int64_t
+      integer(c_int), value :: device
+      integer(c_int64_t), intent(out) :: free_bytes, total_bytes
     end function
 
     integer(c_int) function or_model_create(handle, device, nn, ne_hex, ne_q4, npt, precision) &
@@ -183,6 +192,56 @@ module or_gpu_mod
       import :: c_int, c_ptr, c_int64_t
       type(c_ptr), value :: handle
       integer(c_int64_t), intent(out) :: nbytes
+    end function
+
+    integer(c_int) function or_model_footprint_estimate(nn, ne_hex, ne_q4, npt, precision, strain_rate_filter, &
+        rows_hex, rows_q4, nbytes) bind(c, name="or_model_footprint_estimate")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      integer(c_int64_t), value :: nn, ne_hex, ne_q4
+      integer(c_int32_t), value :: npt
+      integer(c_int), value :: precision, strain_rate_filter
+      type(c_ptr), value :: rows_hex, rows_q4
+      integer(c_int64_t), intent(out) :: nbytes
+    end function
+
+    ! batches: many resident models advanced together, each to its own end time
+    integer(c_int) function or_batch_create(batch, device) bind(c, name="or_batch_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: batch
+      integer(c_int), value :: device
+    end function
+
+    integer(c_int) function or_batch_destroy(batch) bind(c, name="or_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
+    integer(c_int) function or_batch_add(batch, handle, t_end) bind(c, name="or_batch_add")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch, handle
+      real(c_double), value :: t_end
+    end function
+
+    integer(c_int) function or_batch_remove(batch, handle) bind(c, name="or_batch_remove")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch, handle
+    end function
+
+    integer(c_int) function or_batch_run(batch, ncycles) bind(c, name="or_batch_run")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int64_t), value :: ncycles
+    end function
+
+    integer(c_int) function or_batch_sync(batch) bind(c, name="or_batch_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
+    integer(c_int) function or_batch_active(batch, nactive) bind(c, name="or_batch_active")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int32_t), intent(out) :: nactive
     end function
 
     ! SFORC3 on one group (sforc3.F:131-166 argument meaning): GBUF%SIG, EINT, RHO, VOL, PLA,

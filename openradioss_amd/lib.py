@@ -23,7 +23,7 @@ OR_PREC_SPLIT = 1
 
 # Every symbol declared in include/or_gpu.h.
 EXPORTS = (
-    "or_version", "or_last_error", "or_device_count", "or_model_create", "or_model_destroy",
+    "or_version", "or_last_error", "or_device_count", "or_device_memory", "or_model_create", "or_model_destroy",
     "or_model_set_hex_params", "or_model_set_q4_params", "or_model_upload_nodes", "or_model_upload_hex",
     "or_model_upload_q4", "or_model_upload_skyline", "or_model_set_time", "or_model_get_time", "or_model_run",
     "or_model_sync", "or_model_download_nodes", "or_model_download_hex", "or_model_download_q4",
@@ -91,6 +91,7 @@ _SIGS = {
     "or_cforc3_group": ([C.c_int, _P, C.c_int64, C.c_int32, C.c_int64] + [_P] * 16 + [C.c_int64, _P, _P,
                                                                                       C.c_double, _P], C.c_int),
     "or_math_eval": ([C.c_int, C.c_int, C.c_int64, _P, _P, _P], C.c_int),
+    "or_device_memory": ([C.c_int, _P, _P], C.c_int),
     "or_batch_create": ([_P, C.c_int], C.c_int),
     "or_batch_destroy": ([_P], C.c_int),
     "or_batch_add": ([_P, _P, C.c_double], C.c_int),
@@ -120,6 +121,13 @@ def load():
 def check(rc: int):
     if rc != OR_OK:
         raise OrError(rc, load().or_last_error().decode())
+
+
+def device_memory(device: int = 0) -> tuple[int, int]:
+    """(free, total) bytes of device memory."""
+    f, t = C.c_int64(), C.c_int64()
+    check(load().or_device_memory(device, C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def device_count() -> int:
