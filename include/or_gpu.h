@@ -140,6 +140,11 @@ int or_model_get_time(or_model *m, or_time_state *ts);
 /* Run ncycles cycles asynchronously on the model's stream. */
 int or_model_run(or_model *m, int64_t ncycles);
 int or_model_sync(or_model *m);
+/* One cycle with ECRIT's energy output (ecrit.F:193-327 at its call point resol.F:8523,
+ * between ACCELE/BCS10/FIXVEL and VELOCITY): out[0] internal energy (SBILAN EINT*VOL,
+ * CBILAN EINT(1)+EINT(2)), out[1] translational kinetic sum (V+DT05*A)^2*HALF*MS,
+ * out[2] rotational kinetic sum (VR+DT05*AR)^2*HALF*IN, DT05 = HALF*DT1.  Synchronous. */
+int or_model_run_output(or_model *m, double out[3]);
 /* Any output pointer may be NULL. */
 int or_model_download_nodes(or_model *m, double *x, double *v, double *vr, double *d);
 int or_model_download_hex(or_model *m, double *sig, double *eint, double *rho, double *pla,

@@ -154,6 +154,7 @@ struct ModelDev {
   DevTime *ts;                // [2] cycle scalars, by cycle parity
   unsigned long long *dtmin;  // [2] element time-step minimum (FP64 bits), by parity
   int *flags;                 // bit 0: an element reported MSTOP
+  double *ecrit;              // [2][nn] per-node kinetic terms of ECRIT on output cycles, or null
   double dtmx;                // DTMX
   double tend;                // the model stops after the cycle whose time reaches tend
   HexConsts hc;
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(q4_waves
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restrict__ md,
                                                         const Tile *__restrict__ tiles, int parity,
-                                                        double dt_growth) {
+                                                        double dt_growth, int output) {
   const Tile tl = tiles[blockIdx.x];
   const ModelDev &M = md[tl.model];
   const NodePtrs &nd = M.nd;
@@ -451,6 +452,28 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
     if (code & kIY) ay = (vimp[1] - vy) / DT12;
     if (code & kIZ) az = (vimp[2] - vz) / DT12;
   }
+  if (output && M.ecrit) {
+    // ECRIT's kinetic energies at its output point, between FIXVEL and VELOCITY
+    // (sortie_main.F:368 from resol.F:8523): V + DT05*A with DT05 = HALF*DT1 (ecrit.F:193,
+    // 221-225, 319-324; MAS = MS*WEIGHT_MD, WEIGHT_MD = 1 on one domain)
+    const double DT05 = (1.0 / 2.0) * DT1;
+    const double ux = vx + DT05 * ax, uy = vy + DT05 * ay, uz = vz + DT05 * az;
+    const gptr<double> ke = to_global(M.ecrit);
+    ke[n] = (ux * ux + uy * uy + uz * uz) * (1.0 / 2.0) * msn;
+    double kr = 0.0;
+    if (nd.vr) {
+      double qx = wx, qy = wy, qz = wz, inn = 0.0;
+      if (!pr) { // rotational work skipped: AR = 0, VR as stored
+        const gptr<const v2d> q = (gptr<const v2d>)to_global(nd.vr + 4 * n);
+        const v2d q0 = q[0], q1 = q[1];
+        qx = q0.x; qy = q0.y; qz = q1.x;
+      }
+      inn = to_global(nd.in)[n];
+      const double sx = qx + DT05 * rx, sy = qy + DT05 * ry, sz = qz + DT05 * rz;
+      kr = (sx * sx + sy * sy + sz * sz) * (1.0 / 2.0) * inn;
+    }
+    ke[nd.nn + n] = kr;
+  }
   // VELOCITY (velocity.F:58-86)
   vx = vx + DT12 * ax; vy = vy + DT12 * ay; vz = vz + DT12 * az;
   if (pr) {
@@ -487,13 +510,18 @@ template <typename S>
 __global__ __launch_bounds__(kBlock) void k_energy_partial(const S *heint, const S *hvol0, int64_t neh,
                                                            const S *qeint, int64_t neq, const double *xv,
                                                            const double *vr, const double *ms, const double *in,
-                                                           int64_t nn, int64_t per_block, double *partial) {
+                                                           const double *ecrit, int64_t nn, int64_t per_block,
+                                                           double *partial) {
   double ie = 0.0, ke = 0.0, kr = 0.0;
   const int64_t b0 = (int64_t)blockIdx.x * per_block;
   for (int64_t j = b0 + threadIdx.x; j < b0 + per_block; j += kBlock) {
     if (j < neh) ie = ie + double(heint[j]) * double(hvol0[j]);
     if (j < neq) ie = ie + (double(qeint[j]) + double(qeint[neq + j]));
-    if (j < nn) {
+    if (j < nn && ecrit) {
+      // per-node terms written by the node kernel on an output cycle
+      ke = ke + ecrit[j];
+      kr = kr + ecrit[nn + j];
+    } else if (j < nn) {
       const double *p = xv + 6 * j;
       ke = ke + ms[j] * (p[3] * p[3] + p[4] * p[4] + p[5] * p[5]);
       if (vr) {
@@ -512,7 +540,8 @@ __global__ __launch_bounds__(kBlock) void k_energy_partial(const S *heint, const
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_energy_final(const double *partial, int nb, double *out) {
+__global__ __launch_bounds__(kBlock) void k_energy_final(const double *partial, int nb, double kin_factor,
+                                                         double *out) {
   double ie = 0.0, ke = 0.0, kr = 0.0;
   for (int j = threadIdx.x; j < nb; j += kBlock) {
     ie = ie + partial[3 * j];
@@ -524,8 +553,8 @@ __global__ __launch_bounds__(kBlock) void k_energy_final(const double *partial, 
   kr = block_sum(kr);
   if (threadIdx.x == 0) {
     out[0] = ie;
-    out[1] = 0.5 * ke;
-    out[2] = 0.5 * kr;
+    out[1] = kin_factor * ke;
+    out[2] = kin_factor * kr;
   }
 }
 
@@ -779,6 +808,7 @@ struct or_model {
   unsigned long long *dtmin = nullptr; // [2]
   int *flags = nullptr;
   double *energy = nullptr;            // [3 + 3*nb]
+  double *ecrit = nullptr;             // [2][nn] per-node ECRIT terms, allocated at the first output cycle
   int64_t parity = 0;
   // what the kernels see (host copy, device copy) and the model's own tile lists
   ModelDev hdev{};
@@ -980,7 +1010,7 @@ int or_model_destroy(or_model *m) {
                   m->hsig, m->heint, m->hrho, m->hvol0, m->hpla, m->hepsd, m->hoffg, m->hqvis, m->skh, m->qconn,
                   m->qiadc, m->qoffg, m->qthk, m->qthke, m->qforc, m->qmom, m->qgstr, m->qhourg, m->qeint, m->qepsd,
                   m->qsig, m->qpla, m->qepsdip, m->skq, m->ts, m->dtmin, m->flags, m->energy, m->dev,
-                  m->tiles_h, m->tiles_q, m->tiles_n};
+                  m->tiles_h, m->tiles_q, m->tiles_n, m->ecrit};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
@@ -1307,7 +1337,7 @@ static void fill_dev(or_model *m, ModelDev &d, double tend) {
                   m->nn};
   d.skh = m->neh > 0 ? m->skh : nullptr;
   d.skq = m->neq > 0 ? m->skq : nullptr;
-  d.ts = m->ts; d.dtmin = m->dtmin; d.flags = m->flags;
+  d.ts = m->ts; d.dtmin = m->dtmin; d.flags = m->flags; d.ecrit = m->ecrit;
   d.dtmx = m->neh > 0 ? m->hc.dtmx : 1.0e20;
   d.tend = tend;
   d.hc = m->hc;
@@ -1332,7 +1362,8 @@ struct LaunchSet {
 };
 
 template <typename R, typename S>
-static int launch_cycle(hipStream_t st, const LaunchSet &L, int parity, hipEvent_t after_hex, hipEvent_t after_q4) {
+static int launch_cycle(hipStream_t st, const LaunchSet &L, int parity, hipEvent_t after_hex, hipEvent_t after_q4,
+                        int output = 0) {
   if (L.nth > 0) {
     hipLaunchKernelGGL((k_hex_forces<R, S>), dim3((unsigned)L.nth), dim3(kBlock), 0, st, L.md, L.th, parity);
     OR_HIP(hipGetLastError());
@@ -1345,7 +1376,7 @@ static int launch_cycle(hipStream_t st, const LaunchSet &L, int parity, hipEvent
   if (after_q4) OR_HIP(hipEventRecord(after_q4, st));
   if (L.ntn > 0) {
     hipLaunchKernelGGL((k_node_update<S>), dim3((unsigned)L.ntn), dim3(kBlock), 0, st, L.md, L.tn, parity,
-                       (double)1.1f);
+                       (double)1.1f, output);
     OR_HIP(hipGetLastError());
   }
   return OR_OK;
@@ -1387,6 +1418,36 @@ int or_model_run(or_model *m, int64_t ncycles) {
     }
   }
   return OR_OK;
+}
+
+static int energy_sums(or_model *m, const double *ecrit, double out[3]);
+
+int or_model_run_output(or_model *m, double out[3]) {
+  if (!m || !out) return fail(OR_ERR_ARG, "null argument");
+  if (!m->have_nodes || !m->have_sky || (m->neh > 0 && (!m->have_hparams || !m->have_hex)) ||
+      (m->neq > 0 && (!m->have_qparams || !m->have_q4)))
+    return fail(OR_ERR_STATE, "params, nodes, element state and skyline must be uploaded before run");
+  if (m->batch) return fail(OR_ERR_STATE, "the model belongs to a batch: advance it with or_batch_run");
+  OR_HIP(hipSetDevice(m->device));
+  if (!m->ecrit) {
+    OR_HIP(quiesce(m));
+    OR_HIP(hipMalloc((void **)&m->ecrit, sizeof(double) * 2 * (size_t)m->nn));
+    m->bytes += (int64_t)(sizeof(double) * 2 * m->nn);
+    mark_changed(m);
+  }
+  int rc = refresh_dev(m);
+  if (rc) return rc;
+  LaunchSet L;
+  L.md = m->dev;
+  L.th = m->tiles_h; L.nth = m->nt_h;
+  L.tq = m->tiles_q; L.ntq = m->nt_q;
+  L.tn = m->tiles_n; L.ntn = m->nt_n;
+  const int p = (int)(m->parity & 1);
+  rc = (m->prec == OR_PREC_FP64) ? launch_cycle<double, double>(m->stream, L, p, nullptr, nullptr, 1)
+                                 : launch_cycle<float, float>(m->stream, L, p, nullptr, nullptr, 1);
+  if (rc) return rc;
+  m->parity ^= 1;
+  return energy_sums(m, m->ecrit, out);
 }
 
 int or_model_sync(or_model *m) {
@@ -1495,9 +1556,9 @@ int or_model_download_q4(or_model *m, double *offg, double *thk, double *forc, d
   return OR_OK;
 }
 
-int or_model_energy(or_model *m, double out[3]) {
-  if (!m || !out) return fail(OR_ERR_ARG, "null argument");
-  OR_HIP(hipSetDevice(m->device));
+// Fixed-order reduction of the internal energy and of the kinetic terms: 0.5*MS*|V|^2 from the
+// resident state, or the per-node ECRIT terms of the last output cycle (ecrit != null).
+static int energy_sums(or_model *m, const double *ecrit, double out[3]) {
   int64_t n = m->nn;
   if (m->neh > n) n = m->neh;
   if (m->neq > n) n = m->neq;
@@ -1505,20 +1566,28 @@ int or_model_energy(or_model *m, double out[3]) {
   const int64_t per = (n + nb - 1) / nb;
   const int64_t per_block = ((per + kBlock - 1) / kBlock) * kBlock;
   double *partial = m->energy + 3;
+  OR_HIP(quiesce(m));
   if (m->prec == OR_PREC_FP64)
     hipLaunchKernelGGL((k_energy_partial<double>), dim3(nb), dim3(kBlock), 0, m->stream, (const double *)m->heint,
                        (const double *)m->hvol0, m->neh, (const double *)m->qeint, m->neq, m->xv, m->vr, m->ms, m->in,
-                       m->nn, per_block, partial);
+                       ecrit, m->nn, per_block, partial);
   else
     hipLaunchKernelGGL((k_energy_partial<float>), dim3(nb), dim3(kBlock), 0, m->stream, (const float *)m->heint,
                        (const float *)m->hvol0, m->neh, (const float *)m->qeint, m->neq, m->xv, m->vr, m->ms, m->in,
-                       m->nn, per_block, partial);
+                       ecrit, m->nn, per_block, partial);
   OR_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, m->stream, (const double *)partial, nb, m->energy);
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, m->stream, (const double *)partial, nb,
+                     ecrit ? 1.0 : 0.5, m->energy);
   OR_HIP(hipGetLastError());
   OR_HIP(quiesce(m));
   OR_HIP(hipMemcpy(out, m->energy, 3 * sizeof(double), hipMemcpyDeviceToHost));
   return OR_OK;
+}
+
+int or_model_energy(or_model *m, double out[3]) {
+  if (!m || !out) return fail(OR_ERR_ARG, "null argument");
+  OR_HIP(hipSetDevice(m->device));
+  return energy_sums(m, nullptr, out);
 }
 
 int or_model_footprint_estimate(int64_t nn, int64_t ne_hex, int64_t ne_q4, int32_t npt, int precision,

@@ -160,6 +160,13 @@ class GpuModel:
         L.check(self._lib.or_model_energy(self._h, _p(out)))
         return out
 
+    def run_output(self) -> np.ndarray:
+        """One cycle with ECRIT's energy output: [internal, translational kinetic, rotational
+        kinetic] at the output point of that cycle (or_model_run_output)."""
+        out = np.zeros(3)
+        L.check(self._lib.or_model_run_output(self._h, _p(out)))
+        return out
+
     def set_timing(self, enable: bool):
         L.check(self._lib.or_model_set_timing(self._h, int(bool(enable))))
 

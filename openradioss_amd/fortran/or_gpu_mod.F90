@@ -15,7 +15,7 @@ module or_gpu_mod
   public :: or_version, or_device_count, or_device_memory, or_error_message
   public :: or_model_create, or_model_destroy, or_model_set_hex_params, or_model_set_q4_params
   public :: or_model_upload_nodes, or_model_upload_hex, or_model_upload_q4, or_model_upload_skyline
-  public :: or_model_set_time, or_model_get_time, or_model_run, or_model_sync
+  public :: or_model_set_time, or_model_get_time, or_model_run, or_model_run_output, or_model_sync
   public :: or_model_download_nodes, or_model_download_hex, or_model_download_q4
   public :: or_model_energy, or_model_kernel_times, or_model_set_timing, or_model_footprint
   public :: or_model_footprint_estimate
@@ -143,6 +143,13 @@ module or_gpu_mod
       import :: c_int, c_int64_t, c_ptr
       type(c_ptr), value :: handle
       integer(c_int64_t), value :: ncycles
+    end function
+
+    ! one cycle with ECRIT's energies: e(1) internal, e(2) translational, e(3) rotational kinetic
+    integer(c_int) function or_model_run_output(handle, e) bind(c, name="or_model_run_output")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: e(3)
     end function
 
     integer(c_int) function or_model_sync(handle) bind(c, name="or_model_sync")

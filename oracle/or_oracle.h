@@ -170,6 +170,9 @@ typedef struct {
   const int32_t *qconn; const int64_t *qiadc;
   or_q4_state q;
   const or_q4_params *qp;
+  /* if non-NULL, or_cycle_model writes the energies of ECRIT for this cycle:
+   * [0] internal, [1] translational kinetic, [2] rotational kinetic */
+  double *ecrit;
 } or_model_c;
 
 void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads);

@@ -715,6 +715,10 @@ void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads) {
   ts->dt2old = ts->dt2;
   ts->dt12 = HALF * (ts->dt1 + ts->dt2);
   const double DT12 = ts->dt12, DT2 = ts->dt2;
+  /* ECRIT output point (sortie_main.F:368 from resol.F:8523, after ACCELE/BCS10/FIXVEL and
+   * before VELOCITY): DT05 = HALF*DT1 (ecrit.F:193) */
+  const double DT05 = HALF * ts->dt1;
+  double encin = ZERO, enrot = ZERO;
   for (int64_t n = 0; n < nn; ++n) {
     double *a = m->a + 3 * n, *v = m->v + 3 * n;
     /* ACCELE (accele.F:65-80, 97-109) */
@@ -747,6 +751,15 @@ void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads) {
       if (code & 128) a[1] = (m->vimp[3 * n + 1] - v[1]) / DT12;
       if (code & 64) a[2] = (m->vimp[3 * n + 2] - v[2]) / DT12;
     }
+    if (m->ecrit) {
+      /* ecrit.F:221-225 (MAS = MS*WEIGHT_MD, WEIGHT_MD = 1) and 319-324 */
+      const double vx = v[0] + DT05 * a[0], vy = v[1] + DT05 * a[1], vz = v[2] + DT05 * a[2];
+      encin = encin + (vx * vx + vy * vy + vz * vz) * HALF * m->ms[n];
+      if (m->in) {
+        const double wx = vr[0] + DT05 * ar[0], wy = vr[1] + DT05 * ar[1], wz = vr[2] + DT05 * ar[2];
+        enrot = enrot + (wx * wx + wy * wy + wz * wz) * HALF * m->in[n];
+      }
+    }
     /* VELOCITY (velocity.F:58-86) */
     v[0] = v[0] + DT12 * a[0]; v[1] = v[1] + DT12 * a[1]; v[2] = v[2] + DT12 * a[2];
     a[0] = ZERO; a[1] = ZERO; a[2] = ZERO;
@@ -761,6 +774,16 @@ void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads) {
       d[c] = d[c] + vdt;
       x[c] = x[c] + vdt;
     }
+  }
+  if (m->ecrit) {
+    /* internal energy: SBILAN EI = EINT*VOL (sbilan.F:140), CBILAN EI = EINT(1)+EINT(2)
+     * (cbilan.F:183), summed in element order */
+    double ie = ZERO;
+    for (int64_t e = 0; e < m->ne_hex; ++e) ie = ie + m->heint[e] * m->hvol0[e];
+    for (int64_t e = 0; e < m->ne_q4; ++e) ie = ie + (m->q.eint[e] + m->q.eint[m->ne_q4 + e]);
+    m->ecrit[0] = ie;
+    m->ecrit[1] = encin;
+    m->ecrit[2] = enrot;
   }
   ts->tt = ts->tt + ts->dt2;
   ts->ncycle += 1;

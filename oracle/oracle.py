@@ -179,7 +179,7 @@ class ModelC(C.Structure):
                 ("ne_hex", C.c_int64), ("hconn", _I32), ("hiads", _I64)] + \
                [(n, _D) for n in ("hsig", "heint", "hrho", "hpla", "hepsd", "hoffg", "hqvis", "hvol0")] + \
                [("hp", C.POINTER(HexParams)), ("ne_q4", C.c_int64), ("qconn", _I32), ("qiadc", _I64),
-                ("q", Q4State), ("qp", C.POINTER(Q4Params))]
+                ("q", Q4State), ("qp", C.POINTER(Q4Params)), ("ecrit", _D)]
 
 
 def q4_params(part) -> Q4Params:
@@ -270,4 +270,15 @@ class ModelOracle:
     def energy(self):
         out = np.zeros(3)
         lib(self.math).or_energy_model(C.byref(self.c), _pd(out))
+        return out
+
+    def cycle_output(self) -> np.ndarray:
+        """One cycle that also returns ECRIT's energies of the cycle (internal, translational and
+        rotational kinetic at the output point between ACCELE and VELOCITY)."""
+        out = np.zeros(3)
+        self.c.ecrit = _pd(out)
+        try:
+            lib(self.math).or_cycle_model(C.byref(self.c), C.byref(self.ts), C.c_int(self.nthreads))
+        finally:
+            self.c.ecrit = None
         return out

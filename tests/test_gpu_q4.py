@@ -132,3 +132,26 @@ def test_cforc3_group_dropin_matches_oracle(gpu_lib):
     assert np.array_equal(fsky, o.fsky)
     assert np.array_equal(lb["sig"], o.q["sig"])
     assert np.array_equal(gb["forc"], o.q["forc"])
+
+
+@pytest.mark.parametrize("name", ["mixed_c120", "tube_8x20_c150"])
+def test_ecrit_output_cycle_matches_oracle(name, gpu_lib):
+    """or_model_run_output: ECRIT's energies of the cycle (fixed-order GPU sums against the
+    oracle's sequential sums) and an unchanged trajectory."""
+    c = Q4_CASES[name]
+    print(f"case: {name}, ECRIT output on cycle {c['cycles']}", flush=True)
+    m = build_q4_case(c)
+    o = O.ModelOracle(m, nthreads=8, math="cr")
+    g = E.GpuModel(m)
+    o.cycle(c["cycles"] - 1)
+    g.run(c["cycles"] - 1)
+    eo = o.cycle_output()
+    eg = g.run_output()
+    print(f"ECRIT energies gpu {eg}, oracle {eo}")
+    assert np.all(np.abs(eg - eo) <= 1e-13 * np.abs(eo).max())
+    x, v, vr, d = g.nodes6()
+    assert np.array_equal(d, o.d) and np.array_equal(v, o.v) and np.array_equal(vr, o.vr)
+    g.run(5)
+    o.cycle(5)
+    assert np.array_equal(g.nodes6()[3], o.d)
+    g.close()
