@@ -44,6 +44,7 @@ def gpu_run(model, precision: str, steps: int, warmup: int):
 
     g = E.GpuModel(model, precision=precision)
     g.run(warmup)
+    variants = g.autotune(3)  # launch variants of the element kernels (results do not depend on them)
     g.sync()
     t0 = time.perf_counter()
     g.run(steps, sync=False)
@@ -61,7 +62,21 @@ def gpu_run(model, precision: str, steps: int, warmup: int):
     foot = g.footprint()
     g.close()
     return dict(wall=wall, hex_ms=hex_ms, q4_ms=q4_ms, node_ms=node_ms, ts=ts, d=d, energy=e, footprint=foot,
-                finite=bool(np.isfinite(d).all()))
+                finite=bool(np.isfinite(d).all()), variants=variants)
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on the default workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled with the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "block160_plate1000_hbm_traffic.md")
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.split("|")]
+            if len(cells) > 4 and cells[1] == f"`{kernel}`":
+                return float(cells[4]) * 1e9
+    except OSError:
+        return None
+    return None
 
 
 def cpu_baseline(model, budget_s: float = 12.0):
@@ -101,6 +116,7 @@ def sweep_leg(precision: str, cycles: int, warmup: int, n: int = 256):
         gms.append(g)
     setup = time.perf_counter() - t0
     b.run(warmup)
+    variants = b.autotune(2)
     t1 = time.perf_counter()
     b.run(cycles)
     wall = time.perf_counter() - t1
@@ -112,6 +128,7 @@ def sweep_leg(precision: str, cycles: int, warmup: int, n: int = 256):
         g.close()
     return dict(models=n, shells=ne, precision=precision, cycles=cycles, ms_per_cycle=1e3 * wall / cycles,
                 element_updates_per_s=ne * cycles / wall, active_after=active, setup_s=setup,
+                variants=dict(zip(("fp64_hex", "fp64_q4", "split_hex", "split_q4"), variants)),
                 sim_time_range_s=[min(tts), max(tts)])
 
 
@@ -199,7 +216,9 @@ def main():
                                f"tied on the block top, law-2 steel, v0 = 10 m/s + N(0,0.5) seed {SEED}",
                    "hex_elements": model.ne_hex, "q4_elements": model.ne_q4, "nodes": model.nn},
         "roofline": {"bound": "hbm", "achieved": hex_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hex_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": hex_gbs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_hex_forces<double, double>")
+                     if (args.n, args.plate) == (160, 1000) else None,
                      "kernel": "k_hex_forces<double,double>", "kernel_ms": r64["hex_ms"],
                      "bytes_per_launch": tb["hex"]["as_implemented"],
                      "compulsory_bytes_per_launch": tb["hex"]["compulsory"]},
@@ -211,6 +230,7 @@ def main():
                        "gbs": tb["total"]["as_implemented"] / (cyc_ms * 1e-3) / 1e9,
                        "bytes_per_element_update": tb["total"]["as_implemented"] / ne},
         "final_time": r64["ts"].tt,
+        "kernel_variants_fp64": {"hex": r64["variants"][0], "q4": r64["variants"][1]},
         "footprint_gb_fp64": r64["footprint"] / 1e9,
     }
     if not args.no_split:
@@ -226,6 +246,7 @@ def main():
                         for k in ("hex", "q4", "node")},
             "cycle_gbs": ts32["total"]["as_implemented"] / (cyc32 * 1e-3) / 1e9,
             "disp_rel_err_vs_fp64_gpu": derr, "energy_rel_err_vs_fp64_gpu": eerr,
+            "kernel_variants": {"hex": rs["variants"][0], "q4": rs["variants"][1]},
             "dtype": "f32 element / f64 node"}
         log(f"split: {eus:.3e} element-updates/s (x{eus / eu64:.2f}), disp err {derr:.2e}, energy err {eerr:.2e}")
     if not args.no_sweep:

@@ -209,6 +209,17 @@ int or_batch_sync(or_batch *b);
  * launches of later cycles. */
 int or_batch_active(or_batch *b, int32_t *nactive);
 
+/* ---- launch variants of the element kernels and their autotuning ----
+ * No counterpart in the reference.  Variant v (0..3) of the hex (family 0) or shell
+ * (family 1) kernel sets its workgroup size (256 elements for even v, 64 for odd v) and
+ * the register budget it is compiled for (one more wave per SIMD for v >= 2).  All
+ * variants compute bit-identical results.  The autotuners time cycles_per_variant real
+ * cycles per candidate (the models advance meanwhile) and keep the fastest:
+ * chosen[family] for a model, chosen[2*precision + family] for a batch. */
+int or_model_set_variant(or_model *m, int family, int variant);
+int or_model_autotune(or_model *m, int32_t cycles_per_variant, int32_t chosen[2]);
+int or_batch_autotune(or_batch *b, int32_t cycles_per_variant, int32_t chosen[4]);
+
 /* Diagnostic, no counterpart in the reference: evaluates on the device the FP64
  * elementary functions the element kernels use in place of the host libm's
  * (fn 0: log(x), 1: exp(x), 2: pow(x, y)), for n arguments, 0 <= n <= 2^30.

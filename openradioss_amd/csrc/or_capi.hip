@@ -44,22 +44,15 @@ int fail(int code, const std::string &msg) {
 constexpr int kBlock = 256;
 constexpr double kNoEnd = 1.0e300; // end time of a model advanced by or_model_run
 
-// Minimum waves per SIMD the element kernels are compiled for (register budget
-// 512 / waves per lane, VGPRs and AGPRs together).
-#ifndef OR_HEX_WAVES_FP64
-#define OR_HEX_WAVES_FP64 1
-#endif
-#ifndef OR_HEX_WAVES_FP32
-#define OR_HEX_WAVES_FP32 2
-#endif
-#ifndef OR_Q4_WAVES_FP64
-#define OR_Q4_WAVES_FP64 1
-#endif
-#ifndef OR_Q4_WAVES_FP32
-#define OR_Q4_WAVES_FP32 2
-#endif
-template <typename R> constexpr int hex_waves() { return sizeof(R) == 8 ? OR_HEX_WAVES_FP64 : OR_HEX_WAVES_FP32; }
-template <typename R> constexpr int q4_waves() { return sizeof(R) == 8 ? OR_Q4_WAVES_FP64 : OR_Q4_WAVES_FP32; }
+// Launch variants of the element kernels.  Variant v picks the workgroup size
+// (256 elements for even v, 64 for odd v) and the minimum number of waves per SIMD
+// the register allocation is built for (the precision's base for v < 2, one more
+// for v >= 2; a wave's register budget is 512 / waves VGPRs and AGPRs together).
+// Every variant runs the same arithmetic, so the choice (or_model_set_variant,
+// or_model_autotune, or_batch_autotune) changes speed only, never a result bit.
+constexpr int kVariants = 4;
+constexpr int variant_block(int v) { return (v & 1) ? 64 : 256; }
+template <typename R> constexpr int variant_waves(int v) { return (sizeof(R) == 8 ? 1 : 2) + (v >> 1); }
 
 // node code bits (include/or_gpu.h)
 constexpr int32_t kTX = 4, kTY = 2, kTZ = 1, kRX = 32, kRY = 16, kRZ = 8, kIX = 256, kIY = 128, kIZ = 64;
@@ -161,7 +154,8 @@ struct ModelDev {
   Q4Consts qc;
 };
 
-// One block of a cycle launch: kBlock consecutive elements (or nodes) of one model.
+// One block of a cycle launch: consecutive elements (the launch's workgroup size) or
+// kBlock consecutive nodes of one model.
 struct Tile {
   int32_t model; // index in the launch's ModelDev array
   int32_t first; // first element (node) of the block
@@ -182,8 +176,8 @@ template <typename S> __device__ __forceinline__ Q4Ptrs<S> q4_ptrs(const ModelDe
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-__device__ __forceinline__ double block_min(double v) {
-  __shared__ double red[kBlock / 64];
+template <int TB> __device__ __forceinline__ double block_min(double v) {
+  __shared__ double red[TB / 64];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     double o = __shfl_xor(v, off, 64);
@@ -193,7 +187,7 @@ __device__ __forceinline__ double block_min(double v) {
   if (lane == 0) red[wid] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < kBlock / 64; ++w) v = (red[w] < v) ? red[w] : v;
+    for (int w = 1; w < TB / 64; ++w) v = (red[w] < v) ? red[w] : v;
   }
   return v;
 }
@@ -206,13 +200,15 @@ __device__ __forceinline__ void load_node(gptr<const double> xv, int64_t n, doub
 }
 
 // ---------------------------------------------------------------- hex element kernel
-template <typename R, typename S>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(hex_waves<R>()))) void k_hex_forces(
+template <typename R, typename S, int TB, int W>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_hex_forces(
     const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity) {
   const Tile tl = tiles[xcd_contiguous_block(blockIdx.x, gridDim.x)];
   const ModelDev &M = md[tl.model];
   const DevTime *ts = M.ts;
-  if (ts[parity].done) return;
+  // a stopped model computes but stores nothing; the flag is tested only before the
+  // stores, so that its load overlaps the element's own loads
+  const int stopped = ts[parity].done;
   const HexPtrs<S> e = hex_ptrs<S>(M);
   const gptr<const double> xv = to_global<const double>(M.nd.xv);
   const gptr<S> sky = to_global<S>(M.skh);
@@ -238,20 +234,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(hex_wave
     st.epsd = e.epsd[i]; st.offg = e.offg[i]; st.qvis = e.qvis[i];
     HexOut<R> o;
     hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o);
+    if (!stopped) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) e.sig[(int64_t)k * e.ne + i] = st.sig[k];
-    e.eint[i] = st.eint; e.rho[i] = st.rho; e.pla[i] = st.pla;
-    e.epsd[i] = st.epsd; e.offg[i] = st.offg; e.qvis[i] = st.qvis;
+      for (int k = 0; k < 6; ++k) e.sig[(int64_t)k * e.ne + i] = st.sig[k];
+      e.eint[i] = st.eint; e.rho[i] = st.rho; e.pla[i] = st.pla;
+      e.epsd[i] = st.epsd; e.offg[i] = st.offg; e.qvis[i] = st.qvis;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const gptr<S> r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
-      r[0] = S(o.f1[k]); r[kBlock] = S(o.f2[k]); r[2 * kBlock] = S(o.f3[k]);
+      for (int k = 0; k < 8; ++k) {
+        const gptr<S> r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
+        r[0] = S(o.f1[k]); r[kBlock] = S(o.f2[k]); r[2 * kBlock] = S(o.f3[k]);
+      }
     }
     dtx = o.dtx;
     mstop = o.mstop;
   }
+  if (stopped) return;
   if (mstop) atomicOr(M.flags, 1);
-  const double m = block_min(dtx);
+  const double m = block_min<TB>(dtx);
   if (threadIdx.x == 0) atomicMin(M.dtmin + parity, dbits(m));
 }
 
@@ -259,6 +258,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(hex_wave
 template <typename R, typename S> struct Q4IpIO {
   const Q4Ptrs<S> &q;
   int64_t i;
+  bool active; // false: the model has stopped, nothing is stored
   __device__ __forceinline__ void load(int ip, R (&s)[5], R &pla, R &epsd_ip) const {
     const gptr<const S> sg = q.sig + (int64_t)ip * 5 * q.ne + i;
 #pragma unroll
@@ -267,6 +267,7 @@ template <typename R, typename S> struct Q4IpIO {
     epsd_ip = q.epsd_ip ? R(q.epsd_ip[(int64_t)ip * q.ne + i]) : R(0);
   }
   __device__ __forceinline__ void store(int ip, const R (&s)[5], R pla, R epsd_ip) const {
+    if (!active) return;
     const gptr<S> sg = q.sig + (int64_t)ip * 5 * q.ne + i;
 #pragma unroll
     for (int k = 0; k < 5; ++k) sg[(int64_t)k * q.ne] = S(s[k]);
@@ -275,13 +276,13 @@ template <typename R, typename S> struct Q4IpIO {
   }
 };
 
-template <typename R, typename S>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(q4_waves<R>()))) void k_q4_forces(
+template <typename R, typename S, int TB, int W>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_q4_forces(
     const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity) {
   const Tile tl = tiles[xcd_contiguous_block(blockIdx.x, gridDim.x)];
   const ModelDev &M = md[tl.model];
   const DevTime *ts = M.ts;
-  if (ts[parity].done) return;
+  const int stopped = ts[parity].done; // tested before the stores only
   const Q4Ptrs<S> q = q4_ptrs<S>(M);
   const gptr<const double> xv = to_global<const double>(M.nd.xv);
   const gptr<const double> vr = to_global<const double>(M.nd.vr);
@@ -316,28 +317,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(q4_waves
 #pragma unroll
     for (int k = 0; k < 5; ++k) st.hourg[k] = R(q.hourg[(int64_t)k * q.ne + i]);
     st.eint[0] = R(q.eint[i]); st.eint[1] = R(q.eint[q.ne + i]);
-    Q4IpIO<R, S> io{q, i};
+    Q4IpIO<R, S> io{q, i, !stopped};
     Q4Out<R> o;
     q4_element<R>(xg, yg, zg, vl, vrl, nc[2] == nc[3], st, c, DT1, io, o);
-    q.offg[i] = S(st.offg); q.thk[i] = S(st.thk); q.epsd[i] = S(st.epsd);
+    if (!stopped) {
+      q.offg[i] = S(st.offg); q.thk[i] = S(st.thk); q.epsd[i] = S(st.epsd);
 #pragma unroll
-    for (int k = 0; k < 5; ++k) q.forc[(int64_t)k * q.ne + i] = S(st.forc[k]);
+      for (int k = 0; k < 5; ++k) q.forc[(int64_t)k * q.ne + i] = S(st.forc[k]);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) q.mom[(int64_t)k * q.ne + i] = S(st.mom[k]);
+      for (int k = 0; k < 3; ++k) q.mom[(int64_t)k * q.ne + i] = S(st.mom[k]);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) q.gstr[(int64_t)k * q.ne + i] = S(st.gstr[k]);
+      for (int k = 0; k < 8; ++k) q.gstr[(int64_t)k * q.ne + i] = S(st.gstr[k]);
 #pragma unroll
-    for (int k = 0; k < 5; ++k) q.hourg[(int64_t)k * q.ne + i] = S(st.hourg[k]);
-    q.eint[i] = S(st.eint[0]); q.eint[q.ne + i] = S(st.eint[1]);
+      for (int k = 0; k < 5; ++k) q.hourg[(int64_t)k * q.ne + i] = S(st.hourg[k]);
+      q.eint[i] = S(st.eint[0]); q.eint[q.ne + i] = S(st.eint[1]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const gptr<S> r = sky + row_word(q.iadc[(int64_t)k * q.ne + i], kQ4RowWords);
-      r[0] = S(-o.f[0][k]); r[kBlock] = S(-o.f[1][k]); r[2 * kBlock] = S(-o.f[2][k]);
-      r[3 * kBlock] = S(-o.m[0][k]); r[4 * kBlock] = S(-o.m[1][k]); r[5 * kBlock] = S(-o.m[2][k]);
+      for (int k = 0; k < 4; ++k) {
+        const gptr<S> r = sky + row_word(q.iadc[(int64_t)k * q.ne + i], kQ4RowWords);
+        r[0] = S(-o.f[0][k]); r[kBlock] = S(-o.f[1][k]); r[2 * kBlock] = S(-o.f[2][k]);
+        r[3 * kBlock] = S(-o.m[0][k]); r[4 * kBlock] = S(-o.m[1][k]); r[5 * kBlock] = S(-o.m[2][k]);
+      }
     }
     dt = o.dt;
   }
-  const double m = block_min(dt);
+  if (stopped) return;
+  const double m = block_min<TB>(dt);
   if (threadIdx.x == 0) atomicMin(M.dtmin + parity, dbits(m));
 }
 
@@ -356,6 +360,27 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
   const double dtmx = M.dtmx;
   const int64_t n = (int64_t)tl.first + threadIdx.x;
   const DevTime prev = ts[parity];
+  const bool in_range = n < nd.nn;
+  // ASSPAR4: rows of this node in skyline order (solids, then shells), from zero
+  double ax = 0.0, ay = 0.0, az = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
+  const int lane = (int)(n % kBlock);
+  if (skh && in_range) {
+    const int32_t deg = to_global(nd.deg_h)[n];
+    gptr<const S> r = skh + (int64_t)to_global(nd.slab_h)[n / kBlock] * kHexRowWords * kBlock + lane;
+    for (int32_t j = 0; j < deg; ++j, r += kHexRowWords * kBlock) {
+      ax = ax + double(r[0]); ay = ay + double(r[kBlock]); az = az + double(r[2 * kBlock]);
+    }
+  }
+  bool rot = false; // node carries rotational DOF work this cycle
+  if (skq && in_range) {
+    const int32_t deg = to_global(nd.deg_q)[n];
+    rot = deg > 0;
+    gptr<const S> r = skq + (int64_t)to_global(nd.slab_q)[n / kBlock] * kQ4RowWords * kBlock + lane;
+    for (int32_t j = 0; j < deg; ++j, r += kQ4RowWords * kBlock) {
+      ax = ax + double(r[0]); ay = ay + double(r[kBlock]); az = az + double(r[2 * kBlock]);
+      rx = rx + double(r[3 * kBlock]); ry = ry + double(r[4 * kBlock]); rz = rz + double(r[5 * kBlock]);
+    }
+  }
   if (prev.done) {
     // a stopped model keeps its state; its scalars are carried to the other parity
     if (n == 0) {
@@ -388,27 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
     ts[parity ^ 1] = nx;
     dtmin[parity ^ 1] = dbits(kEP06);
   }
-  if (n >= nd.nn) return;
-  // ASSPAR4: rows of this node in skyline order (solids, then shells), from zero
-  double ax = 0.0, ay = 0.0, az = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
-  const int lane = (int)(n % kBlock);
-  if (skh) {
-    const int32_t deg = to_global(nd.deg_h)[n];
-    gptr<const S> r = skh + (int64_t)to_global(nd.slab_h)[n / kBlock] * kHexRowWords * kBlock + lane;
-    for (int32_t j = 0; j < deg; ++j, r += kHexRowWords * kBlock) {
-      ax = ax + double(r[0]); ay = ay + double(r[kBlock]); az = az + double(r[2 * kBlock]);
-    }
-  }
-  bool rot = false; // node carries rotational DOF work this cycle
-  if (skq) {
-    const int32_t deg = to_global(nd.deg_q)[n];
-    rot = deg > 0;
-    gptr<const S> r = skq + (int64_t)to_global(nd.slab_q)[n / kBlock] * kQ4RowWords * kBlock + lane;
-    for (int32_t j = 0; j < deg; ++j, r += kQ4RowWords * kBlock) {
-      ax = ax + double(r[0]); ay = ay + double(r[kBlock]); az = az + double(r[2 * kBlock]);
-      rx = rx + double(r[3 * kBlock]); ry = ry + double(r[4 * kBlock]); rz = rz + double(r[5 * kBlock]);
-    }
-  }
+  if (!in_range) return;
   // ACCELE (accele.F:65-80, 97-109)
   const double msn = to_global(nd.ms)[n];
   if (msn > 0.0) {
@@ -816,6 +821,7 @@ struct or_model {
   bool dev_dirty = true;
   Tile *tiles_h = nullptr, *tiles_q = nullptr, *tiles_n = nullptr;
   int32_t nt_h = 0, nt_q = 0, nt_n = 0;
+  int var_h = 0, var_q = 0;            // element kernel variants (tiles_h / tiles_q follow their block)
   or_batch *batch = nullptr;           // set while the model belongs to a batch
   // timing
   bool timing = false;
@@ -836,6 +842,7 @@ struct or_batch {
   ModelDev *md = nullptr;          // device array of the slots' kernel views
   int64_t md_cap = 0;
   Tile *tiles[2][3] = {};          // [precision][hex, q4, node]
+  int var[2][2] = {};              // element kernel variants [precision][hex, q4]
   int32_t nt[2][3] = {};
   int64_t tcap[2][3] = {};
   bool dirty = true;
@@ -859,6 +866,20 @@ static void mark_changed(or_model *m) {
 }
 
 static size_t ssize(const or_model *m) { return m->prec == OR_PREC_FP64 ? sizeof(double) : sizeof(float); }
+
+// Element tiles of a model alone, for its current variants.
+static int put_elem_tiles(or_model *m) {
+  for (int f = 0; f < 2; ++f) {
+    const int64_t ne = f == 0 ? m->neh : m->neq;
+    if (ne == 0) continue;
+    const int tb = variant_block(f == 0 ? m->var_h : m->var_q);
+    std::vector<Tile> tl;
+    for (int64_t first = 0; first < ne; first += tb) tl.push_back(Tile{0, (int32_t)first});
+    OR_HIP(hipMemcpy(f == 0 ? m->tiles_h : m->tiles_q, tl.data(), sizeof(Tile) * tl.size(), hipMemcpyHostToDevice));
+    (f == 0 ? m->nt_h : m->nt_q) = (int32_t)tl.size();
+  }
+  return OR_OK;
+}
 
 // Entry points: C linkage comes from their declarations in or_gpu.h.
 
@@ -951,11 +972,9 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int6
   A(&m->dtmin, 2);
   A(&m->flags, 1);
   A(&m->dev, 1);
-  m->nt_h = (int32_t)((ne_hex + kBlock - 1) / kBlock);
-  m->nt_q = (int32_t)((ne_q4 + kBlock - 1) / kBlock);
   m->nt_n = (int32_t)((nn + kBlock - 1) / kBlock);
-  A(&m->tiles_h, m->nt_h);
-  A(&m->tiles_q, m->nt_q);
+  A(&m->tiles_h, (ne_hex + 63) / 64); // room for the smallest element block
+  A(&m->tiles_q, (ne_q4 + 63) / 64);
   A(&m->tiles_n, m->nt_n);
   const int64_t nb = 1024;
   A(&m->energy, 3 + 3 * nb);
@@ -983,9 +1002,8 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int6
       if (hipMemcpy(dst, tl.data(), sizeof(Tile) * (size_t)nt, hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(OR_ERR_HIP, "tile upload");
     };
-    put_tiles(m->tiles_h, m->nt_h);
-    put_tiles(m->tiles_q, m->nt_q);
     put_tiles(m->tiles_n, m->nt_n);
+    if (rc == OR_OK) rc = put_elem_tiles(m);
     if (rc == OR_OK && (hipMemcpyAsync(m->ts, tsh, sizeof(tsh), hipMemcpyHostToDevice, m->stream) != hipSuccess ||
                         hipMemcpyAsync(m->dtmin, dm, sizeof(dm), hipMemcpyHostToDevice, m->stream) != hipSuccess ||
                         hipStreamSynchronize(m->stream) != hipSuccess))
@@ -1359,18 +1377,43 @@ struct LaunchSet {
   const ModelDev *md = nullptr;
   const Tile *th = nullptr, *tq = nullptr, *tn = nullptr;
   int32_t nth = 0, ntq = 0, ntn = 0;
+  int vh = 0, vq = 0; // element kernel variants (their block sizes match th / tq)
 };
+
+template <typename R, typename S>
+static void launch_hex(int v, hipStream_t st, const LaunchSet &L, int parity) {
+  constexpr int W0 = variant_waves<R>(0), W1 = variant_waves<R>(2);
+  const dim3 g((unsigned)L.nth);
+  switch (v) {
+  case 0: hipLaunchKernelGGL((k_hex_forces<R, S, 256, W0>), g, dim3(256), 0, st, L.md, L.th, parity); break;
+  case 1: hipLaunchKernelGGL((k_hex_forces<R, S, 64, W0>), g, dim3(64), 0, st, L.md, L.th, parity); break;
+  case 2: hipLaunchKernelGGL((k_hex_forces<R, S, 256, W1>), g, dim3(256), 0, st, L.md, L.th, parity); break;
+  default: hipLaunchKernelGGL((k_hex_forces<R, S, 64, W1>), g, dim3(64), 0, st, L.md, L.th, parity); break;
+  }
+}
+
+template <typename R, typename S>
+static void launch_q4(int v, hipStream_t st, const LaunchSet &L, int parity) {
+  constexpr int W0 = variant_waves<R>(0), W1 = variant_waves<R>(2);
+  const dim3 g((unsigned)L.ntq);
+  switch (v) {
+  case 0: hipLaunchKernelGGL((k_q4_forces<R, S, 256, W0>), g, dim3(256), 0, st, L.md, L.tq, parity); break;
+  case 1: hipLaunchKernelGGL((k_q4_forces<R, S, 64, W0>), g, dim3(64), 0, st, L.md, L.tq, parity); break;
+  case 2: hipLaunchKernelGGL((k_q4_forces<R, S, 256, W1>), g, dim3(256), 0, st, L.md, L.tq, parity); break;
+  default: hipLaunchKernelGGL((k_q4_forces<R, S, 64, W1>), g, dim3(64), 0, st, L.md, L.tq, parity); break;
+  }
+}
 
 template <typename R, typename S>
 static int launch_cycle(hipStream_t st, const LaunchSet &L, int parity, hipEvent_t after_hex, hipEvent_t after_q4,
                         int output = 0) {
   if (L.nth > 0) {
-    hipLaunchKernelGGL((k_hex_forces<R, S>), dim3((unsigned)L.nth), dim3(kBlock), 0, st, L.md, L.th, parity);
+    launch_hex<R, S>(L.vh, st, L, parity);
     OR_HIP(hipGetLastError());
   }
   if (after_hex) OR_HIP(hipEventRecord(after_hex, st));
   if (L.ntq > 0) {
-    hipLaunchKernelGGL((k_q4_forces<R, S>), dim3((unsigned)L.ntq), dim3(kBlock), 0, st, L.md, L.tq, parity);
+    launch_q4<R, S>(L.vq, st, L, parity);
     OR_HIP(hipGetLastError());
   }
   if (after_q4) OR_HIP(hipEventRecord(after_q4, st));
@@ -1394,8 +1437,8 @@ int or_model_run(or_model *m, int64_t ncycles) {
   if (rc0) return rc0;
   LaunchSet L;
   L.md = m->dev;
-  L.th = m->tiles_h; L.nth = m->nt_h;
-  L.tq = m->tiles_q; L.ntq = m->nt_q;
+  L.th = m->tiles_h; L.nth = m->nt_h; L.vh = m->var_h;
+  L.tq = m->tiles_q; L.ntq = m->nt_q; L.vq = m->var_q;
   L.tn = m->tiles_n; L.ntn = m->nt_n;
   for (int64_t c = 0; c < ncycles; ++c) {
     if (m->timing) {
@@ -1439,8 +1482,8 @@ int or_model_run_output(or_model *m, double out[3]) {
   if (rc) return rc;
   LaunchSet L;
   L.md = m->dev;
-  L.th = m->tiles_h; L.nth = m->nt_h;
-  L.tq = m->tiles_q; L.ntq = m->nt_q;
+  L.th = m->tiles_h; L.nth = m->nt_h; L.vh = m->var_h;
+  L.tq = m->tiles_q; L.ntq = m->nt_q; L.vq = m->var_q;
   L.tn = m->tiles_n; L.ntn = m->nt_n;
   const int p = (int)(m->parity & 1);
   rc = (m->prec == OR_PREC_FP64) ? launch_cycle<double, double>(m->stream, L, p, nullptr, nullptr, 1)
@@ -1827,8 +1870,9 @@ int batch_rebuild(or_batch *b) {
     if (!b->live[(size_t)i]) continue;
     const int c = m->prec == OR_PREC_FP64 ? 0 : 1;
     const int64_t cnt[3] = {m->neh, m->neq, m->nn};
+    const int step[3] = {variant_block(b->var[c][0]), variant_block(b->var[c][1]), kBlock};
     for (int f = 0; f < 3; ++f)
-      for (int64_t first = 0; first < cnt[f]; first += kBlock) tl[c][f].push_back(Tile{(int32_t)i, (int32_t)first});
+      for (int64_t first = 0; first < cnt[f]; first += step[f]) tl[c][f].push_back(Tile{(int32_t)i, (int32_t)first});
   }
   if (ns > 0) OR_HIP(hipMemcpy(b->md, hd.data(), sizeof(ModelDev) * (size_t)ns, hipMemcpyHostToDevice));
   for (int c = 0; c < 2; ++c)
@@ -1928,8 +1972,8 @@ int or_batch_run(or_batch *b, int64_t ncycles) {
   LaunchSet L[2];
   for (int c = 0; c < 2; ++c) {
     L[c].md = b->md;
-    L[c].th = b->tiles[c][0]; L[c].nth = b->nt[c][0];
-    L[c].tq = b->tiles[c][1]; L[c].ntq = b->nt[c][1];
+    L[c].th = b->tiles[c][0]; L[c].nth = b->nt[c][0]; L[c].vh = b->var[c][0];
+    L[c].tq = b->tiles[c][1]; L[c].ntq = b->nt[c][1]; L[c].vq = b->var[c][1];
     L[c].tn = b->tiles[c][2]; L[c].ntn = b->nt[c][2];
   }
   for (int64_t k = 0; k < ncycles; ++k) {
@@ -1969,5 +2013,104 @@ int or_batch_active(or_batch *b, int32_t *nactive) {
     n += alive;
   }
   *nactive = n;
+  return OR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel variants and their autotuning
+namespace {
+
+// Device time of `ncycles` cycles launched by `launch` on `st` (events around the loop).
+template <typename F> int time_cycles(hipStream_t st, int ncycles, F launch, float &ms) {
+  hipEvent_t e0, e1;
+  OR_HIP(hipEventCreate(&e0));
+  OR_HIP(hipEventCreate(&e1));
+  int rc = OR_OK;
+  if (hipEventRecord(e0, st) != hipSuccess) rc = fail(OR_ERR_HIP, "hipEventRecord");
+  for (int k = 0; k < ncycles && rc == OR_OK; ++k) rc = launch();
+  if (rc == OR_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                      hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = fail(OR_ERR_HIP, "event timing");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+} // namespace
+
+int or_model_set_variant(or_model *m, int family, int variant) {
+  if (!m) return fail(OR_ERR_ARG, "null model");
+  if (family < 0 || family > 1 || variant < 0 || variant >= kVariants) return fail(OR_ERR_ARG, "bad family / variant");
+  OR_HIP(hipSetDevice(m->device));
+  OR_HIP(quiesce(m));
+  (family == 0 ? m->var_h : m->var_q) = variant;
+  return put_elem_tiles(m);
+}
+
+int or_model_autotune(or_model *m, int32_t cycles_per_variant, int32_t chosen[2]) {
+  if (!m || !chosen) return fail(OR_ERR_ARG, "null argument");
+  if (cycles_per_variant < 1) return fail(OR_ERR_ARG, "cycles_per_variant must be >= 1");
+  if (m->batch) return fail(OR_ERR_STATE, "the model belongs to a batch: use or_batch_autotune");
+  if (!m->have_nodes || !m->have_sky || (m->neh > 0 && (!m->have_hparams || !m->have_hex)) ||
+      (m->neq > 0 && (!m->have_qparams || !m->have_q4)))
+    return fail(OR_ERR_STATE, "params, nodes, element state and skyline must be uploaded before autotune");
+  OR_HIP(hipSetDevice(m->device));
+  int rc = refresh_dev(m);
+  if (rc) return rc;
+  for (int f = 0; f < 2; ++f) {
+    if ((f == 0 ? m->neh : m->neq) == 0) continue;
+    int best = 0;
+    float best_ms = 0.f;
+    for (int v = 0; v < kVariants; ++v) {
+      if ((rc = or_model_set_variant(m, f, v))) return rc;
+      LaunchSet L;
+      L.md = m->dev;
+      L.th = m->tiles_h; L.nth = m->nt_h; L.vh = m->var_h;
+      L.tq = m->tiles_q; L.ntq = m->nt_q; L.vq = m->var_q;
+      L.tn = m->tiles_n; L.ntn = m->nt_n;
+      float ms = 0.f;
+      auto one = [&]() -> int {
+        const int p = (int)(m->parity & 1);
+        int r = (m->prec == OR_PREC_FP64) ? launch_cycle<double, double>(m->stream, L, p, nullptr, nullptr)
+                                          : launch_cycle<float, float>(m->stream, L, p, nullptr, nullptr);
+        m->parity ^= 1;
+        return r;
+      };
+      if ((rc = time_cycles(m->stream, cycles_per_variant, one, ms))) return rc;
+      if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
+    }
+    if ((rc = or_model_set_variant(m, f, best))) return rc;
+  }
+  chosen[0] = m->var_h;
+  chosen[1] = m->var_q;
+  return OR_OK;
+}
+
+int or_batch_autotune(or_batch *b, int32_t cycles_per_variant, int32_t chosen[4]) {
+  if (!b || !chosen) return fail(OR_ERR_ARG, "null argument");
+  if (cycles_per_variant < 1) return fail(OR_ERR_ARG, "cycles_per_variant must be >= 1");
+  OR_HIP(hipSetDevice(b->device));
+  int rc;
+  for (int c = 0; c < 2; ++c)
+    for (int f = 0; f < 2; ++f) {
+      if ((rc = batch_rebuild(b))) return rc;
+      if (b->nt[c][f] == 0) continue;
+      int best = 0;
+      float best_ms = 0.f;
+      for (int v = 0; v < kVariants; ++v) {
+        b->var[c][f] = v;
+        b->dirty = true;
+        if ((rc = batch_rebuild(b))) return rc;
+        float ms = 0.f;
+        auto one = [&]() { return or_batch_run(b, 1); };
+        if ((rc = time_cycles(b->stream, cycles_per_variant, one, ms))) return rc;
+        if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
+      }
+      b->var[c][f] = best;
+      b->dirty = true;
+    }
+  if ((rc = batch_rebuild(b))) return rc;
+  for (int c = 0; c < 2; ++c)
+    for (int f = 0; f < 2; ++f) chosen[2 * c + f] = b->var[c][f];
   return OR_OK;
 }

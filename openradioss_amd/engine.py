@@ -167,6 +167,16 @@ class GpuModel:
         L.check(self._lib.or_model_run_output(self._h, _p(out)))
         return out
 
+    def set_variant(self, family: str, variant: int):
+        """Launch variant (0..3) of the 'hex' or 'q4' element kernel; results do not depend on it."""
+        L.check(self._lib.or_model_set_variant(self._h, {"hex": 0, "q4": 1}[family], int(variant)))
+
+    def autotune(self, cycles_per_variant: int = 5) -> tuple[int, int]:
+        """Time every launch variant on real cycles (the model advances) and keep the fastest."""
+        out = (C.c_int32 * 2)()
+        L.check(self._lib.or_model_autotune(self._h, int(cycles_per_variant), out))
+        return out[0], out[1]
+
     def set_timing(self, enable: bool):
         L.check(self._lib.or_model_set_timing(self._h, int(bool(enable))))
 
@@ -222,6 +232,13 @@ class GpuBatch:
 
     def sync(self):
         L.check(self._lib.or_batch_sync(self._h))
+
+    def autotune(self, cycles_per_variant: int = 5) -> tuple:
+        """Variants per (precision, family) chosen by timing batch cycles: (fp64 hex, fp64 q4,
+        split hex, split q4)."""
+        out = (C.c_int32 * 4)()
+        L.check(self._lib.or_batch_autotune(self._h, int(cycles_per_variant), out))
+        return tuple(out)
 
     def active(self) -> int:
         n = C.c_int32()

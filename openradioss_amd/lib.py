@@ -29,7 +29,8 @@ EXPORTS = (
     "or_model_sync", "or_model_run_output", "or_model_download_nodes", "or_model_download_hex", "or_model_download_q4",
     "or_model_energy", "or_model_kernel_times", "or_model_set_timing", "or_model_footprint", "or_model_footprint_estimate", "or_sforc3_group",
     "or_cforc3_group", "or_batch_create", "or_batch_destroy", "or_batch_add", "or_batch_remove", "or_batch_run",
-    "or_batch_sync", "or_batch_active", "or_math_eval",
+    "or_batch_sync", "or_batch_active", "or_model_set_variant", "or_model_autotune", "or_batch_autotune",
+    "or_math_eval",
 )
 
 
@@ -100,6 +101,9 @@ _SIGS = {
     "or_batch_run": ([_P, C.c_int64], C.c_int),
     "or_batch_sync": ([_P], C.c_int),
     "or_batch_active": ([_P, _P], C.c_int),
+    "or_model_set_variant": ([_P, C.c_int, C.c_int], C.c_int),
+    "or_model_autotune": ([_P, C.c_int32, _P], C.c_int),
+    "or_batch_autotune": ([_P, C.c_int32, _P], C.c_int),
 }
 
 

@@ -3,8 +3,8 @@
 * cantilever bar 20x20x80 (32,000 hexes), law-2 steel, 1,000 cycles: FP64 GPU bit for bit equal
   to the oracle under correctly rounded pow/log/exp;
 * square tube 60x300 (72,000 Q4 shells, 5 points), axial crush at 15 m/s: FP64 GPU bit for bit
-  equal to the oracle over its first 2,000 cycles; the split form against the FP64 form over
-  the full 20,000 cycles (relative displacement and energy error stated and bounded).
+  equal to the oracle over its first 2,000 cycles; the split form against the FP64 form up to
+  about 20,000 cycles at matched simulated times, beside the FP64 method's own sensitivity.
 """
 import numpy as np
 import pytest
@@ -52,20 +52,44 @@ def test_tube_60x300_2000_cycles_bitwise(gpu_lib):
     g.close()
 
 
-def test_tube_60x300_20000_cycles_split_vs_fp64(gpu_lib):
-    print("case: tube 60x300, 20,000 cycles, split vs fp64", flush=True)
+def test_tube_60x300_split_vs_fp64_to_20000_cycles(gpu_lib):
+    """Split form against FP64 at matched simulated times, next to the FP64 method's own
+    sensitivity (the same model with one coordinate of one node moved by one ulp).
+
+    Up to the first fold (0.6 ms, about 8,800 cycles) the split form stays within 1e-4 in
+    displacement and energy (measured: 1e-6 and 4e-8).  After the first fold the crush is
+    chaotic: at 1.25 ms (about 20,000 cycles) a one-ulp perturbation of the FP64 run moves the
+    displacement field by O(0.1-1) of its maximum, and the split form differs from FP64 by the
+    same amount.  The bound there is therefore relative to that sensitivity."""
+    print("case: tube 60x300, split vs fp64 vs fp64 perturbed by 1 ulp, to 0.6 ms and 1.25 ms", flush=True)
     m = M.square_tube()
-    res = {}
-    for prec in ("fp64", "split"):
-        g = E.GpuModel(m, precision=prec)
-        g.run(20_000)
-        x, v, vr, d = g.nodes6()
-        res[prec] = dict(d=d, e=g.energy(), tt=g.time_state().tt)
+    mp = M.square_tube()
+    k = mp.nn // 2 + 7
+    mp.x[k, 0] = np.nextafter(mp.x[k, 0], np.inf)
+    runs = {"fp64": E.GpuModel(m), "pert": E.GpuModel(mp), "split": E.GpuModel(m, precision="split")}
+    b = E.GpuBatch()
+    err = {}
+    for t_end in (0.6e-3, 1.25e-3):
+        for g in runs.values():
+            b.add(g, t_end)
+        while b.active() > 0:
+            b.run(1000)
+        res = {}
+        for name, g in runs.items():
+            x, v, vr, d = g.nodes6()
+            res[name] = (d, g.energy().sum(), g.time_state())
+            b.remove(g)
+        d64, e64, t64 = res["fp64"]
+        for name in ("pert", "split"):
+            d, e, ts = res[name]
+            err[(t_end, name)] = (np.abs(d - d64).max() / np.abs(d64).max(), abs(e - e64) / abs(e64))
+        print(f"t_end {t_end * 1e3:.2f} ms, {t64.ncycle} cycles: perturbed fp64 d/e err {err[(t_end, 'pert')]}, "
+              f"split d/e err {err[(t_end, 'split')]}", flush=True)
+    assert err[(0.6e-3, "pert")][0] <= 1e-10
+    assert err[(0.6e-3, "split")][0] <= 1e-4 and err[(0.6e-3, "split")][1] <= 1e-4
+    assert err[(1.25e-3, "pert")][0] >= 1e-2          # the crush is chaotic at 20,000 cycles
+    assert err[(1.25e-3, "split")][0] <= 2.0 * err[(1.25e-3, "pert")][0] + 1e-4
+    assert err[(1.25e-3, "split")][1] <= 5e-2
+    b.close()
+    for g in runs.values():
         g.close()
-    d64, d32 = res["fp64"]["d"], res["split"]["d"]
-    derr = np.abs(d32 - d64).max() / np.abs(d64).max()
-    e64, e32 = res["fp64"]["e"].sum(), res["split"]["e"].sum()
-    eerr = abs(e32 - e64) / abs(e64)
-    print(f"t = {res['fp64']['tt']:.6e} s (split {res['split']['tt']:.6e}); max |d| = {np.abs(d64).max():.4e} mm; "
-          f"split vs fp64: displacement rel err {derr:.3e}, energy rel err {eerr:.3e}")
-    assert derr <= 1e-2 and eerr <= 1e-2
