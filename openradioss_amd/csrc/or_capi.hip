@@ -218,6 +218,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   int mstop = 0;
   if (i < e.ne) {
     const R DT1 = R(ts[parity].dt2);
+    HexState<S> st;
+    HexOut<R> o;
     double xd[8], yd[8], zd[8];
     R vx[8], vy[8], vz[8];
 #pragma unroll
@@ -227,23 +229,22 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
       xd[k] = a; yd[k] = b; zd[k] = cc;
       vx[k] = R(u); vy[k] = R(v); vz[k] = R(w);
     }
-    HexState<S> st;
 #pragma unroll
     for (int k = 0; k < 6; ++k) st.sig[k] = e.sig[(int64_t)k * e.ne + i];
     st.eint = e.eint[i]; st.rho = e.rho[i]; st.vol0 = e.vol0[i]; st.pla = e.pla[i];
     st.epsd = e.epsd[i]; st.offg = e.offg[i]; st.qvis = e.qvis[i];
-    HexOut<R> o;
-    hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o);
+    // each node's force triple goes to its skyline row as soon as it is final
+    auto write_row = [&](int k, R fx, R fy, R fz) {
+      if (stopped) return;
+      const gptr<S> r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
+      r[0] = S(fx); r[kBlock] = S(fy); r[2 * kBlock] = S(fz);
+    };
+    hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, write_row);
     if (!stopped) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) e.sig[(int64_t)k * e.ne + i] = st.sig[k];
       e.eint[i] = st.eint; e.rho[i] = st.rho; e.pla[i] = st.pla;
       e.epsd[i] = st.epsd; e.offg[i] = st.offg; e.qvis[i] = st.qvis;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const gptr<S> r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
-        r[0] = S(o.f1[k]); r[kBlock] = S(o.f2[k]); r[2 * kBlock] = S(o.f3[k]);
-      }
     }
     dtx = o.dtx;
     mstop = o.mstop;
@@ -588,19 +589,18 @@ __global__ __launch_bounds__(kBlock) void k_sforc3_group(int nel, int64_t nft, c
   st.eint = eint[i]; st.rho = rho[i]; st.vol0 = vol[i]; st.pla = pla[i];
   st.epsd = epsd[i]; st.offg = off[i]; st.qvis = qvis[i];
   HexOut<double> o;
-  hex_element<double, double>(xd, yd, zd, vx, vy, vz, st, c, dt1, o);
+  auto write_row = [&](int k, double fx, double fy, double fz) {
+    const int64_t K = (int64_t)iads[8 * col + k] - 1;
+    fsky[8 * K + 0] = fx;
+    fsky[8 * K + 1] = fy;
+    fsky[8 * K + 2] = fz;
+    fsky[8 * K + 6] = o.sti4;
+  };
+  hex_element<double, double>(xd, yd, zd, vx, vy, vz, st, c, dt1, o, write_row);
 #pragma unroll
   for (int k = 0; k < 6; ++k) sig[(int64_t)k * nel + i] = st.sig[k];
   eint[i] = st.eint; rho[i] = st.rho; pla[i] = st.pla;
   epsd[i] = st.epsd; off[i] = st.offg; qvis[i] = st.qvis;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int64_t K = (int64_t)iads[8 * col + k] - 1;
-    fsky[8 * K + 0] = o.f1[k];
-    fsky[8 * K + 1] = o.f2[k];
-    fsky[8 * K + 2] = o.f3[k];
-    fsky[8 * K + 6] = o.sti4;
-  }
   dtx_out[i] = o.dtx;
   mstop_out[i] = o.mstop;
 }

@@ -72,32 +72,66 @@ template <typename S> struct HexState {
   S eint, rho, vol0, pla, epsd, offg, qvis;
 };
 
-// Output of one element: 8 nodal force triples, STI/4, DTX (or +EP06 when excluded).
+// Scalar output of one element: STI/4, DTX (or +EP06 when excluded), MSTOP, EHOU.  The
+// 8 nodal force triples leave through the caller's write(node, fx, fy, fz).
 template <typename R> struct HexOut {
-  R f1[8], f2[8], f3[8];
   R sti4;
   double dtx;
   int mstop;
   R ehou;
 };
 
-// xd/yd/zd: gathered coordinates (FP64, X in the reference); vx/vy/vz gathered velocities.
-template <typename R, typename S>
-__device__ __forceinline__ void hex_element(const double (&xd)[8], const double (&yd)[8], const double (&zd)[8],
-                                            R (&vx)[8], R (&vy)[8], R (&vz)[8], HexState<S> &st,
-                                            const HexConsts &c, const R DT1, HexOut<R> &o) {
-  const R ZERO = R(0), ONE = R(1), TWO = R(2), THREE = R(3), FOUR = R(4);
-  const R HALF = R(1.0 / 2.0), THIRD = R(1.0 / 3.0), FOURTH = R(1.0 / 4.0);
-  const R ONE_OVER_64 = R(1.0 / 64.0), TWO_THIRD = R(2.0 / 3.0), HUNDRED = R(100.0);
-  const R EM15 = R(kEM15), EM20 = R(kEM20);
-  o.mstop = 0;
-  // ---- SCOOR3 (scoor3.F:233-258 OFF; 327-340 OFFG<0 zero velocities) ----
-  R offg = R(st.offg);
-  R off = fabsR(offg);
-  if (offg < ZERO) {
+// SHVIS3 hourglass force of node j (0..7) for one direction, h = H(1..4) of that direction:
+// F = -G(j,1)*H1 - G(j,2)*H2 - G(j,3)*H3 -/+ H4 with G(j,m) = +-1 -/+ PXH(m, P(j)) as SHVIS3
+// forms them (P = 1,2,3,4,3,4,1,2; minus for nodes 1-4, plus for nodes 5-8).
+template <typename R>
+__device__ __forceinline__ R hg_node_force(int j, const R (&PXH)[3][4], const R (&h)[4]) {
+  constexpr signed char kS[3][8] = {{1, -1, 1, -1, 1, -1, 1, -1},
+                                    {1, 1, -1, -1, -1, -1, 1, 1},
+                                    {1, -1, -1, 1, -1, 1, 1, -1}};
+  constexpr int kP[8] = {0, 1, 2, 3, 2, 3, 0, 1};
+  constexpr signed char kH4[8] = {-1, 1, -1, 1, 1, -1, 1, -1}; // sign of H4
+  R G[3];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) { vx[k] = ZERO; vy[k] = ZERO; vz[k] = ZERO; }
-  }
+  for (int m = 0; m < 3; ++m)
+    G[m] = (j < 4) ? R(kS[m][j]) - PXH[m][kP[j]] : R(kS[m][j]) + PXH[m][kP[j]];
+  const R a = -G[0] * h[0] - G[1] * h[1] - G[2] * h[2];
+  return kH4[j] > 0 ? a + h[3] : a - h[3];
+}
+
+// The SFORC3 chain of one element in three phases with the operation order of the
+// reference: geometry (coordinates only), kinematics (velocities), then material
+// and forces (element state).  A kernel can load each phase's inputs just before
+// the phase, which keeps fewer values live at once; hex_element runs all three.
+
+// Geometry: SCOOR3 OFF, SDERI3, SCHKJABT3, the PXiHj projections, SDLEN3.
+template <typename R> struct HexGeom {
+  R PX1, PX2, PX3, PX4, PY1, PY2, PY3, PY4, PZ1, PZ2, PZ3, PZ4;
+  R PXH[3][4];
+  R VOLN, DELTAX;
+  R off, offg;
+  bool zero_vel; // OFFG < 0 on entry: SCOOR3 zeroes the velocities
+  int mstop;
+};
+
+// Kinematics: SDEFO3 (JCVT=0) and the hourglass velocity projections of SHVIS3.
+template <typename R> struct HexKin {
+  R DXX, DYY, DZZ, D4, D5, D6, WXX, WYY, WZZ;
+  R HG[3][4]; // [x,y,z][mode]
+};
+
+// xd/yd/zd: gathered coordinates (FP64, X in the reference); offg_in: GBUF%OFF.
+template <typename R>
+__device__ __forceinline__ void hex_geometry(const double (&xd)[8], const double (&yd)[8], const double (&zd)[8],
+                                             double offg_in, HexGeom<R> &g) {
+  const R ZERO = R(0), ONE = R(1), FOUR = R(4);
+  const R ONE_OVER_64 = R(1.0 / 64.0);
+  const R EM20 = R(kEM20);
+  g.mstop = 0;
+  // ---- SCOOR3 (scoor3.F:233-258 OFF; 327-340 OFFG<0 zero velocities) ----
+  R offg = R(offg_in);
+  R off = fabsR(offg);
+  g.zero_vel = offg < ZERO;
   // ---- SDERI3 (sderi3.F): differences in FP64, Jacobian terms in R ----
   R JAC1, JAC2, JAC3, JAC4, JAC5, JAC6, JAC7, JAC8, JAC9;
   {
@@ -122,9 +156,11 @@ __device__ __forceinline__ void hex_element(const double (&xd)[8], const double 
     DET = ONE;
   } else if (offg > ONE) {
   } else if (DET <= ZERO) {
-    DET = ONE; off = ZERO; offg = ZERO; o.mstop = 1;
+    DET = ONE; off = ZERO; offg = ZERO; g.mstop = 1;
   }
-  R PX1, PX2, PX3, PX4, PY1, PY2, PY3, PY4, PZ1, PZ2, PZ3, PZ4;
+  R &PX1 = g.PX1, &PX2 = g.PX2, &PX3 = g.PX3, &PX4 = g.PX4;
+  R &PY1 = g.PY1, &PY2 = g.PY2, &PY3 = g.PY3, &PY4 = g.PY4;
+  R &PZ1 = g.PZ1, &PZ2 = g.PZ2, &PZ3 = g.PZ3, &PZ4 = g.PZ4;
   {
     R DETT = ONE_OVER_64 / DET;
     R JACI1 = DETT * JAC_59_68, JACI4 = DETT * JAC_67_49, JACI7 = DETT * JAC_48_57;
@@ -142,7 +178,7 @@ __device__ __forceinline__ void hex_element(const double (&xd)[8], const double 
     PX4 = JAC12 - JACI3; PY4 = JAC45 - JACI6; PZ4 = JAC78 - JACI9;
   }
   // hourglass projections PXiHj (JHBE/=0); the H vectors are FP64 sums, then rounded to R
-  R PXH[3][4];
+  R (&PXH)[3][4] = g.PXH;
   {
     double hx, hy, hz;
     hx = (xd[0] - xd[1] + xd[2] - xd[3] + xd[4] - xd[5] + xd[6] - xd[7]);
@@ -172,7 +208,7 @@ __device__ __forceinline__ void hex_element(const double (&xd)[8], const double 
   }
   R VOLN = DET;
   // ---- SDLEN3 / SLEN: characteristic length (coordinates rounded to R like X1..Z8) ----
-  R DELTAX;
+  R &DELTAX = g.DELTAX;
   {
     R x[8], y[8], z[8];
 #pragma unroll
@@ -192,8 +228,27 @@ __device__ __forceinline__ void hex_element(const double (&xd)[8], const double 
     AREAM = dmaxT(ar, AREAM);
     DELTAX = FOUR * VOLN * ONE / sqrtR(AREAM);
   }
+  g.VOLN = VOLN;
+  g.off = off;
+  g.offg = offg;
+}
+
+// vx/vy/vz: gathered velocities (zeroed here when SCOOR3 asks for it).
+template <typename R>
+__device__ __forceinline__ void hex_kinematics(R (&vx)[8], R (&vy)[8], R (&vz)[8], const HexGeom<R> &g, const R DT1,
+                                               HexKin<R> &kn) {
+  const R ZERO = R(0), ONE = R(1), HALF = R(1.0 / 2.0);
+  if (g.zero_vel) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) { vx[k] = ZERO; vy[k] = ZERO; vz[k] = ZERO; }
+  }
+  const R PX1 = g.PX1, PX2 = g.PX2, PX3 = g.PX3, PX4 = g.PX4;
+  const R PY1 = g.PY1, PY2 = g.PY2, PY3 = g.PY3, PY4 = g.PY4;
+  const R PZ1 = g.PZ1, PZ2 = g.PZ2, PZ3 = g.PZ3, PZ4 = g.PZ4;
+  const R(&PXH)[3][4] = g.PXH;
   // ---- SDEFO3 (JCVT=0, JHBE=1 branch) ----
-  R DXX, DYY, DZZ, D4, D5, D6, WXX, WYY, WZZ;
+  R &DXX = kn.DXX, &DYY = kn.DYY, &DZZ = kn.DZZ, &D4 = kn.D4, &D5 = kn.D5, &D6 = kn.D6;
+  R &WXX = kn.WXX, &WYY = kn.WYY, &WZZ = kn.WZZ;
   {
     R VX17 = vx[0] - vx[6], VX28 = vx[1] - vx[7], VX35 = vx[2] - vx[4], VX46 = vx[3] - vx[5];
     R VY17 = vy[0] - vy[6], VY28 = vy[1] - vy[7], VY35 = vy[2] - vy[4], VY46 = vy[3] - vy[5];
@@ -215,7 +270,7 @@ __device__ __forceinline__ void hex_element(const double (&xd)[8], const double 
   }
   // ---- hourglass velocity projections of SHVIS3 (JHBE>=1); they depend only on
   //      PXiHj and the velocities, so they are formed here to free the velocities ----
-  R HG[3][4]; // [x,y,z][mode]
+  R (&HG)[3][4] = kn.HG;
   {
     R G11 = ONE - PXH[0][0], G21 = -ONE - PXH[0][1], G31 = ONE - PXH[0][2], G41 = -ONE - PXH[0][3];
     R G51 = ONE + PXH[0][2], G61 = -ONE + PXH[0][3], G71 = ONE + PXH[0][0], G81 = -ONE + PXH[0][1];
@@ -233,6 +288,29 @@ __device__ __forceinline__ void hex_element(const double (&xd)[8], const double 
     OR_HG(vz, HG[2]);
 #undef OR_HG
   }
+}
+
+// Material and forces: SRHO3, SROTA3, MMAIN / M2LAW, MSTRAIN_RATE, MQVISCB, SMALLB3,
+// SHVIS3 forces, SFINT3, SCUMU3P.  Updates the element state st and fills o.
+template <typename R, typename S, typename Write>
+__device__ __forceinline__ void hex_material_forces(const HexGeom<R> &g, const HexKin<R> &kn, HexState<S> &st,
+                                                    const HexConsts &c, const R DT1, HexOut<R> &o, Write &&write) {
+  const R ZERO = R(0), ONE = R(1), TWO = R(2), THREE = R(3);
+  const R HALF = R(1.0 / 2.0), THIRD = R(1.0 / 3.0), FOURTH = R(1.0 / 4.0);
+  const R TWO_THIRD = R(2.0 / 3.0), HUNDRED = R(100.0);
+  const R EM15 = R(kEM15), EM20 = R(kEM20);
+  o.mstop = g.mstop;
+  const R PX1 = g.PX1, PX2 = g.PX2, PX3 = g.PX3, PX4 = g.PX4;
+  const R PY1 = g.PY1, PY2 = g.PY2, PY3 = g.PY3, PY4 = g.PY4;
+  const R PZ1 = g.PZ1, PZ2 = g.PZ2, PZ3 = g.PZ3, PZ4 = g.PZ4;
+  const R(&PXH)[3][4] = g.PXH;
+  const R(&HG)[3][4] = kn.HG;
+  const R DXX = kn.DXX, DYY = kn.DYY, DZZ = kn.DZZ, D4 = kn.D4, D5 = kn.D5, D6 = kn.D6;
+  const R WXX = kn.WXX, WYY = kn.WYY, WZZ = kn.WZZ;
+  const R DELTAX = g.DELTAX;
+  R VOLN = g.VOLN;
+  const R off = g.off;
+  R offg = g.offg;
   // ---- SRHO3 (Lagrangian) ----
   const R RHO0 = R(c.rho0);
   const R volo = R(st.vol0);
@@ -403,7 +481,13 @@ __device__ __forceinline__ void hex_element(const double (&xd)[8], const double 
     else offg = off;
   }
   st.offg = S(offg);
-  // ---- SHVIS3 forces (INVSTR>=35: CAQ from GEO(13); FLUID=0; ISMSTR=4; JHBE>=1) ----
+  // ---- SHVIS3 forces (INVSTR>=35: CAQ from GEO(13); FLUID=0; ISMSTR=4; JHBE>=1), SFINT3 and
+  //      SCUMU3P, node pair by node pair: each force is the node's SHVIS3 hourglass force,
+  //      from which SFINT3 subtracts (nodes 1-4) or to which it adds (nodes 7, 8, 5, 6) the
+  //      stress term of the pair, as the two routines do in sequence; deleted elements
+  //      (OFFG < 0) contribute zero.  Forces leave through write(node, fx, fy, fz). ----
+  o.sti4 = FOURTH * STI;
+  o.dtx = dtx_out;
   {
     R CAQ = FOURTH * off * R(c.qh);
     R FCL = CAQ * rhon * powR(VOLN, TWO_THIRD);
@@ -414,57 +498,52 @@ __device__ __forceinline__ void hex_element(const double (&xd)[8], const double 
     for (int d = 0; d < 3; ++d)
 #pragma unroll
       for (int j = 0; j < 4; ++j) H[d][j] = HG[d][j] * (FCL + fabsR(HG[d][j]) * FCQ);
-    // F(node) = -G(node,1)*H1 - G(node,2)*H2 - G(node,3)*H3 +/- H4
-    R G11 = ONE - PXH[0][0], G21 = -ONE - PXH[0][1], G31 = ONE - PXH[0][2], G41 = -ONE - PXH[0][3];
-    R G51 = ONE + PXH[0][2], G61 = -ONE + PXH[0][3], G71 = ONE + PXH[0][0], G81 = -ONE + PXH[0][1];
-    R G12 = ONE - PXH[1][0], G22 = ONE - PXH[1][1], G32 = -ONE - PXH[1][2], G42 = -ONE - PXH[1][3];
-    R G52 = -ONE + PXH[1][2], G62 = -ONE + PXH[1][3], G72 = ONE + PXH[1][0], G82 = ONE + PXH[1][1];
-    R G13 = ONE - PXH[2][0], G23 = -ONE - PXH[2][1], G33 = -ONE - PXH[2][2], G43 = ONE - PXH[2][3];
-    R G53 = -ONE + PXH[2][2], G63 = ONE + PXH[2][3], G73 = ONE + PXH[2][0], G83 = -ONE + PXH[2][1];
-#define OR_HF(F, h)                                       \
-  F[0] = -G11 * h[0] - G12 * h[1] - G13 * h[2] - h[3];    \
-  F[1] = -G21 * h[0] - G22 * h[1] - G23 * h[2] + h[3];    \
-  F[2] = -G31 * h[0] - G32 * h[1] - G33 * h[2] - h[3];    \
-  F[3] = -G41 * h[0] - G42 * h[1] - G43 * h[2] + h[3];    \
-  F[4] = -G51 * h[0] - G52 * h[1] - G53 * h[2] + h[3];    \
-  F[5] = -G61 * h[0] - G62 * h[1] - G63 * h[2] - h[3];    \
-  F[6] = -G71 * h[0] - G72 * h[1] - G73 * h[2] + h[3];    \
-  F[7] = -G81 * h[0] - G82 * h[1] - G83 * h[2] - h[3];
-    OR_HF(o.f1, H[0]);
-    OR_HF(o.f2, H[1]);
-    OR_HF(o.f3, H[2]);
-#undef OR_HF
     // EHOU order: z modes, then x, then y (shvis3.F)
     o.ehou = DT1 * (H[2][0] * HG[2][0] + H[2][1] * HG[2][1] + H[2][2] * HG[2][2] + H[2][3] * HG[2][3] +
                     H[0][0] * HG[0][0] + H[0][1] * HG[0][1] + H[0][2] * HG[0][2] + H[0][3] * HG[0][3] +
                     H[1][0] * HG[1][0] + H[1][1] * HG[1][1] + H[1][2] * HG[1][2] + H[1][3] * HG[1][3]);
-  }
-  // ---- SFINT3: F(1..4) -= FINT, F(7,8,5,6) += FINT ----
-  {
     R s1 = (sg0 - QNEW) * VOLN, s2 = (sg1 - QNEW) * VOLN, s3 = (sg2 - QNEW) * VOLN;
     R s4 = sg3 * VOLN, s5 = sg4 * VOLN, s6 = sg5 * VOLN;
     const R PXk[4] = {PX1, PX2, PX3, PX4};
     const R PYk[4] = {PY1, PY2, PY3, PY4};
     const R PZk[4] = {PZ1, PZ2, PZ3, PZ4};
+    const bool deleted = offg < ZERO;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int op = (k < 2) ? k + 6 : k + 2; // 1-7, 2-8, 3-5, 4-6
+      R fk[3], fo[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        fk[d] = hg_node_force<R>(k, PXH, H[d]);
+        fo[d] = hg_node_force<R>(op, PXH, H[d]);
+      }
       R FINT;
       FINT = s1 * PXk[k] + s4 * PYk[k] + s6 * PZk[k];
-      o.f1[k] = o.f1[k] - FINT; o.f1[op] = o.f1[op] + FINT;
+      fk[0] = fk[0] - FINT; fo[0] = fo[0] + FINT;
       FINT = s2 * PYk[k] + s4 * PXk[k] + s5 * PZk[k];
-      o.f2[k] = o.f2[k] - FINT; o.f2[op] = o.f2[op] + FINT;
+      fk[1] = fk[1] - FINT; fo[1] = fo[1] + FINT;
       FINT = s3 * PZk[k] + s6 * PXk[k] + s5 * PYk[k];
-      o.f3[k] = o.f3[k] - FINT; o.f3[op] = o.f3[op] + FINT;
+      fk[2] = fk[2] - FINT; fo[2] = fo[2] + FINT;
+      if (deleted) {
+        fk[0] = ZERO; fk[1] = ZERO; fk[2] = ZERO;
+        fo[0] = ZERO; fo[1] = ZERO; fo[2] = ZERO;
+      }
+      write(k, fk[0], fk[1], fk[2]);
+      write(op, fo[0], fo[1], fo[2]);
     }
   }
-  // ---- SCUMU3P: deleted (OFFG<0) elements contribute zero forces; STI/4 per node ----
-  if (offg < ZERO) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) { o.f1[k] = ZERO; o.f2[k] = ZERO; o.f3[k] = ZERO; }
-  }
-  o.sti4 = FOURTH * STI;
-  o.dtx = dtx_out;
+}
+
+// The whole chain for one element (all inputs gathered beforehand).
+template <typename R, typename S, typename Write>
+__device__ __forceinline__ void hex_element(const double (&xd)[8], const double (&yd)[8], const double (&zd)[8],
+                                            R (&vx)[8], R (&vy)[8], R (&vz)[8], HexState<S> &st,
+                                            const HexConsts &c, const R DT1, HexOut<R> &o, Write &&write) {
+  HexGeom<R> g;
+  hex_geometry<R>(xd, yd, zd, double(st.offg), g);
+  HexKin<R> kn;
+  hex_kinematics<R>(vx, vy, vz, g, DT1, kn);
+  hex_material_forces<R, S>(g, kn, st, c, DT1, o, write);
 }
 
 } // namespace orgpu
