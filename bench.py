@@ -132,6 +132,45 @@ def sweep_leg(precision: str, cycles: int, warmup: int, n: int = 256):
                 sim_time_range_s=[min(tts), max(tts)])
 
 
+def sweep_full(precision: str, n: int = 256, t_end: float = 5.0e-3, step: int = 500, max_wall: float = 900.0):
+    """The whole tube sweep to its end time: every tube runs its own cycles to t_end in one batch;
+    stopped tubes leave the launches.  Reports wall time and element-updates/s."""
+    from openradioss_amd import engine as E
+    from openradioss_amd import model as M
+
+    sw = M.tube_sweep(n, seed=SEED)
+    b = E.GpuBatch()
+    gms = []
+    for m, _ in sw:
+        g = E.GpuModel(m, precision=precision)
+        b.add(g, t_end)
+        gms.append(g)
+    b.run(10)
+    variants = b.autotune(2)
+    t0 = time.perf_counter()
+    active = b.active()
+    last = t0
+    while active > 0:
+        b.run(step, sync=False)
+        active = b.active()
+        now = time.perf_counter()
+        if now - last > 30.0:
+            log(f"sweep-full {precision}: {active} tubes running after {now - t0:.0f} s")
+            last = now
+        if now - t0 > max_wall:
+            break
+    wall = time.perf_counter() - t0
+    cyc = [g.time_state().ncycle for g in gms]
+    tts = [g.time_state().tt for g in gms]
+    eu = sum(m.ne_q4 * c for (m, _), c in zip(sw, cyc))
+    b.close()
+    for g in gms:
+        g.close()
+    return dict(models=n, precision=precision, t_end=t_end, wall_s=wall, finished=active == 0,
+                element_updates=eu, element_updates_per_s=eu / wall, cycles_min=min(cyc), cycles_max=max(cyc),
+                sim_time_min=min(tts), variants=variants)
+
+
 def serve_leg(nreq: int, rate: float, time_scale: float):
     """Served crash solves: Poisson arrivals, bars and tubes of 5k-500k elements, FP64 / split,
     end times U(1, 10) ms x time_scale, admitted by HBM footprint into one running batch."""
@@ -172,11 +211,17 @@ def main():
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision form")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched tube sweep")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
+    ap.add_argument("--sweep-full", action="store_true",
+                    help="only run the whole 256-tube sweep to 5 ms (FP64 and split) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.sweep_full:
+        res = {p: sweep_full(p) for p in ("fp64", "split")}
+        print(json.dumps({"sweep_full": res}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; runs on several GPUs are not part of it",

@@ -189,6 +189,19 @@ int or_cforc3_group(int device, const or_q4_params *p, int64_t numnod, int32_t n
                     double *eint, double *epsd, double *sig, double *pla, double *epsd_ip, int64_t lsky,
                     double *fsky, const int32_t *iadc, double dt1, double *dt2t);
 
+/* ---- nodal half of a cycle, drop-in with the reference host arrays ----
+ * ASSPAR4 (asspar4.F:164-180) over FSKY(8,lsky) rows listed by ADSKY(numnod+1) (1-based
+ * CSR, FILLCNE order), the DT2 pipeline from the element minimum dt2t (resol.F:2721-2722,
+ * 6125-6129, 6352, 6494-6497), ACCELE (accele.F), BCS10 / FIXVEL (bcs10.F, fixvel.F:366-379)
+ * with the node code bits and vimp(3,numnod), VELOCITY (velocity.F:58-86), DEPLA
+ * (displacement.F:108-118) and the time advance (resol.F:8599-8613) of *ts.
+ * x, v, d, vr: X/V/D/VR(3,numnod) updated in place (vr and in NULL without rotations);
+ * stifn, stifr: STIFN/STIFR(numnod) from FSKY words 7, 8, or NULL. */
+int or_nodal_update(int device, int64_t numnod, int64_t lsky, const int64_t *adsky, const double *fsky,
+                    const double *ms, const double *in, const int32_t *code, const double *vimp, double *x,
+                    double *v, double *vr, double *d, double *stifn, double *stifr, double dt2t, double dtmx,
+                    or_time_state *ts);
+
 /* ---- batches: many resident models advanced together ----
  * No counterpart in the reference, which runs one deck per process: a batch advances
  * every member model by its own explicit cycles (its own DT2 pipeline, time step and

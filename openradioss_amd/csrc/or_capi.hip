@@ -346,6 +346,45 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   if (threadIdx.x == 0) atomicMin(M.dtmin + parity, dbits(m));
 }
 
+// ---------------------------------------------------------------- nodal update pieces
+// ACCELE (accele.F:65-80, 97-109): A = F / MS, zero where the mass (inertia) is not positive.
+__device__ __forceinline__ void accele3(double &a0, double &a1, double &a2, double mass) {
+  if (mass > 0.0) {
+    const double rtmp = 1.0 / mass;
+    a0 = a0 * rtmp; a1 = a1 * rtmp; a2 = a2 * rtmp;
+  } else {
+    a0 = 0.0; a1 = 0.0; a2 = 0.0;
+  }
+}
+
+// BCS1V (bcs1.F, global frame) and FIXVEL (fixvel.F:366-379, global frame) on one node:
+// fixed DOFs get zero acceleration and velocity, imposed ones A = (VIMP - V) / DT12.
+template <typename P>
+__device__ __forceinline__ void bcs_fixvel(int32_t code, double &ax, double &ay, double &az, double &vx, double &vy,
+                                           double &vz, double &rx, double &ry, double &rz, double &wx, double &wy,
+                                           double &wz, P vimp, double DT12) {
+  if (code & kTX) { ax = 0.0; vx = 0.0; }
+  if (code & kTY) { ay = 0.0; vy = 0.0; }
+  if (code & kTZ) { az = 0.0; vz = 0.0; }
+  if (code & kRX) { rx = 0.0; wx = 0.0; }
+  if (code & kRY) { ry = 0.0; wy = 0.0; }
+  if (code & kRZ) { rz = 0.0; wz = 0.0; }
+  if (code & kIX) ax = (vimp[0] - vx) / DT12;
+  if (code & kIY) ay = (vimp[1] - vy) / DT12;
+  if (code & kIZ) az = (vimp[2] - vz) / DT12;
+}
+
+// The DT2 pipeline (resol.F:2826, 6125-6129, 6352, 6494-6497): DT2 from the element
+// minimum dt2t, the growth limit on DT2OLD and DTMX.
+__host__ __device__ __forceinline__ double dt2_pipeline(double dt2t, double dt2old, double dt_growth, double dtmx) {
+  double DT2 = kEP06;
+  if (dt2t < DT2) DT2 = dt2t;
+  const double g = dt_growth * dt2old;
+  DT2 = (g < DT2) ? g : DT2;
+  DT2 = (dtmx < DT2) ? dtmx : DT2;
+  return DT2;
+}
+
 // ---------------------------------------------------------------- node kernel
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restrict__ md,
@@ -392,14 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
   }
   // DT2 pipeline (resol.F:2721-2722, 2826, 6125-6129, 6352, 6494-6497)
   const double DT1 = prev.dt2;
-  const double dt2t = bitsd(dtmin[parity]);
-  double DT2 = kEP06;
-  if (dt2t < DT2) DT2 = dt2t;
-  {
-    double g = dt_growth * prev.dt2old;
-    DT2 = (g < DT2) ? g : DT2;
-    DT2 = (dtmx < DT2) ? dtmx : DT2;
-  }
+  const double DT2 = dt2_pipeline(bitsd(dtmin[parity]), prev.dt2old, dt_growth, dtmx);
   const double DT12 = (1.0 / 2.0) * (DT1 + DT2);
   if (n == 0) {
     DevTime nx;
@@ -417,12 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
   if (!in_range) return;
   // ACCELE (accele.F:65-80, 97-109)
   const double msn = to_global(nd.ms)[n];
-  if (msn > 0.0) {
-    const double rtmp = 1.0 / msn;
-    ax = ax * rtmp; ay = ay * rtmp; az = az * rtmp;
-  } else {
-    ax = 0.0; ay = 0.0; az = 0.0;
-  }
+  accele3(ax, ay, az, msn);
   const int32_t code = to_global(nd.code)[n];
   // A node without shell rows and without rotational BCs has AR = 0, so VELOCITY leaves
   // its VR as it is: its VR and IN are not touched.
@@ -430,13 +457,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
   double wx = 0.0, wy = 0.0, wz = 0.0;
   gptr<v2d> pr = nullptr;
   if (nd.vr && rot) {
-    const double inn = to_global(nd.in)[n];
-    if (inn > 0.0) {
-      const double rtmp = 1.0 / inn;
-      rx = rx * rtmp; ry = ry * rtmp; rz = rz * rtmp;
-    } else {
-      rx = 0.0; ry = 0.0; rz = 0.0;
-    }
+    accele3(rx, ry, rz, to_global(nd.in)[n]);
     pr = (gptr<v2d>)to_global(nd.vr + 4 * n);
     const v2d r0 = pr[0], r1 = pr[1];
     wx = r0.x; wy = r0.y; wz = r1.x;
@@ -444,20 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
   const gptr<v2d> p = (gptr<v2d>)to_global(nd.xv + 6 * n);
   const v2d pa = p[0], pb = p[1], pc = p[2];
   double x = pa.x, y = pa.y, z = pb.x, vx = pb.y, vy = pc.x, vz = pc.y;
-  if (code) {
-    // BCS1V global frame (bcs1.F)
-    if (code & kTX) { ax = 0.0; vx = 0.0; }
-    if (code & kTY) { ay = 0.0; vy = 0.0; }
-    if (code & kTZ) { az = 0.0; vz = 0.0; }
-    if (code & kRX) { rx = 0.0; wx = 0.0; }
-    if (code & kRY) { ry = 0.0; wy = 0.0; }
-    if (code & kRZ) { rz = 0.0; wz = 0.0; }
-    // FIXVEL global frame (fixvel.F:366-379): A = (VIMP - V)/DT12
-    const gptr<const double> vimp = to_global(nd.vimp) + 3 * n;
-    if (code & kIX) ax = (vimp[0] - vx) / DT12;
-    if (code & kIY) ay = (vimp[1] - vy) / DT12;
-    if (code & kIZ) az = (vimp[2] - vz) / DT12;
-  }
+  if (code) bcs_fixvel(code, ax, ay, az, vx, vy, vz, rx, ry, rz, wx, wy, wz, to_global(nd.vimp) + 3 * n, DT12);
   if (output && M.ecrit) {
     // ECRIT's kinetic energies at its output point, between FIXVEL and VELOCITY
     // (sortie_main.F:368 from resol.F:8523): V + DT05*A with DT05 = HALF*DT1 (ecrit.F:193,
@@ -677,6 +685,50 @@ __global__ __launch_bounds__(kBlock) void k_cforc3_group(int nel, int64_t nft, c
     f[6] = o.sti; f[7] = 0.0;
   }
   dt_out[i] = o.dt;
+}
+
+// Nodal half of a cycle on the reference host layouts: ASSPAR4 over FSKY(8,LSKY) rows
+// listed by ADSKY (1-based CSR, FILLCNE order), ACCELE, BCS10, FIXVEL, VELOCITY, DEPLA on
+// X/V/VR/D(3,NUMNOD).  DT12 and DT2 are the cycle's, from the DT2 pipeline on the host.
+__global__ __launch_bounds__(kBlock) void k_nodal_dropin(int64_t nn, const int64_t *adsky, const double *fsky,
+                                                         const double *ms, const double *in, const int32_t *code,
+                                                         const double *vimp, double *x, double *v, double *vr,
+                                                         double *d, double *stifn, double *stifr, double DT12,
+                                                         double DT2) {
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= nn) return;
+  // ASSPAR4 (asspar4.F:164-180)
+  double ax = 0.0, ay = 0.0, az = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, sn = 0.0, sr = 0.0;
+  for (int64_t k = adsky[n] - 1; k < adsky[n + 1] - 1; ++k) {
+    const double *f = fsky + 8 * k;
+    ax = ax + f[0]; ay = ay + f[1]; az = az + f[2];
+    rx = rx + f[3]; ry = ry + f[4]; rz = rz + f[5];
+    sn = sn + f[6]; sr = sr + f[7];
+  }
+  if (stifn) stifn[n] = sn;
+  if (stifr) stifr[n] = sr;
+  accele3(ax, ay, az, ms[n]);
+  const bool rot = vr != nullptr;
+  double wx = 0.0, wy = 0.0, wz = 0.0;
+  if (rot) {
+    accele3(rx, ry, rz, in[n]);
+    wx = vr[3 * n]; wy = vr[3 * n + 1]; wz = vr[3 * n + 2];
+  }
+  double vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
+  const int32_t cd = code ? code[n] : 0;
+  if (cd) bcs_fixvel(cd, ax, ay, az, vx, vy, vz, rx, ry, rz, wx, wy, wz, vimp + 3 * n, DT12);
+  // VELOCITY (velocity.F:58-86)
+  vx = vx + DT12 * ax; vy = vy + DT12 * ay; vz = vz + DT12 * az;
+  v[3 * n] = vx; v[3 * n + 1] = vy; v[3 * n + 2] = vz;
+  if (rot) {
+    wx = wx + DT12 * rx; wy = wy + DT12 * ry; wz = wz + DT12 * rz;
+    vr[3 * n] = wx; vr[3 * n + 1] = wy; vr[3 * n + 2] = wz;
+  }
+  // DEPLA (displacement.F:108-118)
+  double vdt;
+  vdt = DT2 * vx; d[3 * n] = d[3 * n] + vdt; x[3 * n] = x[3 * n] + vdt;
+  vdt = DT2 * vy; d[3 * n + 1] = d[3 * n + 1] + vdt; x[3 * n + 1] = x[3 * n + 1] + vdt;
+  vdt = DT2 * vz; d[3 * n + 2] = d[3 * n + 2] + vdt; x[3 * n + 2] = x[3 * n + 2] + vdt;
 }
 
 // ---------------------------------------------------------------- host helpers
@@ -2112,5 +2164,55 @@ int or_batch_autotune(or_batch *b, int32_t cycles_per_variant, int32_t chosen[4]
   if ((rc = batch_rebuild(b))) return rc;
   for (int c = 0; c < 2; ++c)
     for (int f = 0; f < 2; ++f) chosen[2 * c + f] = b->var[c][f];
+  return OR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Nodal half of a cycle with host arrays (ASSPAR4 -> DEPLA drop-in)
+int or_nodal_update(int device, int64_t numnod, int64_t lsky, const int64_t *adsky, const double *fsky,
+                    const double *ms, const double *in, const int32_t *code, const double *vimp, double *x,
+                    double *v, double *vr, double *d, double *stifn, double *stifr, double dt2t, double dtmx,
+                    or_time_state *ts) {
+  if (!adsky || !fsky || !ms || !x || !v || !d || !ts) return fail(OR_ERR_ARG, "null argument");
+  if (vr && !in) return fail(OR_ERR_ARG, "rotational velocities need the nodal inertias IN");
+  if (numnod <= 0 || lsky < 0) return fail(OR_ERR_ARG, "bad sizes");
+  if (adsky[0] != 1 || adsky[numnod] != lsky + 1) return fail(OR_ERR_ARG, "ADSKY must run from 1 to LSKY+1");
+  for (int64_t n = 0; n < numnod; ++n) {
+    if (adsky[n + 1] < adsky[n]) return fail(OR_ERR_ARG, "ADSKY must be non-decreasing");
+    if (code && (code[n] & ~511)) return fail(OR_ERR_ARG, "node code has bits outside 0..511");
+    if (code && (code[n] & (kIX | kIY | kIZ)) && !vimp) return fail(OR_ERR_ARG, "imposed-velocity bits without vimp");
+  }
+  OR_HIP(hipSetDevice(device));
+  // DT2 pipeline on the host (resol.F:2721-2722, 6125-6129, 6352, 6494-6497)
+  const double DT1 = ts->dt2;
+  const double DT2 = dt2_pipeline(dt2t, ts->dt2old, (double)1.1f, dtmx);
+  const double DT12 = (1.0 / 2.0) * (DT1 + DT2);
+  Stage s;
+  const int64_t *dads = s.up(adsky, numnod + 1);
+  const double *dfsky = s.up(fsky, 8 * lsky), *dms = s.up(ms, numnod);
+  const double *din = in ? s.up(in, numnod) : nullptr;
+  const int32_t *dcode = code ? s.up(code, numnod) : nullptr;
+  const double *dvimp = vimp ? s.up(vimp, 3 * numnod) : nullptr;
+  double *dx = s.up(x, 3 * numnod), *dv = s.up(v, 3 * numnod), *dd = s.up(d, 3 * numnod);
+  double *dvr = vr ? s.up(vr, 3 * numnod) : nullptr;
+  double *dsn = stifn ? s.up<double>(nullptr, numnod) : nullptr;
+  double *dsr = stifr ? s.up<double>(nullptr, numnod) : nullptr;
+  if (s.e == hipSuccess) {
+    hipLaunchKernelGGL(k_nodal_dropin, dim3((unsigned)((numnod + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0, numnod,
+                       dads, dfsky, dms, din, dcode, dvimp, dx, dv, dvr, dd, dsn, dsr, DT12, DT2);
+    s.e = hipGetLastError();
+  }
+  if (s.e == hipSuccess) s.e = hipDeviceSynchronize();
+  s.down(x, dx, 3 * numnod); s.down(v, dv, 3 * numnod); s.down(d, dd, 3 * numnod);
+  if (vr) s.down(vr, dvr, 3 * numnod);
+  if (stifn) s.down(stifn, dsn, numnod);
+  if (stifr) s.down(stifr, dsr, numnod);
+  if (s.e != hipSuccess) return fail(OR_ERR_HIP, std::string("or_nodal_update: ") + hipGetErrorString(s.e));
+  ts->dt1 = DT1;
+  ts->dt2 = DT2;
+  ts->dt12 = DT12;
+  ts->dt2old = DT2;
+  ts->tt = ts->tt + DT2;
+  ts->ncycle += 1;
   return OR_OK;
 }

@@ -311,3 +311,17 @@ def math_eval(fn: str, x, y=None, device: int = 0) -> np.ndarray:
     out = np.empty_like(x)
     L.check(L.load().or_math_eval(device, code, x.size, _p(x), _p(yy), _p(out)))
     return out
+
+
+def nodal_update(adsky, fsky, ms, x, v, d, ts: L.OrTimeState, dt2t: float, dtmx: float = 1.0e20, inr=None,
+                 vr=None, code=None, vimp=None, stifn=None, stifr=None, device: int = 0):
+    """ASSPAR4 -> DEPLA with the reference host arrays (or_nodal_update): adsky 1-based CSR
+    (numnod+1), fsky (lsky,8) = FSKY(8,LSKY); x, v, d, vr (numnod,3) updated in place; ts advanced."""
+    for a in (x, v, d) + tuple(a for a in (vr, stifn, stifr) if a is not None):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    adsky = np.ascontiguousarray(adsky, np.int64)
+    L.check(L.load().or_nodal_update(device, x.shape[0], fsky.shape[0], _p(adsky), _p(_f64(fsky)), _p(_f64(ms)),
+                                     _p(None if inr is None else _f64(inr)),
+                                     _p(None if code is None else np.ascontiguousarray(code, np.int32)),
+                                     _p(None if vimp is None else _f64(vimp)), _p(x), _p(v), _p(vr), _p(d),
+                                     _p(stifn), _p(stifr), float(dt2t), float(dtmx), C.byref(ts)))

@@ -21,7 +21,7 @@ module or_gpu_mod
   public :: or_model_footprint_estimate
   public :: or_batch_create, or_batch_destroy, or_batch_add, or_batch_remove, or_batch_run, or_batch_sync
   public :: or_batch_active
-  public :: or_sforc3_group, or_cforc3_group, or_math_eval
+  public :: or_sforc3_group, or_cforc3_group, or_nodal_update, or_math_eval
 
   integer(c_int), parameter, public :: OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_HIP = 2, OR_ERR_NODEV = 3, &
                                        OR_ERR_STATE = 4, OR_ERR_MSTOP = 5
@@ -286,6 +286,19 @@ module or_gpu_mod
       real(c_double), intent(inout) :: fsky(8, *)
       real(c_double), value :: dt1
       real(c_double), intent(inout) :: dt2t
+    end function
+
+    ! ASSPAR4 -> DEPLA on the Engine's arrays: ADSKY(NUMNOD+1), FSKY(8,LSKY), MS, IN, node codes,
+    ! imposed velocities, X/V/VR/D(3,NUMNOD) updated, STIFN/STIFR out (vr, in, stifn, stifr
+    ! may be c_null_ptr through the c_ptr forms), time state advanced
+    integer(c_int) function or_nodal_update(device, numnod, lsky, adsky, fsky, ms, in, code, vimp, x, v, vr, d, &
+        stifn, stifr, dt2t, dtmx, ts) bind(c, name="or_nodal_update")
+      import :: c_int, c_int64_t, c_double, c_ptr, or_time_state_t
+      integer(c_int), value :: device
+      integer(c_int64_t), value :: numnod, lsky
+      type(c_ptr), value :: adsky, fsky, ms, in, code, vimp, x, v, vr, d, stifn, stifr
+      real(c_double), value :: dt2t, dtmx
+      type(or_time_state_t), intent(inout) :: ts
     end function
 
     integer(c_int) function or_math_eval(device, fn, n, x, y, out) bind(c, name="or_math_eval")

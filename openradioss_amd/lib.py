@@ -28,7 +28,7 @@ EXPORTS = (
     "or_model_upload_q4", "or_model_upload_skyline", "or_model_set_time", "or_model_get_time", "or_model_run",
     "or_model_sync", "or_model_run_output", "or_model_download_nodes", "or_model_download_hex", "or_model_download_q4",
     "or_model_energy", "or_model_kernel_times", "or_model_set_timing", "or_model_footprint", "or_model_footprint_estimate", "or_sforc3_group",
-    "or_cforc3_group", "or_batch_create", "or_batch_destroy", "or_batch_add", "or_batch_remove", "or_batch_run",
+    "or_cforc3_group", "or_nodal_update", "or_batch_create", "or_batch_destroy", "or_batch_add", "or_batch_remove", "or_batch_run",
     "or_batch_sync", "or_batch_active", "or_model_set_variant", "or_model_autotune", "or_batch_autotune",
     "or_math_eval",
 )
@@ -93,6 +93,7 @@ _SIGS = {
                                                                                       C.c_double, _P], C.c_int),
     "or_math_eval": ([C.c_int, C.c_int, C.c_int64, _P, _P, _P], C.c_int),
     "or_model_run_output": ([_P, _P], C.c_int),
+    "or_nodal_update": ([C.c_int, C.c_int64, C.c_int64] + [_P] * 12 + [C.c_double, C.c_double, _P], C.c_int),
     "or_device_memory": ([C.c_int, _P, _P], C.c_int),
     "or_batch_create": ([_P, C.c_int], C.c_int),
     "or_batch_destroy": ([_P], C.c_int),

@@ -176,6 +176,7 @@ typedef struct {
 } or_model_c;
 
 void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads);
+void or_nodes_model(or_model_c *m, or_time_state *ts, double dt2t, int mstop);
 void or_energy_model(const or_model_c *m, double out[3]);
 
 #ifdef __cplusplus

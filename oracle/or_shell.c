@@ -674,10 +674,8 @@ void or_q4_forces(int64_t e0, int64_t e1, const int32_t *conn, const double *X, 
  * One cycle of a mixed hex + Q4 model (RESOL order, resol.F:2721-9042).
  * ---------------------------------------------------------------------- */
 void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads) {
-  const int64_t nn = m->nn;
-  ts->dt1 = ts->dt2;
-  ts->dt2 = EP06;
-  double DT2T = ts->dt2;
+  const double DT1 = ts->dt2; /* resol.F:2721-2722 */
+  double DT2T = EP06;
   int mstop = 0;
   const int64_t G = 128;
   const int64_t ngh = (m->ne_hex + G - 1) / G, ngq = (m->ne_q4 + G - 1) / G;
@@ -691,15 +689,26 @@ void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads) {
       int64_t e0 = g * G, e1 = (e0 + G < m->ne_hex) ? e0 + G : m->ne_hex;
       int ms = 0;
       or_hex_forces(e0, e1, m->ne_hex, m->hconn, m->x, m->v, m->hsig, m->heint, m->hrho, m->hvol0, m->hpla,
-                    m->hepsd, m->hoffg, m->hqvis, NULL, m->hiads, m->fsky, m->hp, ts->dt1, &dtl, &ms);
+                    m->hepsd, m->hoffg, m->hqvis, NULL, m->hiads, m->fsky, m->hp, DT1, &dtl, &ms);
       mstop |= ms;
     } else {
       int64_t e0 = (g - ngh) * G, e1 = (e0 + G < m->ne_q4) ? e0 + G : m->ne_q4;
-      or_q4_forces(e0, e1, m->qconn, m->x, m->v, m->vr, &m->q, m->qiadc, m->fsky, m->qp, ts->dt1, &dtl);
+      or_q4_forces(e0, e1, m->qconn, m->x, m->v, m->vr, &m->q, m->qiadc, m->fsky, m->qp, DT1, &dtl);
     }
     if (dtl < DT2T) DT2T = dtl;
   }
   (void)nthreads;
+  or_nodes_model(m, ts, DT2T, mstop);
+}
+
+/* The nodal half of a cycle whose element half has filled FSKY with DT1 = ts->dt2:
+ * ASSPAR4, the DT2 pipeline with the element minimum dt2t, ACCELE, BCS10, FIXVEL,
+ * (ECRIT), VELOCITY, DEPLA and the time advance. */
+void or_nodes_model(or_model_c *m, or_time_state *ts, double dt2t, int mstop) {
+  const int64_t nn = m->nn;
+  const double DT2T = dt2t;
+  ts->dt1 = ts->dt2;
+  ts->dt2 = EP06;
   ts->mstop |= mstop;
   /* ASSPAR4 (asspar4.F:164-180) */
   or_asspar4(nn, m->adsky, m->fsky, m->a, m->ar, m->stifn, m->stifr);
