@@ -233,13 +233,28 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
     for (int k = 0; k < 6; ++k) st.sig[k] = e.sig[(int64_t)k * e.ne + i];
     st.eint = e.eint[i]; st.rho = e.rho[i]; st.vol0 = e.vol0[i]; st.pla = e.pla[i];
     st.epsd = e.epsd[i]; st.offg = e.offg[i]; st.qvis = e.qvis[i];
-    // each node's force triple goes to its skyline row as soon as it is final
-    auto write_row = [&](int k, R fx, R fy, R fz) {
-      if (stopped) return;
-      const gptr<S> r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
-      r[0] = S(fx); r[kBlock] = S(fy); r[2 * kBlock] = S(fz);
-    };
-    hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, write_row);
+    if constexpr (sizeof(R) == 8) {
+      // FP64: each node's force triple goes to its skyline row as soon as it is final,
+      // which keeps the chain within two waves' register budget
+      auto write_row = [&](int k, R fx, R fy, R fz) {
+        if (stopped) return;
+        const gptr<S> r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
+        r[0] = S(fx); r[kBlock] = S(fy); r[2 * kBlock] = S(fz);
+      };
+      hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, write_row);
+    } else {
+      // FP32: the forces are gathered first and stored together (registers are not the limit)
+      R f1[8], f2[8], f3[8];
+      auto keep = [&](int k, R fx, R fy, R fz) { f1[k] = fx; f2[k] = fy; f3[k] = fz; };
+      hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, keep);
+      if (!stopped) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const gptr<S> r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
+          r[0] = S(f1[k]); r[kBlock] = S(f2[k]); r[2 * kBlock] = S(f3[k]);
+        }
+      }
+    }
     if (!stopped) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) e.sig[(int64_t)k * e.ne + i] = st.sig[k];
@@ -1704,7 +1719,7 @@ int or_model_footprint_estimate(int64_t nn, int64_t ne_hex, int64_t ne_q4, int32
     if (strain_rate_filter) b += ss * (int64_t)npt * ne_q4;            // LBUF EPSD (or_model_set_q4_params)
   }
   b += (int64_t)sizeof(DevTime) * 2 + 8 * 2 + 4 + 8 * (3 + 3 * 1024) + (int64_t)sizeof(ModelDev);
-  b += (int64_t)sizeof(Tile) * ((ne_hex + kBlock - 1) / kBlock + (ne_q4 + kBlock - 1) / kBlock + nsl);
+  b += (int64_t)sizeof(Tile) * ((ne_hex + 63) / 64 + (ne_q4 + 63) / 64 + nsl); // element tiles sized for 64
   // skyline rows in sliced-ELL form (or_model_upload_skyline): per slice, its largest row count
   for (int f = 0; f < 2; ++f) {
     const int32_t *rows = f == 0 ? rows_hex : rows_q4;

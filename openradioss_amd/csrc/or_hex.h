@@ -81,11 +81,11 @@ template <typename R> struct HexOut {
   R ehou;
 };
 
-// SHVIS3 hourglass force of node j (0..7) for one direction, h = H(1..4) of that direction:
+// SHVIS3 hourglass force of node j (0..7) in the three directions, H(d, 1..4) per direction:
 // F = -G(j,1)*H1 - G(j,2)*H2 - G(j,3)*H3 -/+ H4 with G(j,m) = +-1 -/+ PXH(m, P(j)) as SHVIS3
 // forms them (P = 1,2,3,4,3,4,1,2; minus for nodes 1-4, plus for nodes 5-8).
 template <typename R>
-__device__ __forceinline__ R hg_node_force(int j, const R (&PXH)[3][4], const R (&h)[4]) {
+__device__ __forceinline__ void hg_node_force3(int j, const R (&PXH)[3][4], const R (&H)[3][4], R (&f)[3]) {
   constexpr signed char kS[3][8] = {{1, -1, 1, -1, 1, -1, 1, -1},
                                     {1, 1, -1, -1, -1, -1, 1, 1},
                                     {1, -1, -1, 1, -1, 1, 1, -1}};
@@ -95,8 +95,11 @@ __device__ __forceinline__ R hg_node_force(int j, const R (&PXH)[3][4], const R 
 #pragma unroll
   for (int m = 0; m < 3; ++m)
     G[m] = (j < 4) ? R(kS[m][j]) - PXH[m][kP[j]] : R(kS[m][j]) + PXH[m][kP[j]];
-  const R a = -G[0] * h[0] - G[1] * h[1] - G[2] * h[2];
-  return kH4[j] > 0 ? a + h[3] : a - h[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const R a = -G[0] * H[d][0] - G[1] * H[d][1] - G[2] * H[d][2];
+    f[d] = kH4[j] > 0 ? a + H[d][3] : a - H[d][3];
+  }
 }
 
 // The SFORC3 chain of one element in three phases with the operation order of the
@@ -508,28 +511,56 @@ __device__ __forceinline__ void hex_material_forces(const HexGeom<R> &g, const H
     const R PYk[4] = {PY1, PY2, PY3, PY4};
     const R PZk[4] = {PZ1, PZ2, PZ3, PZ4};
     const bool deleted = offg < ZERO;
+    if constexpr (sizeof(R) == 8) {
+      // FP64: node pair by node pair, each pair written as soon as it is final (fewest live
+      // values; lets the FP64 chain run two waves per SIMD)
+      const bool deleted = offg < ZERO;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int op = (k < 2) ? k + 6 : k + 2; // 1-7, 2-8, 3-5, 4-6
-      R fk[3], fo[3];
+      for (int k = 0; k < 4; ++k) {
+        const int op = (k < 2) ? k + 6 : k + 2; // 1-7, 2-8, 3-5, 4-6
+        R fk[3], fo[3];
+        hg_node_force3<R>(k, PXH, H, fk);
+        hg_node_force3<R>(op, PXH, H, fo);
+        R FINT;
+        FINT = s1 * PXk[k] + s4 * PYk[k] + s6 * PZk[k];
+        fk[0] = fk[0] - FINT; fo[0] = fo[0] + FINT;
+        FINT = s2 * PYk[k] + s4 * PXk[k] + s5 * PZk[k];
+        fk[1] = fk[1] - FINT; fo[1] = fo[1] + FINT;
+        FINT = s3 * PZk[k] + s6 * PXk[k] + s5 * PYk[k];
+        fk[2] = fk[2] - FINT; fo[2] = fo[2] + FINT;
+        if (deleted) {
+          fk[0] = ZERO; fk[1] = ZERO; fk[2] = ZERO;
+          fo[0] = ZERO; fo[1] = ZERO; fo[2] = ZERO;
+        }
+        write(k, fk[0], fk[1], fk[2]);
+        write(op, fo[0], fo[1], fo[2]);
+      }
+    } else {
+      // FP32: all hourglass forces, then SFINT3 over the pairs, then the writes (registers
+      // are not the limit here and this order schedules better)
+      R f1[8], f2[8], f3[8];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        fk[d] = hg_node_force<R>(k, PXH, H[d]);
-        fo[d] = hg_node_force<R>(op, PXH, H[d]);
+      for (int j = 0; j < 8; ++j) {
+        R f[3];
+        hg_node_force3<R>(j, PXH, H, f);
+        f1[j] = f[0]; f2[j] = f[1]; f3[j] = f[2];
       }
-      R FINT;
-      FINT = s1 * PXk[k] + s4 * PYk[k] + s6 * PZk[k];
-      fk[0] = fk[0] - FINT; fo[0] = fo[0] + FINT;
-      FINT = s2 * PYk[k] + s4 * PXk[k] + s5 * PZk[k];
-      fk[1] = fk[1] - FINT; fo[1] = fo[1] + FINT;
-      FINT = s3 * PZk[k] + s6 * PXk[k] + s5 * PYk[k];
-      fk[2] = fk[2] - FINT; fo[2] = fo[2] + FINT;
-      if (deleted) {
-        fk[0] = ZERO; fk[1] = ZERO; fk[2] = ZERO;
-        fo[0] = ZERO; fo[1] = ZERO; fo[2] = ZERO;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int op = (k < 2) ? k + 6 : k + 2;
+        R FINT;
+        FINT = s1 * PXk[k] + s4 * PYk[k] + s6 * PZk[k];
+        f1[k] = f1[k] - FINT; f1[op] = f1[op] + FINT;
+        FINT = s2 * PYk[k] + s4 * PXk[k] + s5 * PZk[k];
+        f2[k] = f2[k] - FINT; f2[op] = f2[op] + FINT;
+        FINT = s3 * PZk[k] + s6 * PXk[k] + s5 * PYk[k];
+        f3[k] = f3[k] - FINT; f3[op] = f3[op] + FINT;
       }
-      write(k, fk[0], fk[1], fk[2]);
-      write(op, fo[0], fo[1], fo[2]);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (deleted) { f1[j] = ZERO; f2[j] = ZERO; f3[j] = ZERO; }
+        write(j, f1[j], f2[j], f3[j]);
+      }
     }
   }
 }

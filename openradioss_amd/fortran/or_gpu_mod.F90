@@ -20,7 +20,7 @@ module or_gpu_mod
   public :: or_model_energy, or_model_kernel_times, or_model_set_timing, or_model_footprint
   public :: or_model_footprint_estimate
   public :: or_batch_create, or_batch_destroy, or_batch_add, or_batch_remove, or_batch_run, or_batch_sync
-  public :: or_batch_active
+  public :: or_batch_active, or_model_set_variant, or_model_autotune, or_batch_autotune
   public :: or_sforc3_group, or_cforc3_group, or_nodal_update, or_math_eval
 
   integer(c_int), parameter, public :: OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_HIP = 2, OR_ERR_NODEV = 3, &
@@ -249,6 +249,27 @@ module or_gpu_mod
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: batch
       integer(c_int32_t), intent(out) :: nactive
+    end function
+
+    ! launch variants of the element kernels (speed only, results bit-identical)
+    integer(c_int) function or_model_set_variant(handle, family, variant) bind(c, name="or_model_set_variant")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: family, variant
+    end function
+
+    integer(c_int) function or_model_autotune(handle, cycles_per_variant, chosen) bind(c, name="or_model_autotune")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int32_t), value :: cycles_per_variant
+      integer(c_int32_t), intent(out) :: chosen(2)
+    end function
+
+    integer(c_int) function or_batch_autotune(batch, cycles_per_variant, chosen) bind(c, name="or_batch_autotune")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: cycles_per_variant
+      integer(c_int32_t), intent(out) :: chosen(4)
     end function
 
     ! SFORC3 on one group (sforc3.F:131-166 argument meaning): GBUF%SIG, EINT, RHO, VOL, PLA,
