@@ -200,6 +200,46 @@ def copy_probe(mib: int):
     log(f"copy probe: 3 copies of {mib} MiB")
 
 
+def stream_copy(mib: int = 1024, reps: int = 10) -> dict:
+    """Achievable HBM bandwidth: a device-to-device hipMemcpy of mib MiB (reads and writes mib
+    MiB), timed with HIP events over reps copies.  Calls the HIP runtime directly, so it runs
+    beside the path's library without another runtime on the device."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    size = mib * (1 << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ms = C.c_float()
+
+    def ok(rc):
+        if rc != 0:
+            raise RuntimeError(f"HIP error {rc} in the copy measurement")
+
+    ok(hip.hipSetDevice(0))
+    ok(hip.hipMalloc(C.byref(src), C.c_size_t(size)))
+    ok(hip.hipMalloc(C.byref(dst), C.c_size_t(size)))
+    try:
+        ok(hip.hipMemset(src, 1, C.c_size_t(size)))
+        ok(hip.hipEventCreate(C.byref(e0)))
+        ok(hip.hipEventCreate(C.byref(e1)))
+        ok(hip.hipMemcpy(dst, src, C.c_size_t(size), 3))  # hipMemcpyDeviceToDevice, warm-up
+        ok(hip.hipEventRecord(e0, None))
+        for _ in range(reps):
+            ok(hip.hipMemcpy(dst, src, C.c_size_t(size), 3))
+        ok(hip.hipEventRecord(e1, None))
+        ok(hip.hipEventSynchronize(e1))
+        ok(hip.hipEventElapsedTime(C.byref(ms), e0, e1))
+        ok(hip.hipEventDestroy(e0))
+        ok(hip.hipEventDestroy(e1))
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    t = ms.value / reps
+    gbs = 2 * size / (t * 1e-3) / 1e9
+    return {"gbs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS, "bytes_per_copy": 2 * size, "ms_per_copy": t,
+            "what": f"{mib} MiB device-to-device hipMemcpy, read + write bytes over the event time of {reps} copies"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +318,10 @@ def main():
         "kernel_variants_fp64": {"hex": r64["variants"][0], "q4": r64["variants"][1]},
         "footprint_gb_fp64": r64["footprint"] / 1e9,
     }
+    out["hbm_copy"] = stream_copy()
+    out["roofline"]["frac_of_copy"] = hex_gbs / out["hbm_copy"]["gbs"]
+    log(f"copy: {out['hbm_copy']['gbs']:.0f} GB/s ({out['hbm_copy']['frac_of_peak']:.2f} of peak); hex kernel "
+        f"{hex_gbs:.0f} GB/s")
     if not args.no_split:
         rs = gpu_run(model, "split", args.steps, args.warmup)
         eus = ne * args.steps / rs["wall"]

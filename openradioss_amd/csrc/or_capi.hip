@@ -96,6 +96,22 @@ typedef double v2d __attribute__((ext_vector_type(2))); // 16-byte load / store 
 template <typename T> __device__ __forceinline__ gptr<T> to_global(T *p) { return (gptr<T>)p; }
 template <typename T> __device__ __forceinline__ gptr<T> to_global(const void *p) { return (gptr<T>)(T *)p; }
 
+// A model's cycle scalars, read field by field with global loads: a flat load among
+// an element's global loads makes the compiler wait for all of them at its first use.
+static_assert(offsetof(DevTime, ncycle) == 40 && offsetof(DevTime, mstop) == 48 && offsetof(DevTime, done) == 52,
+              "DevTime layout");
+__device__ __forceinline__ DevTime load_time(const DevTime *t) {
+  const gptr<const double> d = to_global<const double>(t);
+  DevTime r;
+  r.tt = d[0]; r.dt1 = d[1]; r.dt2 = d[2]; r.dt12 = d[3]; r.dt2old = d[4];
+  r.ncycle = to_global<const int64_t>(t)[5];
+  const gptr<const int32_t> w = to_global<const int32_t>(t);
+  r.mstop = w[12]; r.done = w[13];
+  return r;
+}
+__device__ __forceinline__ double time_dt2(const DevTime *t) { return to_global<const double>(t)[2]; }
+__device__ __forceinline__ int32_t time_done(const DevTime *t) { return to_global<const int32_t>(t)[13]; }
+
 template <typename S> struct HexPtrs {
   gptr<const int32_t> conn;  // [8][ne]
   gptr<const int32_t> iads;  // [8][ne] row position
@@ -208,7 +224,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   const DevTime *ts = M.ts;
   // a stopped model computes but stores nothing; the flag is tested only before the
   // stores, so that its load overlaps the element's own loads
-  const int stopped = ts[parity].done;
+  const int stopped = time_done(ts + parity);
   const HexPtrs<S> e = hex_ptrs<S>(M);
   const gptr<const double> xv = to_global<const double>(M.nd.xv);
   const gptr<S> sky = to_global<S>(M.skh);
@@ -217,7 +233,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   double dtx = kEP06;
   int mstop = 0;
   if (i < e.ne) {
-    const R DT1 = R(ts[parity].dt2);
+    const R DT1 = R(time_dt2(ts + parity));
     HexState<S> st;
     HexOut<R> o;
     double xd[8], yd[8], zd[8];
@@ -290,6 +306,41 @@ template <typename R, typename S> struct Q4IpIO {
     q.pla[(int64_t)ip * q.ne + i] = S(pla);
     if (q.epsd_ip) q.epsd_ip[(int64_t)ip * q.ne + i] = S(epsd_ip);
   }
+  __device__ __forceinline__ void forc_load(R (&f)[5], R (&m)[3]) const {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) f[k] = R(q.forc[(int64_t)k * q.ne + i]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) m[k] = R(q.mom[(int64_t)k * q.ne + i]);
+  }
+  __device__ __forceinline__ void forc_store(const R (&f)[5], const R (&m)[3]) const {
+    if (!active) return;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q.forc[(int64_t)k * q.ne + i] = S(f[k]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) q.mom[(int64_t)k * q.ne + i] = S(m[k]);
+  }
+  __device__ __forceinline__ void gstr_add(const R (&inc)[8]) const {
+    if (!active) return;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const gptr<S> g = q.gstr + (int64_t)k * q.ne + i;
+      *g = S(R(*g) + inc[k]);
+    }
+  }
+  __device__ __forceinline__ void hourg_load(R (&h)[5]) const {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) h[k] = R(q.hourg[(int64_t)k * q.ne + i]);
+  }
+  __device__ __forceinline__ void hourg_store(const R (&h)[5]) const {
+    if (!active) return;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q.hourg[(int64_t)k * q.ne + i] = S(h[k]);
+  }
+  __device__ __forceinline__ void eint_add(R membrane, R bending) const {
+    if (!active) return;
+    q.eint[i] = S(R(q.eint[i]) + membrane);
+    q.eint[q.ne + i] = S(R(q.eint[q.ne + i]) + bending);
+  }
 };
 
 template <typename R, typename S, int TB, int W>
@@ -298,7 +349,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   const Tile tl = tiles[xcd_contiguous_block(blockIdx.x, gridDim.x)];
   const ModelDev &M = md[tl.model];
   const DevTime *ts = M.ts;
-  const int stopped = ts[parity].done; // tested before the stores only
+  const int stopped = time_done(ts + parity); // tested before the stores only
   const Q4Ptrs<S> q = q4_ptrs<S>(M);
   const gptr<const double> xv = to_global<const double>(M.nd.xv);
   const gptr<const double> vr = to_global<const double>(M.nd.vr);
@@ -307,7 +358,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   const int64_t i = (int64_t)tl.first + threadIdx.x;
   double dt = kEP06;
   if (i < q.ne) {
-    const R DT1 = R(ts[parity].dt2);
+    const R DT1 = R(time_dt2(ts + parity));
     double xg[4], yg[4], zg[4];
     R vl[4][3], vrl[4][3];
     int32_t nc[4];
@@ -324,29 +375,18 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
     }
     Q4Elem<R> st;
     st.offg = R(q.offg[i]); st.thk = R(q.thk[i]); st.thke = R(q.thke[i]); st.epsd = R(q.epsd[i]);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) st.forc[k] = R(q.forc[(int64_t)k * q.ne + i]);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) st.mom[k] = R(q.mom[(int64_t)k * q.ne + i]);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) st.gstr[k] = R(q.gstr[(int64_t)k * q.ne + i]);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) st.hourg[k] = R(q.hourg[(int64_t)k * q.ne + i]);
-    st.eint[0] = R(q.eint[i]); st.eint[1] = R(q.eint[q.ne + i]);
     Q4IpIO<R, S> io{q, i, !stopped};
     Q4Out<R> o;
-    q4_element<R>(xg, yg, zg, vl, vrl, nc[2] == nc[3], st, c, DT1, io, o);
+    auto nload = [&](int k, double(&x)[3], R(&v)[3]) {
+      __asm__ __volatile__("" ::: "memory"); // a fresh load, not the values held since the gather
+      double a, b, cc, u, w, z;
+      load_node(xv, nc[k], a, b, cc, u, w, z);
+      x[0] = a; x[1] = b; x[2] = cc;
+      v[0] = R(u); v[1] = R(w); v[2] = R(z);
+    };
+    q4_element<R>(xg, yg, zg, vl, vrl, nc[2] == nc[3], st, c, DT1, io, nload, o);
     if (!stopped) {
       q.offg[i] = S(st.offg); q.thk[i] = S(st.thk); q.epsd[i] = S(st.epsd);
-#pragma unroll
-      for (int k = 0; k < 5; ++k) q.forc[(int64_t)k * q.ne + i] = S(st.forc[k]);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) q.mom[(int64_t)k * q.ne + i] = S(st.mom[k]);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) q.gstr[(int64_t)k * q.ne + i] = S(st.gstr[k]);
-#pragma unroll
-      for (int k = 0; k < 5; ++k) q.hourg[(int64_t)k * q.ne + i] = S(st.hourg[k]);
-      q.eint[i] = S(st.eint[0]); q.eint[q.ne + i] = S(st.eint[1]);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const gptr<S> r = sky + row_word(q.iadc[(int64_t)k * q.ne + i], kQ4RowWords);
@@ -414,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
   unsigned long long *dtmin = M.dtmin;
   const double dtmx = M.dtmx;
   const int64_t n = (int64_t)tl.first + threadIdx.x;
-  const DevTime prev = ts[parity];
+  const DevTime prev = load_time(ts + parity);
   const bool in_range = n < nd.nn;
   // ASSPAR4: rows of this node in skyline order (solids, then shells), from zero
   double ax = 0.0, ay = 0.0, az = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
@@ -446,7 +486,8 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
   }
   // DT2 pipeline (resol.F:2721-2722, 2826, 6125-6129, 6352, 6494-6497)
   const double DT1 = prev.dt2;
-  const double DT2 = dt2_pipeline(bitsd(dtmin[parity]), prev.dt2old, dt_growth, dtmx);
+  const double DT2 = dt2_pipeline(bitsd(to_global<const unsigned long long>(dtmin)[parity]), prev.dt2old,
+                                  dt_growth, dtmx);
   const double DT12 = (1.0 / 2.0) * (DT1 + DT2);
   if (n == 0) {
     DevTime nx;
@@ -630,8 +671,36 @@ __global__ __launch_bounds__(kBlock) void k_sforc3_group(int nel, int64_t nft, c
 
 // Per-point state of the CFORC3 drop-in: LBUF arrays of one group, SIG(NEL,5) per point.
 struct GroupIpIO {
-  double *sig, *pla, *epsd_ip;
+  double *sig, *pla, *epsd_ip, *gstr, *hourg, *eint, *forc, *mom;
   int nel, i;
+  __device__ __forceinline__ void forc_load(double (&f)[5], double (&m)[3]) const {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) f[k] = forc[(int64_t)k * nel + i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) m[k] = mom[(int64_t)k * nel + i];
+  }
+  __device__ __forceinline__ void forc_store(const double (&f)[5], const double (&m)[3]) const {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) forc[(int64_t)k * nel + i] = f[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mom[(int64_t)k * nel + i] = m[k];
+  }
+  __device__ __forceinline__ void gstr_add(const double (&inc)[8]) const {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) gstr[(int64_t)k * nel + i] = gstr[(int64_t)k * nel + i] + inc[k];
+  }
+  __device__ __forceinline__ void hourg_load(double (&h)[5]) const {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) h[k] = hourg[(int64_t)k * nel + i];
+  }
+  __device__ __forceinline__ void hourg_store(const double (&h)[5]) const {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) hourg[(int64_t)k * nel + i] = h[k];
+  }
+  __device__ __forceinline__ void eint_add(double membrane, double bending) const {
+    eint[i] = eint[i] + membrane;
+    eint[nel + i] = eint[nel + i] + bending;
+  }
   __device__ __forceinline__ void load(int ip, double (&s)[5], double &p, double &e) const {
 #pragma unroll
     for (int k = 0; k < 5; ++k) s[k] = sig[((int64_t)ip * 5 + k) * nel + i];
@@ -670,28 +739,15 @@ __global__ __launch_bounds__(kBlock) void k_cforc3_group(int nel, int64_t nft, c
   }
   Q4Elem<double> st;
   st.offg = offg[i]; st.thk = thk[i]; st.thke = thke[i]; st.epsd = epsd[i];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) st.forc[k] = forc[(int64_t)k * nel + i];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) st.mom[k] = mom[(int64_t)k * nel + i];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) st.gstr[k] = gstr[(int64_t)k * nel + i];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) st.hourg[k] = hourg[(int64_t)k * nel + i];
-  st.eint[0] = eint[i]; st.eint[1] = eint[nel + i];
-  GroupIpIO io{sig, pla, epsd_ip, nel, i};
+  GroupIpIO io{sig, pla, epsd_ip, gstr, hourg, eint, forc, mom, nel, i};
   Q4Out<double> o;
-  q4_element<double>(xg, yg, zg, vl, vrl, nc[2] == nc[3], st, c, dt1, io, o);
+  auto nload = [&](int k, double(&p)[3], double(&w)[3]) {
+    const int64_t n = (int64_t)nc[k] - 1;
+    p[0] = x[3 * n]; p[1] = x[3 * n + 1]; p[2] = x[3 * n + 2];
+    w[0] = v[3 * n]; w[1] = v[3 * n + 1]; w[2] = v[3 * n + 2];
+  };
+  q4_element<double>(xg, yg, zg, vl, vrl, nc[2] == nc[3], st, c, dt1, io, nload, o);
   offg[i] = st.offg; thk[i] = st.thk; epsd[i] = st.epsd;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) forc[(int64_t)k * nel + i] = st.forc[k];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) mom[(int64_t)k * nel + i] = st.mom[k];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) gstr[(int64_t)k * nel + i] = st.gstr[k];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) hourg[(int64_t)k * nel + i] = st.hourg[k];
-  eint[i] = st.eint[0]; eint[nel + i] = st.eint[1];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     double *f = fsky + 8 * ((int64_t)iadc[4 * col + k] - 1);
@@ -815,6 +871,11 @@ int make_q4_consts(const or_q4_params &p, Q4Consts &c) {
   c.asrate = p.asrate; c.icc = p.icc; c.vp = p.vp; c.israte = p.israte; c.npt = p.npt; c.ipla = p.ipla;
   c.ithk = p.ithk; c.ishear = p.ishear; c.hm = p.hm; c.hf = p.hf; c.hr = p.hr; c.fsh = p.fsh; c.dtfac = p.dtfac;
   c.dtmin = p.dtmin;
+  {
+    volatile double a1 = p.young / (1.0 - p.nu * p.nu);
+    c.a1 = a1;
+    c.a2 = a1 * p.nu;
+  }
   {
     volatile double zep1 = 1.0 / 10.0, zep01 = 1.0 / 100.0, zep03 = 3.0 / 100.0;
     volatile double zep09 = zep1 - zep01;

@@ -43,6 +43,14 @@ template <typename T> __device__ __forceinline__ T dminT(T a, T b) { return (a <
 // form): the device library.
 __device__ __forceinline__ double powR(double a, double b) { return ormath::or_pow(a, b); }
 __device__ __forceinline__ float powR(float a, float b) { return powf(a, b); }
+// a**b1 and a**b2 (FP64: one shared logarithm, the results of two powR calls)
+__device__ __forceinline__ void pow2R(double a, double b1, double b2, double &r1, double &r2) {
+  ormath::or_pow2(a, b1, b2, r1, r2);
+}
+__device__ __forceinline__ void pow2R(float a, float b1, float b2, float &r1, float &r2) {
+  r1 = powf(a, b1);
+  r2 = powf(a, b2);
+}
 __device__ __forceinline__ double sqrtR(double a) { return sqrt(a); }
 __device__ __forceinline__ float sqrtR(float a) { return sqrtf(a); }
 __device__ __forceinline__ double logR(double a) { return ormath::or_log(a); }
@@ -397,9 +405,11 @@ __device__ __forceinline__ void hex_material_forces(const HexGeom<R> &g, const H
     AK = R(c.ca) + CB * epxe;
     QH = CB * EPD;
   } else if (epxe > ZERO) {
-    AK = R(c.ca) + CB * powR(epxe, CN);
-    if (CN > ONE) QH = (CB * CN * powR(epxe, CN - ONE)) * EPD;
-    else QH = (CB * CN / powR(epxe, ONE - CN)) * EPD;
+    R pn, pq; // EPXE**CN, and EPXE**(CN-1) or EPXE**(1-CN)
+    pow2R(epxe, CN, (CN > ONE) ? CN - ONE : ONE - CN, pn, pq);
+    AK = R(c.ca) + CB * pn;
+    if (CN > ONE) QH = (CB * CN * pq) * EPD;
+    else QH = (CB * CN / pq) * EPD;
   } else {
     AK = R(c.ca);
     QH = ZERO;
@@ -417,17 +427,16 @@ __device__ __forceinline__ void hex_material_forces(const HexGeom<R> &g, const H
     epxe = epxe + DPLA;
   }
   // ---- MQVISCB (IMPL=0, N2D=0, NPG=1, JSMS=0, VIS=VD2=0 on the Lagrangian branch) ----
-  R QNEW, STI;
+  R QNEW, STI, vol23;
   double dtx_out;
   {
     R DD = -DXX - DYY - DZZ;
     R AD = ZERO, AL = ZERO;
     R CX = SSP + ZERO; // SSP + SQRT(VD2), VD2 = 0
-    if (off == ONE) {
-      if (VOLN > ZERO) AL = powR(VOLN, ONE / THREE);
-      else AL = ZERO;
-      AD = dmaxT(ZERO, DD);
-    }
+    // AL = VOLN**(1/3) here and VOLN**(2/3) of SHVIS3 below, from one logarithm of VOLN
+    if (off == ONE && VOLN > ZERO) pow2R(VOLN, ONE / THREE, TWO_THIRD, AL, vol23);
+    else vol23 = powR(VOLN, TWO_THIRD);
+    if (off == ONE) AD = dmaxT(ZERO, DD);
     R NRHO = sqrtR(rhoref * RHO0);
     R QA = ONE * R(c.qa), QB = ONE * R(c.qb);
     R CNS1_0 = ONE * R(c.vns1), CNS2_0 = ONE * R(c.vns2);
@@ -493,7 +502,7 @@ __device__ __forceinline__ void hex_material_forces(const HexGeom<R> &g, const H
   o.dtx = dtx_out;
   {
     R CAQ = FOURTH * off * R(c.qh);
-    R FCL = CAQ * rhon * powR(VOLN, TWO_THIRD);
+    R FCL = CAQ * rhon * vol23;
     R FCQ = FCL * CAQ * HUNDRED;
     FCL = FCL * SSP;
     R H[3][4];

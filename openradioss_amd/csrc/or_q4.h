@@ -21,16 +21,20 @@ struct Q4Consts {
   double rho, young, nu, shear, ssp, ca, cb, cn, epmx, ymax, cc, epdr, m_exp, asrate;
   int32_t icc, vp, israte, npt, ipla, ithk, ishear, pad;
   double hm, hf, hr, fsh, dtfac, dtmin;
+  // plane-stress moduli of M2CPLR, A1 = E / (1 - nu^2) and A2 = A1 * nu, in FP64 on the host
+  double a1, a2;
   // parameter constants of constant_mod.F / CHVIS3, computed on the host
   double zep81, zep072169, sr2d2, srh1, srh2, srh3;
   // through-thickness rule of COQINI for npt points (Z0, WF, WM), widened from float
   double z0[10], wf[10], wm[10];
 };
 
-// Per-element shell state in registers (storage type S in memory).
+// Per-element shell state held in registers for the whole element.  GBUF%FOR, MOM,
+// STRA, HOURG and EINT are read and written through the element's accessor at the
+// point of use instead (forc_load / forc_store, gstr_add, hourg_load / hourg_store,
+// eint_add), so that they are not live across the through-thickness loop.
 template <typename R> struct Q4Elem {
-  R offg, thk, thke;
-  R forc[5], mom[3], gstr[8], hourg[5], eint[2], epsd;
+  R offg, thk, thke, epsd;
 };
 
 template <typename R> struct Q4Out {
@@ -147,87 +151,102 @@ __device__ __forceinline__ void m2cplr_point(const Q4Consts &c, const R a1, cons
   s[0] = sxx; s[1] = syy; s[2] = sxy; s[3] = syz; s[4] = szx;
 }
 
+// The convected frame and local geometry of CNVEC3 (ISHFRAM=0) and CDERI3, in FP64 from
+// the FP64 coordinates, rounded to R.
+template <typename R> struct Q4Frame {
+  R E1X, E1Y, E1Z, E2X, E2Y, E2Z, E3X, E3Y, E3Z;
+  R X2, X3, X4, Y2, Y3, Y4, PX1, PX2, PY1, PY2, AREA, VHX, VHY;
+};
+template <typename R>
+__device__ __forceinline__ void q4_frame(const double (&xg)[4], const double (&yg)[4], const double (&zg)[4],
+                                         Q4Frame<R> &f) {
+  const double DZERO = 0.0, DONE = 1.0, DHALF = 0.5, DTWO = 2.0, DEM20 = 1.0 / 1.0e20;
+  double X21 = xg[1] - xg[0], X32 = xg[2] - xg[1], X34 = xg[2] - xg[3], X41 = xg[3] - xg[0];
+  double Y21 = yg[1] - yg[0], Y32 = yg[2] - yg[1], Y34 = yg[2] - yg[3], Y41 = yg[3] - yg[0];
+  double Z21 = zg[1] - zg[0], Z32 = zg[2] - zg[1], Z34 = zg[2] - zg[3], Z41 = zg[3] - zg[0];
+  double e1x = (X21 + X34), e1y = (Y21 + Y34), e1z = (Z21 + Z34);
+  double e2x = (X32 + X41), e2y = (Y32 + Y41), e2z = (Z32 + Z41);
+  double e3x = e1y * e2z - e1z * e2y;
+  double e3y = e1z * e2x - e1x * e2z;
+  double e3z = e1x * e2y - e1y * e2x;
+  double suma = e3x * e3x + e3y * e3y + e3z * e3z;
+  suma = DONE / dmaxT(sqrt(suma), DEM20);
+  e3x = e3x * suma; e3y = e3y * suma; e3z = e3z * suma;
+  double s1 = e1x * e1x + e1y * e1y + e1z * e1z;
+  double s2 = e2x * e2x + e2y * e2y + e2z * e2z;
+  suma = sqrt(s1 / s2);
+  e1x = e1x + (e2y * e3z - e2z * e3y) * suma;
+  e1y = e1y + (e2z * e3x - e2x * e3z) * suma;
+  e1z = e1z + (e2x * e3y - e2y * e3x) * suma;
+  suma = e1x * e1x + e1y * e1y + e1z * e1z;
+  suma = DONE / dmaxT(sqrt(suma), DEM20);
+  e1x = e1x * suma; e1y = e1y * suma; e1z = e1z * suma;
+  e2x = e3y * e1z - e3z * e1y;
+  e2y = e3z * e1x - e3x * e1z;
+  e2z = e3x * e1y - e3y * e1x;
+  double X31GA = xg[2] - xg[0], Y31GA = yg[2] - yg[0], Z31GA = zg[2] - zg[0];
+  double X41GA = xg[3] - xg[0], Y41GA = yg[3] - yg[0], Z41GA = zg[3] - zg[0];
+  double x2 = e1x * X21 + e1y * Y21 + e1z * Z21;
+  double y2 = e2x * X21 + e2y * Y21 + e2z * Z21;
+  double y3 = e2x * X31GA + e2y * Y31GA + e2z * Z31GA;
+  double x3 = e1x * X31GA + e1y * Y31GA + e1z * Z31GA;
+  double x4 = e1x * X41GA + e1y * Y41GA + e1z * Z41GA;
+  double y4 = e2x * X41GA + e2y * Y41GA + e2z * Z41GA;
+  double px1 = DHALF * (y2 - y4), py1 = DHALF * (x4 - x2), px2 = DHALF * y3, py2 = -DHALF * x3;
+  double area = dmaxT(DTWO * (py2 * px1 - py1 * px2), DEM20);
+  double vhx = (-x2 + x3 - x4) / area, vhy = (-y2 + y3 - y4) / area;
+  (void)DZERO;
+  f.E1X = R(e1x); f.E1Y = R(e1y); f.E1Z = R(e1z); f.E2X = R(e2x); f.E2Y = R(e2y); f.E2Z = R(e2z);
+  f.E3X = R(e3x); f.E3Y = R(e3y); f.E3Z = R(e3z);
+  f.X2 = R(x2); f.X3 = R(x3); f.X4 = R(x4); f.Y2 = R(y2); f.Y3 = R(y3); f.Y4 = R(y4);
+  f.PX1 = R(px1); f.PX2 = R(px2); f.PY1 = R(py1); f.PY2 = R(py2); f.AREA = R(area); f.VHX = R(vhx); f.VHY = R(vhy);
+}
+
 // The CFORC3 chain for one element.  xg/yg/zg: gathered coordinates (FP64);
 // vl/vrl: gathered velocities and rotational velocities; tri: node 3 == node 4.
 // sig/pla/epsd_ip: per-point state, loaded and stored by the caller through the
-// accessor ip_io (so that the kernel chooses the memory layout).
-template <typename R, typename IPIO>
+// accessor ip_io (so that the kernel chooses the memory layout), which also reads
+// and writes FOR, MOM, STRA, HOURG and EINT.  nload(k, x, v) gives node k's coordinates and
+// velocity again after the through-thickness loop (the values of xg/yg/zg and vl), from
+// which the frame and the local velocities that CHVIS3 and CFINT3 use are formed again
+// rather than kept live across the loop.
+template <typename R, typename IPIO, typename NLOAD>
 __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (&yg)[4], const double (&zg)[4],
                                            R (&vl)[4][3], R (&vrl)[4][3], bool tri, Q4Elem<R> &st,
-                                           const Q4Consts &c, const R DT1, IPIO &ip_io, Q4Out<R> &o) {
+                                           const Q4Consts &c, const R DT1, IPIO &ip_io, NLOAD &&nload,
+                                           Q4Out<R> &o) {
   const R ZERO = R(0), ONE = R(1), TWO = R(2), THREE = R(3), HALF = R(0.5), FOURTH = R(0.25);
   const R EM20 = R(1.0 / 1.0e20), EM01 = R(1.0 / 10.0);
   const R FOUR_OVER_5 = R(4.0 / 5.0);
   // ---- CCOOR3 (ccoor3.F:68-150) ----
   R offg = st.offg;
   R off = dminT(ONE, fabsR(offg));
-  if (offg < ZERO) {
+  const bool zero_vel = offg < ZERO;
+  if (zero_vel) {
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
       for (int d = 0; d < 3; ++d) { vl[k][d] = ZERO; vrl[k][d] = ZERO; }
   }
   R thk0 = st.thke;
-  // ---- CNVEC3 (ISHFRAM=0) and CDERI3, in FP64 from FP64 coordinates ----
-  R E1X, E1Y, E1Z, E2X, E2Y, E2Z, E3X, E3Y, E3Z;
-  R X2, X3, X4, Y2, Y3, Y4, PX1, PX2, PY1, PY2, AREA, VHX, VHY;
-  {
-    const double DZERO = 0.0, DONE = 1.0, DHALF = 0.5, DTWO = 2.0, DEM20 = 1.0 / 1.0e20;
-    double X21 = xg[1] - xg[0], X32 = xg[2] - xg[1], X34 = xg[2] - xg[3], X41 = xg[3] - xg[0];
-    double Y21 = yg[1] - yg[0], Y32 = yg[2] - yg[1], Y34 = yg[2] - yg[3], Y41 = yg[3] - yg[0];
-    double Z21 = zg[1] - zg[0], Z32 = zg[2] - zg[1], Z34 = zg[2] - zg[3], Z41 = zg[3] - zg[0];
-    double e1x = (X21 + X34), e1y = (Y21 + Y34), e1z = (Z21 + Z34);
-    double e2x = (X32 + X41), e2y = (Y32 + Y41), e2z = (Z32 + Z41);
-    double e3x = e1y * e2z - e1z * e2y;
-    double e3y = e1z * e2x - e1x * e2z;
-    double e3z = e1x * e2y - e1y * e2x;
-    double suma = e3x * e3x + e3y * e3y + e3z * e3z;
-    suma = DONE / dmaxT(sqrt(suma), DEM20);
-    e3x = e3x * suma; e3y = e3y * suma; e3z = e3z * suma;
-    double s1 = e1x * e1x + e1y * e1y + e1z * e1z;
-    double s2 = e2x * e2x + e2y * e2y + e2z * e2z;
-    suma = sqrt(s1 / s2);
-    e1x = e1x + (e2y * e3z - e2z * e3y) * suma;
-    e1y = e1y + (e2z * e3x - e2x * e3z) * suma;
-    e1z = e1z + (e2x * e3y - e2y * e3x) * suma;
-    suma = e1x * e1x + e1y * e1y + e1z * e1z;
-    suma = DONE / dmaxT(sqrt(suma), DEM20);
-    e1x = e1x * suma; e1y = e1y * suma; e1z = e1z * suma;
-    e2x = e3y * e1z - e3z * e1y;
-    e2y = e3z * e1x - e3x * e1z;
-    e2z = e3x * e1y - e3y * e1x;
-    double X31GA = xg[2] - xg[0], Y31GA = yg[2] - yg[0], Z31GA = zg[2] - zg[0];
-    double X41GA = xg[3] - xg[0], Y41GA = yg[3] - yg[0], Z41GA = zg[3] - zg[0];
-    double x2 = e1x * X21 + e1y * Y21 + e1z * Z21;
-    double y2 = e2x * X21 + e2y * Y21 + e2z * Z21;
-    double y3 = e2x * X31GA + e2y * Y31GA + e2z * Z31GA;
-    double x3 = e1x * X31GA + e1y * Y31GA + e1z * Z31GA;
-    double x4 = e1x * X41GA + e1y * Y41GA + e1z * Z41GA;
-    double y4 = e2x * X41GA + e2y * Y41GA + e2z * Z41GA;
-    double px1 = DHALF * (y2 - y4), py1 = DHALF * (x4 - x2), px2 = DHALF * y3, py2 = -DHALF * x3;
-    double area = dmaxT(DTWO * (py2 * px1 - py1 * px2), DEM20);
-    double vhx = (-x2 + x3 - x4) / area, vhy = (-y2 + y3 - y4) / area;
-    (void)DZERO;
-    E1X = R(e1x); E1Y = R(e1y); E1Z = R(e1z); E2X = R(e2x); E2Y = R(e2y); E2Z = R(e2z);
-    E3X = R(e3x); E3Y = R(e3y); E3Z = R(e3z);
-    X2 = R(x2); X3 = R(x3); X4 = R(x4); Y2 = R(y2); Y3 = R(y3); Y4 = R(y4);
-    PX1 = R(px1); PX2 = R(px2); PY1 = R(py1); PY2 = R(py2); AREA = R(area); VHX = R(vhx); VHY = R(vhy);
-  }
+  // ---- CNVEC3 (ISHFRAM=0) and CDERI3 ----
+  Q4Frame<R> fr;
+  q4_frame<R>(xg, yg, zg, fr);
+  R E1X = fr.E1X, E1Y = fr.E1Y, E1Z = fr.E1Z, E2X = fr.E2X, E2Y = fr.E2Y, E2Z = fr.E2Z;
+  R E3X = fr.E3X, E3Y = fr.E3Y, E3Z = fr.E3Z;
+  R PX1 = fr.PX1, PX2 = fr.PX2, PY1 = fr.PY1, PY2 = fr.PY2, AREA = fr.AREA, VHX = fr.VHX, VHY = fr.VHY;
   // ---- CCOEF3 (ccoef3.F:84-170) ----
-  R VOL0, THK02;
   if (c.ithk > 0) thk0 = st.thk;
-  VOL0 = thk0 * AREA;
-  THK02 = thk0 * thk0;
-  R SHF;
-  {
+  // SHF of CCOEF3 and ALDT of CDLEN3 are formed from the frame here and again after the
+  // through-thickness loop (same inputs, same values), so they are not live across it
+  auto shear_factor = [&](R area, R thk02) {
     const R nu = R(c.nu), fsh = R(c.fsh), ish = R(c.ishear);
-    R fac1tmp = TWO * (ONE + nu) * THK02;
-    SHF = fsh * ((ONE - ish) + ish * fac1tmp / (fsh * AREA + fac1tmp));
-  }
-  const R GS = R(c.shear) * SHF;
-  // ---- CDLEN3 (cdlen3.F:68-124), IHBE /= 0 ----
-  R ALDT;
-  {
+    R fac1tmp = TWO * (ONE + nu) * thk02;
+    return fsh * ((ONE - ish) + ish * fac1tmp / (fsh * area + fac1tmp));
+  };
+  // CDLEN3 (cdlen3.F:68-124), IHBE /= 0
+  auto cdlen3 = [&](const Q4Frame<R> &f) {
+    const R X2 = f.X2, X3 = f.X3, X4 = f.X4, Y2 = f.Y2, Y3 = f.Y3, Y4 = f.Y4;
     R AL1 = X2 * X2 + Y2 * Y2;
     R AL2 = (X3 - X2) * (X3 - X2) + (Y3 - Y2) * (Y3 - Y2);
     R AL6 = X3 * X3 + Y3 * Y3;
@@ -237,12 +256,17 @@ __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (
     R ALMIN = dminT(dminT(AL1, AL2), AL4);
     R ALQUAD = dminT(dminT(AL3, AL5), AL6);
     if (AL3 != ZERO) ALMIN = dminT(ALMIN, ALQUAD);
-    R DTDYN = AREA * AREA / dmaxT(dmaxT(AL5, AL6), EM20);
-    ALDT = dmaxT(DTDYN, ALMIN);
-    R DTHOUR = HALF * (ALMIN + ALDT) / dmaxT(R(c.hm), R(c.hf));
-    if (DTHOUR < ALDT) ALDT = DTHOUR;
-    ALDT = sqrtR(ALDT);
-  }
+    R DTDYN = f.AREA * f.AREA / dmaxT(dmaxT(AL5, AL6), EM20);
+    R aldt = dmaxT(DTDYN, ALMIN);
+    R DTHOUR = HALF * (ALMIN + aldt) / dmaxT(R(c.hm), R(c.hf));
+    if (DTHOUR < aldt) aldt = DTHOUR;
+    return sqrtR(aldt);
+  };
+  R VOL0 = thk0 * AREA;
+  R THK02 = thk0 * thk0;
+  R SHF = shear_factor(AREA, THK02);
+  const R GS = R(c.shear) * SHF;
+  R ALDT = cdlen3(fr);
   // ---- CDEFO3, IHBE<=1 (cdefo3.F:94-140) ----
   R VX[4], VY[4], VZ[4];
 #pragma unroll
@@ -295,14 +319,16 @@ __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (
     EXZ = EXZ + RYAVTA * (R(0.25) * AREA);
     EYZ = EYZ - RXAVTA * (R(0.25) * AREA);
   }
+  // rotational hourglass rates of CHVIS3 (chvis3.F), formed here from RX / RY
+  const R HGRX = RX[0] - RX[1] + RX[2] - RX[3];
+  const R HGRY = RY[0] - RY[1] + RY[2] - RY[3];
   // ---- CSTRA3 (cstra3.F:83-93, 168-178), ISTRAIN=1 ----
   {
     R FAC1 = DT1 / AREA;
     EXX = EXX * FAC1; EYY = EYY * FAC1; EXY = EXY * FAC1; EYZ = EYZ * FAC1; EXZ = EXZ * FAC1;
     KXX = KXX * FAC1; KYY = KYY * FAC1; KXY = KXY * FAC1;
-    st.gstr[0] = st.gstr[0] + EXX; st.gstr[1] = st.gstr[1] + EYY; st.gstr[2] = st.gstr[2] + EXY;
-    st.gstr[3] = st.gstr[3] + EYZ; st.gstr[4] = st.gstr[4] + EXZ; st.gstr[5] = st.gstr[5] + KXX;
-    st.gstr[6] = st.gstr[6] + KYY; st.gstr[7] = st.gstr[7] + KXY;
+    const R inc[8] = {EXX, EYY, EXY, EYZ, EXZ, KXX, KYY, KXY};
+    ip_io.gstr_add(inc); // GSTR(k) = GSTR(k) + increment
   }
   // ---- element strain rate of CFORC3 (cforc3.F:537-546) ----
   const R dtinv = DT1 / dmaxT(DT1 * DT1, EM20);
@@ -315,28 +341,30 @@ __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (
     st.epsd = ONE * epsd_pg + (ONE - ONE) * st.epsd;
   }
   // ---- MULAWC law-2 branch ----
-  R degmb = st.forc[0] * EXX + st.forc[1] * EYY + st.forc[2] * EXY + st.forc[3] * EYZ + st.forc[4] * EXZ;
-  R degfx = st.mom[0] * KXX + st.mom[1] * KYY + st.mom[2] * KXY;
   VOL0 = AREA * thk0;
   R thkn = st.thk;
   R FOR[5] = {ZERO, ZERO, ZERO, ZERO, ZERO}, MOM[3] = {ZERO, ZERO, ZERO};
   const R off_old = off;
   R epchk = ZERO;
   bool ioff_duct = false;
-  const R a1 = R(c.young / (1.0 - c.nu * c.nu));
-  const R a2 = a1 * R(c.nu);
+  const R a1 = R(c.a1);
+  const R a2 = (sizeof(R) == 8) ? R(c.a2) : a1 * R(c.nu);
   const R epdr = dmaxT(R(c.epdr) * DT1, EM20);
   const R asrate = (c.israte > 0) ? dminT(ONE, R(c.asrate) * DT1) : ONE;
   const R nu = R(c.nu);
+  // each point's state is loaded one point ahead, so that its loads overlap the
+  // previous point's arithmetic
+  R s_next[5], pla_next, epsd_next;
+  ip_io.load(0, s_next, pla_next, epsd_next);
   for (int ip = 0; ip < c.npt; ++ip) {
     const R thkly = R(c.wf[ip]);
     const R wmc = R(c.wm[ip]);
     const R thklyl = thkly * thk0;
     const R zt = (R(c.z0[ip]) + ZERO) * thk0;
     const R dxx = EXX + zt * KXX, dyy = EYY + zt * KYY, dxy = EXY + zt * KXY;
-    R s[5];
-    R pla, epsd_ip;
-    ip_io.load(ip, s, pla, epsd_ip);
+    R s[5] = {s_next[0], s_next[1], s_next[2], s_next[3], s_next[4]};
+    R pla = pla_next, epsd_ip = epsd_next;
+    if (ip + 1 < c.npt) ip_io.load(ip + 1, s_next, pla_next, epsd_next);
     epsd_ip = asrate * epsd_pg + (ONE - asrate) * epsd_ip;
     const R epsdot = epsd_ip * DT1;
     R ezz;
@@ -363,22 +391,46 @@ __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (
     MOM[2] = MOM[2] + wmc * s[2];
   }
   if ((off == FOUR_OVER_5 && !ioff_duct) || (off > ZERO && off_old < EM01)) off = ZERO;
+  // the frame of CNVEC3 / CDERI3 again (same inputs and operations, so the same values)
+  // and the node velocities of CDEFO3 again, zeroed as SCOOR3 / CCOOR3 zero them
+  R vn[4][3];
+  {
+    double xa[4], ya[4], za[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double xk[3];
+      nload(k, xk, vn[k]);
+      xa[k] = xk[0]; ya[k] = xk[1]; za[k] = xk[2];
+      if (zero_vel) { vn[k][0] = ZERO; vn[k][1] = ZERO; vn[k][2] = ZERO; }
+    }
+    q4_frame<R>(xa, ya, za, fr);
+  }
+  E1X = fr.E1X; E1Y = fr.E1Y; E1Z = fr.E1Z; E2X = fr.E2X; E2Y = fr.E2Y; E2Z = fr.E2Z;
+  E3X = fr.E3X; E3Y = fr.E3Y; E3Z = fr.E3Z;
+  PX1 = fr.PX1; PX2 = fr.PX2; PY1 = fr.PY1; PY2 = fr.PY2; AREA = fr.AREA; VHX = fr.VHX; VHY = fr.VHY;
+  VOL0 = AREA * thk0;
+  THK02 = thk0 * thk0;
+  SHF = shear_factor(AREA, THK02);
+  ALDT = cdlen3(fr);
   st.thk = dmaxT(thkn, R(1.0 / (1.0e20 * 1.0e10)));
 #pragma unroll
   for (int k = 0; k < 5; ++k) FOR[k] = FOR[k] * off * ONE;
 #pragma unroll
   for (int k = 0; k < 3; ++k) MOM[k] = MOM[k] * off * ONE;
+  R degmb, degfx; // with the resultants of the previous cycle, then of this one
+  {
+    R fo[5], mo[3];
+    ip_io.forc_load(fo, mo);
+    degmb = fo[0] * EXX + fo[1] * EYY + fo[2] * EXY + fo[3] * EYZ + fo[4] * EXZ;
+    degfx = mo[0] * KXX + mo[1] * KYY + mo[2] * KXY;
+  }
   degmb = degmb + FOR[0] * EXX + FOR[1] * EYY + FOR[2] * EXY + FOR[3] * EYZ + FOR[4] * EXZ;
   degfx = degfx + MOM[0] * KXX + MOM[1] * KYY + MOM[2] * KXY;
   {
     R vol2 = HALF * VOL0;
-    st.eint[0] = st.eint[0] + degmb * vol2;
-    st.eint[1] = st.eint[1] + degfx * thk0 * vol2;
+    ip_io.eint_add(degmb * vol2, degfx * thk0 * vol2); // EINT(k) = EINT(k) + term
   }
-#pragma unroll
-  for (int k = 0; k < 5; ++k) st.forc[k] = FOR[k];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) st.mom[k] = MOM[k];
+  ip_io.forc_store(FOR, MOM);
   // VISCMX = 1, ALPE = 1 on this branch (DM = 0)
   // ---- CHVIS3, IHBE=1, ISMSTR=2 (chvis3.F:112-321) ----
   R H11, H12, H13, H21, H22, H23, H31, H32, H33, B1k, B2k;
@@ -418,24 +470,35 @@ __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (
     R B1 = PX1 * PX1 + PY1 * PY1, B2 = PX2 * PX2 + PY2 * PY2;
     R HH2 = A8 * THK02 * TD / (B1 + B2);
     if (tri) { H1Q = ZERO; H1L = ZERO; H2Q = ZERO; H2L = ZERO; H3Q = ZERO; H3L = ZERO; HH1 = ZERO; HH2 = ZERO; }
+    // the local velocities of CDEFO3 again
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const R(&v)[3] = vn[k];
+      VX[k] = E1X * v[0] + E1Y * v[1] + E1Z * v[2];
+      VY[k] = E2X * v[0] + E2Y * v[1] + E2Z * v[2];
+      VZ[k] = E3X * v[0] + E3Y * v[1] + E3Z * v[2];
+    }
     R HG1 = VX[0] * GAMA1 + VX[1] * GAMA2 + VX[2] * GAMA3 + VX[3] * GAMA4;
     R HG2 = VY[0] * GAMA1 + VY[1] * GAMA2 + VY[2] * GAMA3 + VY[3] * GAMA4;
-    st.hourg[0] = st.hourg[0] + HG1 * HH1;
-    st.hourg[1] = st.hourg[1] + HG2 * HH1;
-    R HOUR1A = st.hourg[0] + HG1 * (H1L + H1Q * fabsR(HG1));
+    R hourg[5];
+    ip_io.hourg_load(hourg); // HOURG(1..3); 4 and 5 are set below
+    hourg[0] = hourg[0] + HG1 * HH1;
+    hourg[1] = hourg[1] + HG2 * HH1;
+    R HOUR1A = hourg[0] + HG1 * (H1L + H1Q * fabsR(HG1));
     H11 = HOUR1A * GAMA1; H12 = HOUR1A * GAMA2; H13 = HOUR1A * GAMA3;
-    R HOUR2A = st.hourg[1] + HG2 * (H1L + H1Q * fabsR(HG2));
+    R HOUR2A = hourg[1] + HG2 * (H1L + H1Q * fabsR(HG2));
     H21 = HOUR2A * GAMA1; H22 = HOUR2A * GAMA2; H23 = HOUR2A * GAMA3;
     HG1 = VZ[0] * GAMA1 + VZ[1] * GAMA2 + VZ[2] * GAMA3 + VZ[3] * GAMA4;
-    st.hourg[2] = st.hourg[2] + HG1 * HH2;
-    R HOUR3A = st.hourg[2] + HG1 * (H2L + H2Q * fabsR(HG1));
+    hourg[2] = hourg[2] + HG1 * HH2;
+    R HOUR3A = hourg[2] + HG1 * (H2L + H2Q * fabsR(HG1));
     H31 = HOUR3A * GAMA1; H32 = HOUR3A * GAMA2; H33 = HOUR3A * GAMA3;
-    HG1 = RX[0] - RX[1] + RX[2] - RX[3];
-    HG2 = RY[0] - RY[1] + RY[2] - RY[3];
-    st.hourg[3] = HG1 * (H3L + H3Q * fabsR(HG1));
-    st.hourg[4] = HG2 * (H3L + H3Q * fabsR(HG2));
-    B1k = st.hourg[3] * off;
-    B2k = st.hourg[4] * off;
+    HG1 = HGRX;
+    HG2 = HGRY;
+    hourg[3] = HG1 * (H3L + H3Q * fabsR(HG1));
+    hourg[4] = HG2 * (H3L + H3Q * fabsR(HG2));
+    ip_io.hourg_store(hourg);
+    B1k = hourg[3] * off;
+    B2k = hourg[4] * off;
   }
   // ---- CDT3 (cdt3.F:108-112, 141-165, 200-233), IDTMIN(3)=2 ----
   R STI;

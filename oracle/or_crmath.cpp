@@ -9,6 +9,7 @@
 
 #include <math.h>
 #include <stdint.h>
+#include <string.h>
 
 extern "C" {
 double or_cr_pow(double x, double y) { return ormath::or_pow(x, y); }
@@ -74,5 +75,32 @@ void or_cr_selftest(int fn, int64_t n, uint64_t seed, int64_t *counts) {
     if (d_host < d_ours) counts[1]++;
     else if (d_ours < d_host) counts[2]++;
   }
+}
+}
+
+extern "C" {
+// Consistency of the shared-logarithm form used by the device path: or_pow2(x, y1, y2)
+// must give exactly or_pow(x, y1) and or_pow(x, y2).  Draws n bases over the pow range
+// above plus the special bases (0, 1, subnormal, huge, negative, infinite, NaN) and the
+// exponent pairs of the path (1/3 with 2/3; n with n-1 or 1-n).  Returns the number of
+// mismatching bits patterns.
+int64_t or_cr_pow2_check(int64_t n, uint64_t seed) {
+  const double special[] = {0.0, 1.0, 0x1p-1074, 0x1p-1030, 1.0e300, -2.0, INFINITY, NAN};
+  uint64_t s = seed;
+  int64_t bad = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    double x = k < 8 ? special[k] : exp(-30.0 + 44.0 * unif(s));
+    const double cn = unif(s) * 1.5;
+    double y1, y2;
+    if (k & 1) { y1 = 1.0 / 3.0; y2 = 2.0 / 3.0; }
+    else { y1 = cn; y2 = cn > 1.0 ? cn - 1.0 : 1.0 - cn; }
+    double r1, r2;
+    ormath::or_pow2(x, y1, y2, r1, r2);
+    const double p1 = ormath::or_pow(x, y1), p2 = ormath::or_pow(x, y2);
+    uint64_t b[4];
+    memcpy(&b[0], &r1, 8); memcpy(&b[1], &p1, 8); memcpy(&b[2], &r2, 8); memcpy(&b[3], &p2, 8);
+    if (b[0] != b[1] || b[2] != b[3]) bad++;
+  }
+  return bad;
 }
 }

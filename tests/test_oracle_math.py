@@ -30,6 +30,16 @@ def test_correct_rounding_against_long_double(fn):
     assert ours_closer >= 0.5 * differ
 
 
+def test_shared_logarithm_powers_equal_two_powers():
+    """or_pow2 (one logarithm for two exponents of the same base, used for VOLN**(1/3) with
+    VOLN**(2/3) and EPXE**n with EPXE**(n-1)) gives exactly the two or_pow results, special
+    bases included."""
+    L = O.lib("cr")
+    L.or_cr_pow2_check.restype = C.c_int64
+    L.or_cr_pow2_check.argtypes = [C.c_int64, C.c_uint64]
+    assert L.or_cr_pow2_check(200_000, 4242) == 0
+
+
 def cr(name):
     L = O.lib("cr")
     f = getattr(L, "or_cr_" + name)
