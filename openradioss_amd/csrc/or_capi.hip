@@ -46,13 +46,14 @@ constexpr double kNoEnd = 1.0e300; // end time of a model advanced by or_model_r
 
 // Launch variants of the element kernels.  Variant v picks the workgroup size
 // (256 elements for even v, 64 for odd v) and the minimum number of waves per SIMD
-// the register allocation is built for (the precision's base for v < 2, one more
-// for v >= 2; a wave's register budget is 512 / waves VGPRs and AGPRs together).
+// the register allocation is built for (two for v < 2, three for v >= 2, where the
+// FP64 kernels spill to scratch; a wave's register budget is 512 / waves VGPRs and
+// AGPRs together).
 // Every variant runs the same arithmetic, so the choice (or_model_set_variant,
 // or_model_autotune, or_batch_autotune) changes speed only, never a result bit.
 constexpr int kVariants = 4;
 constexpr int variant_block(int v) { return (v & 1) ? 64 : 256; }
-template <typename R> constexpr int variant_waves(int v) { return (sizeof(R) == 8 ? 1 : 2) + (v >> 1); }
+template <typename R> constexpr int variant_waves(int v) { return 2 + (v >> 1); }
 
 // node code bits (include/or_gpu.h)
 constexpr int32_t kTX = 4, kTY = 2, kTZ = 1, kRX = 32, kRY = 16, kRZ = 8, kIX = 256, kIY = 128, kIZ = 64;
@@ -306,40 +307,33 @@ template <typename R, typename S> struct Q4IpIO {
     q.pla[(int64_t)ip * q.ne + i] = S(pla);
     if (q.epsd_ip) q.epsd_ip[(int64_t)ip * q.ne + i] = S(epsd_ip);
   }
-  __device__ __forceinline__ void forc_load(R (&f)[5], R (&m)[3]) const {
+  __device__ __forceinline__ void late_load(Q4Late<R> &L) const {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) f[k] = R(q.forc[(int64_t)k * q.ne + i]);
+    for (int k = 0; k < 5; ++k) L.forc[k] = R(q.forc[(int64_t)k * q.ne + i]);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) m[k] = R(q.mom[(int64_t)k * q.ne + i]);
+    for (int k = 0; k < 3; ++k) L.mom[k] = R(q.mom[(int64_t)k * q.ne + i]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) L.gstr[k] = R(q.gstr[(int64_t)k * q.ne + i]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L.hourg[k] = R(q.hourg[(int64_t)k * q.ne + i]);
+    L.eint[0] = R(q.eint[i]);
+    L.eint[1] = R(q.eint[q.ne + i]);
   }
-  __device__ __forceinline__ void forc_store(const R (&f)[5], const R (&m)[3]) const {
+  __device__ __forceinline__ void late_store(const Q4Late<R> &L) const {
     if (!active) return;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) q.forc[(int64_t)k * q.ne + i] = S(f[k]);
+    for (int k = 0; k < 5; ++k) q.forc[(int64_t)k * q.ne + i] = S(L.forc[k]);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) q.mom[(int64_t)k * q.ne + i] = S(m[k]);
-  }
-  __device__ __forceinline__ void gstr_add(const R (&inc)[8]) const {
-    if (!active) return;
+    for (int k = 0; k < 3; ++k) q.mom[(int64_t)k * q.ne + i] = S(L.mom[k]);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const gptr<S> g = q.gstr + (int64_t)k * q.ne + i;
-      *g = S(R(*g) + inc[k]);
-    }
-  }
-  __device__ __forceinline__ void hourg_load(R (&h)[5]) const {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) h[k] = R(q.hourg[(int64_t)k * q.ne + i]);
+    for (int k = 0; k < 8; ++k) q.gstr[(int64_t)k * q.ne + i] = S(L.gstr[k]);
+    q.eint[i] = S(L.eint[0]);
+    q.eint[q.ne + i] = S(L.eint[1]);
   }
   __device__ __forceinline__ void hourg_store(const R (&h)[5]) const {
     if (!active) return;
 #pragma unroll
     for (int k = 0; k < 5; ++k) q.hourg[(int64_t)k * q.ne + i] = S(h[k]);
-  }
-  __device__ __forceinline__ void eint_add(R membrane, R bending) const {
-    if (!active) return;
-    q.eint[i] = S(R(q.eint[i]) + membrane);
-    q.eint[q.ne + i] = S(R(q.eint[q.ne + i]) + bending);
   }
 };
 
@@ -673,33 +667,31 @@ __global__ __launch_bounds__(kBlock) void k_sforc3_group(int nel, int64_t nft, c
 struct GroupIpIO {
   double *sig, *pla, *epsd_ip, *gstr, *hourg, *eint, *forc, *mom;
   int nel, i;
-  __device__ __forceinline__ void forc_load(double (&f)[5], double (&m)[3]) const {
+  __device__ __forceinline__ void late_load(Q4Late<double> &L) const {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) f[k] = forc[(int64_t)k * nel + i];
+    for (int k = 0; k < 5; ++k) L.forc[k] = forc[(int64_t)k * nel + i];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) m[k] = mom[(int64_t)k * nel + i];
+    for (int k = 0; k < 3; ++k) L.mom[k] = mom[(int64_t)k * nel + i];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) L.gstr[k] = gstr[(int64_t)k * nel + i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L.hourg[k] = hourg[(int64_t)k * nel + i];
+    L.eint[0] = eint[i];
+    L.eint[1] = eint[nel + i];
   }
-  __device__ __forceinline__ void forc_store(const double (&f)[5], const double (&m)[3]) const {
+  __device__ __forceinline__ void late_store(const Q4Late<double> &L) const {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) forc[(int64_t)k * nel + i] = f[k];
+    for (int k = 0; k < 5; ++k) forc[(int64_t)k * nel + i] = L.forc[k];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) mom[(int64_t)k * nel + i] = m[k];
-  }
-  __device__ __forceinline__ void gstr_add(const double (&inc)[8]) const {
+    for (int k = 0; k < 3; ++k) mom[(int64_t)k * nel + i] = L.mom[k];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) gstr[(int64_t)k * nel + i] = gstr[(int64_t)k * nel + i] + inc[k];
-  }
-  __device__ __forceinline__ void hourg_load(double (&h)[5]) const {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) h[k] = hourg[(int64_t)k * nel + i];
+    for (int k = 0; k < 8; ++k) gstr[(int64_t)k * nel + i] = L.gstr[k];
+    eint[i] = L.eint[0];
+    eint[nel + i] = L.eint[1];
   }
   __device__ __forceinline__ void hourg_store(const double (&h)[5]) const {
 #pragma unroll
     for (int k = 0; k < 5; ++k) hourg[(int64_t)k * nel + i] = h[k];
-  }
-  __device__ __forceinline__ void eint_add(double membrane, double bending) const {
-    eint[i] = eint[i] + membrane;
-    eint[nel + i] = eint[nel + i] + bending;
   }
   __device__ __forceinline__ void load(int ip, double (&s)[5], double &p, double &e) const {
 #pragma unroll

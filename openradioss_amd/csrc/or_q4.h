@@ -29,12 +29,18 @@ struct Q4Consts {
   double z0[10], wf[10], wm[10];
 };
 
-// Per-element shell state held in registers for the whole element.  GBUF%FOR, MOM,
-// STRA, HOURG and EINT are read and written through the element's accessor at the
-// point of use instead (forc_load / forc_store, gstr_add, hourg_load / hourg_store,
-// eint_add), so that they are not live across the through-thickness loop.
+// Per-element shell state held in registers for the whole element.
 template <typename R> struct Q4Elem {
   R offg, thk, thke, epsd;
+};
+
+// GBUF%FOR, MOM, STRA, HOURG and EINT: read together after the through-thickness loop
+// through the element's accessor (late_load, one memory round trip, nothing live across
+// the loop) and written back as soon as each is final (late_store for FOR, MOM, STRA and
+// EINT, hourg_store for HOURG).  late_load fills hourg[0..2] (HOURG 4 and 5 are set,
+// not updated).
+template <typename R> struct Q4Late {
+  R forc[5], mom[3], gstr[8], hourg[5], eint[2];
 };
 
 template <typename R> struct Q4Out {
@@ -206,7 +212,7 @@ __device__ __forceinline__ void q4_frame(const double (&xg)[4], const double (&y
 // vl/vrl: gathered velocities and rotational velocities; tri: node 3 == node 4.
 // sig/pla/epsd_ip: per-point state, loaded and stored by the caller through the
 // accessor ip_io (so that the kernel chooses the memory layout), which also reads
-// and writes FOR, MOM, STRA, HOURG and EINT.  nload(k, x, v) gives node k's coordinates and
+// and writes FOR, MOM, STRA, HOURG and EINT (Q4Late).  nload(k, x, v) gives node k's coordinates and
 // velocity again after the through-thickness loop (the values of xg/yg/zg and vl), from
 // which the frame and the local velocities that CHVIS3 and CFINT3 use are formed again
 // rather than kept live across the loop.
@@ -327,8 +333,7 @@ __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (
     R FAC1 = DT1 / AREA;
     EXX = EXX * FAC1; EYY = EYY * FAC1; EXY = EXY * FAC1; EYZ = EYZ * FAC1; EXZ = EXZ * FAC1;
     KXX = KXX * FAC1; KYY = KYY * FAC1; KXY = KXY * FAC1;
-    const R inc[8] = {EXX, EYY, EXY, EYZ, EXZ, KXX, KYY, KXY};
-    ip_io.gstr_add(inc); // GSTR(k) = GSTR(k) + increment
+    // GSTR = GSTR + increments, applied with the state read after the loop
   }
   // ---- element strain rate of CFORC3 (cforc3.F:537-546) ----
   const R dtinv = DT1 / dmaxT(DT1 * DT1, EM20);
@@ -392,7 +397,10 @@ __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (
   }
   if ((off == FOUR_OVER_5 && !ioff_duct) || (off > ZERO && off_old < EM01)) off = ZERO;
   // the frame of CNVEC3 / CDERI3 again (same inputs and operations, so the same values)
-  // and the node velocities of CDEFO3 again, zeroed as SCOOR3 / CCOOR3 zero them
+  // and the node velocities of CDEFO3 again, zeroed as SCOOR3 / CCOOR3 zero them; the
+  // element state of Q4Late is read in the same batch of loads
+  Q4Late<R> lt;
+  ip_io.late_load(lt);
   R vn[4][3];
   {
     double xa[4], ya[4], za[4];
@@ -417,20 +425,26 @@ __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (
   for (int k = 0; k < 5; ++k) FOR[k] = FOR[k] * off * ONE;
 #pragma unroll
   for (int k = 0; k < 3; ++k) MOM[k] = MOM[k] * off * ONE;
-  R degmb, degfx; // with the resultants of the previous cycle, then of this one
   {
-    R fo[5], mo[3];
-    ip_io.forc_load(fo, mo);
-    degmb = fo[0] * EXX + fo[1] * EYY + fo[2] * EXY + fo[3] * EYZ + fo[4] * EXZ;
-    degfx = mo[0] * KXX + mo[1] * KYY + mo[2] * KXY;
+    const R inc[8] = {EXX, EYY, EXY, EYZ, EXZ, KXX, KYY, KXY};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) lt.gstr[k] = lt.gstr[k] + inc[k];
   }
+  // with the resultants of the previous cycle, then of this one
+  R degmb = lt.forc[0] * EXX + lt.forc[1] * EYY + lt.forc[2] * EXY + lt.forc[3] * EYZ + lt.forc[4] * EXZ;
+  R degfx = lt.mom[0] * KXX + lt.mom[1] * KYY + lt.mom[2] * KXY;
   degmb = degmb + FOR[0] * EXX + FOR[1] * EYY + FOR[2] * EXY + FOR[3] * EYZ + FOR[4] * EXZ;
   degfx = degfx + MOM[0] * KXX + MOM[1] * KYY + MOM[2] * KXY;
   {
     R vol2 = HALF * VOL0;
-    ip_io.eint_add(degmb * vol2, degfx * thk0 * vol2); // EINT(k) = EINT(k) + term
+    lt.eint[0] = lt.eint[0] + degmb * vol2;
+    lt.eint[1] = lt.eint[1] + degfx * thk0 * vol2;
   }
-  ip_io.forc_store(FOR, MOM);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) lt.forc[k] = FOR[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) lt.mom[k] = MOM[k];
+  ip_io.late_store(lt);
   // VISCMX = 1, ALPE = 1 on this branch (DM = 0)
   // ---- CHVIS3, IHBE=1, ISMSTR=2 (chvis3.F:112-321) ----
   R H11, H12, H13, H21, H22, H23, H31, H32, H33, B1k, B2k;
@@ -480,8 +494,7 @@ __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (
     }
     R HG1 = VX[0] * GAMA1 + VX[1] * GAMA2 + VX[2] * GAMA3 + VX[3] * GAMA4;
     R HG2 = VY[0] * GAMA1 + VY[1] * GAMA2 + VY[2] * GAMA3 + VY[3] * GAMA4;
-    R hourg[5];
-    ip_io.hourg_load(hourg); // HOURG(1..3); 4 and 5 are set below
+    R(&hourg)[5] = lt.hourg;
     hourg[0] = hourg[0] + HG1 * HH1;
     hourg[1] = hourg[1] + HG2 * HH1;
     R HOUR1A = hourg[0] + HG1 * (H1L + H1Q * fabsR(HG1));
@@ -496,7 +509,7 @@ __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (
     HG2 = HGRY;
     hourg[3] = HG1 * (H3L + H3Q * fabsR(HG1));
     hourg[4] = HG2 * (H3L + H3Q * fabsR(HG2));
-    ip_io.hourg_store(hourg);
+    ip_io.hourg_store(lt.hourg);
     B1k = hourg[3] * off;
     B2k = hourg[4] * off;
   }
