@@ -65,16 +65,25 @@ def gpu_run(model, precision: str, steps: int, warmup: int):
                 finite=bool(np.isfinite(d).all()), variants=variants)
 
 
+def kernel_name(family: str, real: str, variant: int) -> str:
+    """Name of an element-kernel launch variant as the profiles list it (workgroup size 256 for
+    even variants, 64 for odd ones; register budget for 2 or 3 waves per SIMD)."""
+    return f"k_{family}_forces<{real}, {real}, {64 if variant & 1 else 256}, {2 + (variant >> 1)}>"
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` on the default workload, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (scaled with the copy-kernel calibration), or None."""
     path = os.path.join(ROOT, "profiles", "block160_plate1000_hbm_traffic.md")
+    col = None
     try:
         for line in open(path):
             cells = [c.strip() for c in line.split("|")]
-            if len(cells) > 4 and cells[1] == f"`{kernel}`":
-                return float(cells[4]) * 1e9
-    except OSError:
+            if "scaled traffic GB" in cells:
+                col = cells.index("scaled traffic GB")
+            elif col is not None and len(cells) > col and cells[1] == f"`{kernel}`":
+                return float(cells[col]) * 1e9
+    except (OSError, ValueError):
         return None
     return None
 
@@ -132,9 +141,10 @@ def sweep_leg(precision: str, cycles: int, warmup: int, n: int = 256):
                 sim_time_range_s=[min(tts), max(tts)])
 
 
-def sweep_full(precision: str, n: int = 256, t_end: float = 5.0e-3, step: int = 500, max_wall: float = 900.0):
+def sweep_full(precision: str, n: int = 256, t_end: float = 5.0e-3, step: int = 500, max_wall: float = 150.0):
     """The whole tube sweep to its end time: every tube runs its own cycles to t_end in one batch;
-    stopped tubes leave the launches.  Reports wall time and element-updates/s."""
+    stopped tubes leave the launches.  Reports wall time and element-updates/s, and the tubes
+    still running after max_wall seconds (their time, cycle count and time step)."""
     from openradioss_amd import engine as E
     from openradioss_amd import model as M
 
@@ -160,13 +170,16 @@ def sweep_full(precision: str, n: int = 256, t_end: float = 5.0e-3, step: int = 
         if now - t0 > max_wall:
             break
     wall = time.perf_counter() - t0
-    cyc = [g.time_state().ncycle for g in gms]
-    tts = [g.time_state().tt for g in gms]
+    states = [g.time_state() for g in gms]
+    cyc = [s.ncycle for s in states]
+    tts = [s.tt for s in states]
+    late = [dict(tube=i, h=p["h"], tt=s.tt, ncycle=s.ncycle, dt2=s.dt2)
+            for i, (s, (_, p)) in enumerate(zip(states, sw)) if s.tt < t_end and s.mstop == 0]
     eu = sum(m.ne_q4 * c for (m, _), c in zip(sw, cyc))
     b.close()
     for g in gms:
         g.close()
-    return dict(models=n, precision=precision, t_end=t_end, wall_s=wall, finished=active == 0,
+    return dict(models=n, precision=precision, t_end=t_end, wall_s=wall, finished=active == 0, running=late,
                 element_updates=eu, element_updates_per_s=eu / wall, cycles_min=min(cyc), cycles_max=max(cyc),
                 sim_time_min=min(tts), variants=variants)
 
@@ -302,9 +315,9 @@ def main():
                    "hex_elements": model.ne_hex, "q4_elements": model.ne_q4, "nodes": model.nn},
         "roofline": {"bound": "hbm", "achieved": hex_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": hex_gbs / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("k_hex_forces<double, double>")
+                     "traffic": measured_traffic(kernel_name("hex", "double", r64["variants"][0]))
                      if (args.n, args.plate) == (160, 1000) else None,
-                     "kernel": "k_hex_forces<double,double>", "kernel_ms": r64["hex_ms"],
+                     "kernel": kernel_name("hex", "double", r64["variants"][0]), "kernel_ms": r64["hex_ms"],
                      "bytes_per_launch": tb["hex"]["as_implemented"],
                      "compulsory_bytes_per_launch": tb["hex"]["compulsory"]},
         "kernels_fp64": {

@@ -119,6 +119,11 @@ int or_model_set_q4_params(or_model *m, const or_q4_params *p);
  * vimp: V(3,nn) imposed velocities, required when any imposed-velocity bit is set. */
 int or_model_upload_nodes(or_model *m, const double *x, const double *v, const double *vr, const double *ms,
                           const double *in, const int32_t *code, const double *vimp);
+/* d: D(3,nn), the displacements of a restart (or_model_upload_nodes sets D to zero).  With
+ * the nodes, the element state and or_model_set_time, it restores a model from the arrays
+ * the or_model_download_* calls return, as the Engine's restart file carries X, D, V and VR
+ * (wrrestp.F:1188-1194) with the element buffers. */
+int or_model_upload_displacement(or_model *m, const double *d);
 /* conn: (ne,8) 0-based node ids; state arrays: sig SIG(NEL,6) component-major over all hexes,
  * eint, rho, vol0, pla, epsd, offg, qvis (ne each); NULL state arrays start at zero (offg at one). */
 int or_model_upload_hex(or_model *m, const int32_t *conn, const double *sig, const double *eint,

@@ -1222,6 +1222,16 @@ int or_model_upload_nodes(or_model *m, const double *x, const double *v, const d
   return OR_OK;
 }
 
+int or_model_upload_displacement(or_model *m, const double *d) {
+  if (!m || !d) return fail(OR_ERR_ARG, "null argument");
+  if (!m->have_nodes) return fail(OR_ERR_STATE, "upload the nodes before their displacements");
+  OR_HIP(hipSetDevice(m->device));
+  OR_HIP(quiesce(m));
+  OR_HIP(hipMemcpyAsync(m->d, d, sizeof(double) * 3 * m->nn, hipMemcpyHostToDevice, m->stream));
+  OR_HIP(quiesce(m));
+  return OR_OK;
+}
+
 template <typename S> static int put(or_model *m, void *dst, const double *src, int64_t n, double dflt) {
   std::vector<S> t((size_t)n);
   for (int64_t i = 0; i < n; ++i) t[i] = (S)(src ? src[i] : dflt);

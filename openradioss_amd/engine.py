@@ -49,7 +49,8 @@ class GpuModel:
     """Resident hex / Q4 model on one GPU (precision 'fp64' or 'split')."""
 
     def __init__(self, model: Model, precision: str = "fp64", device: int = 0, hex_state: dict | None = None,
-                 q4_state: dict | None = None):
+                 q4_state: dict | None = None, nodes: tuple | None = None):
+        """nodes: (x, v, vr, d) to start from instead of the model's x, v, vr and zero d (a restart)."""
         lib = L.load()
         self.model = model
         self.prec = PRECISIONS[precision]
@@ -65,7 +66,8 @@ class GpuModel:
             if model.q4 is not None:
                 self._qp = q4_params(model)
                 L.check(lib.or_model_set_q4_params(h, C.byref(self._qp)))
-            L.check(lib.or_model_upload_nodes(h, _p(_f64(model.x)), _p(_f64(model.v)), _p(_f64(model.vr)),
+            x0, v0, vr0 = (model.x, model.v, model.vr) if nodes is None else nodes[:3]
+            L.check(lib.or_model_upload_nodes(h, _p(_f64(x0)), _p(_f64(v0)), _p(_f64(vr0)),
                                               _p(_f64(model.ms)), _p(_f64(model.inr)),
                                               _p(np.ascontiguousarray(model.code, np.int32)),
                                               _p(_f64(model.vimp))))
@@ -85,6 +87,8 @@ class GpuModel:
             iadc = None if model.q4 is None else np.ascontiguousarray(model.q4.iadc, np.int64)
             L.check(lib.or_model_upload_skyline(h, _p(np.ascontiguousarray(model.adsky, np.int64)), _p(iads),
                                                 _p(iadc)))
+            if nodes is not None:
+                L.check(lib.or_model_upload_displacement(h, _p(_f64(nodes[3]))))
         except Exception:
             self.close()
             raise
@@ -166,6 +170,45 @@ class GpuModel:
         out = np.zeros(3)
         L.check(self._lib.or_model_run_output(self._h, _p(out)))
         return out
+
+    # -- restart ---------------------------------------------------------
+    _TS_FIELDS = ("tt", "dt1", "dt2", "dt12", "dt2old", "ncycle", "mstop")
+
+    def checkpoint(self) -> dict:
+        """Everything a restart needs, as host arrays: X, V, VR, D, the element state and the
+        cycle scalars (the content of the Engine's restart file for this path)."""
+        x, v, vr, d = self.nodes6()
+        ts = self.time_state()
+        out = {"x": x, "v": v, "vr": vr, "d": d,
+               "time": np.array([float(getattr(ts, f)) for f in self._TS_FIELDS])}
+        if self.model.hexp is not None:
+            out.update({"hex_" + k: a for k, a in self.hex_state().items()})
+        if self.model.q4 is not None:
+            out.update({"q4_" + k: a for k, a in self.q4_state().items()})
+        return out
+
+    @classmethod
+    def restore(cls, model: Model, ck: dict, precision: str = "fp64", device: int = 0) -> "GpuModel":
+        """A resident model that continues, bit for bit, from a checkpoint() of the same mesh."""
+        hs = {k[4:]: a for k, a in ck.items() if k.startswith("hex_")} or None
+        qs = {k[3:]: a for k, a in ck.items() if k.startswith("q4_")} or None
+        g = cls(model, precision=precision, device=device, hex_state=hs, q4_state=qs,
+                nodes=(ck["x"], ck["v"], ck["vr"], ck["d"]))
+        ts = L.OrTimeState()
+        for f, val in zip(cls._TS_FIELDS, ck["time"]):
+            setattr(ts, f, int(val) if f in ("ncycle", "mstop") else float(val))
+        g.set_time_state(ts)
+        return g
+
+    def save(self, path: str):
+        """Write checkpoint() to an .npz file."""
+        np.savez(path, **self.checkpoint())
+
+    @classmethod
+    def load(cls, model: Model, path: str, precision: str = "fp64", device: int = 0) -> "GpuModel":
+        with np.load(path, allow_pickle=False) as f:
+            ck = {k: f[k] for k in f.files}
+        return cls.restore(model, ck, precision=precision, device=device)
 
     def set_variant(self, family: str, variant: int):
         """Launch variant (0..3) of the 'hex' or 'q4' element kernel; results do not depend on it."""

@@ -15,6 +15,7 @@ module or_gpu_mod
   public :: or_version, or_device_count, or_device_memory, or_error_message
   public :: or_model_create, or_model_destroy, or_model_set_hex_params, or_model_set_q4_params
   public :: or_model_upload_nodes, or_model_upload_hex, or_model_upload_q4, or_model_upload_skyline
+  public :: or_model_upload_displacement
   public :: or_model_set_time, or_model_get_time, or_model_run, or_model_run_output, or_model_sync
   public :: or_model_download_nodes, or_model_download_hex, or_model_download_q4
   public :: or_model_energy, or_model_kernel_times, or_model_set_timing, or_model_footprint
@@ -107,6 +108,12 @@ module or_gpu_mod
         bind(c, name="or_model_upload_nodes")
       import :: c_int, c_ptr
       type(c_ptr), value :: handle, x, v, vr, ms, in, code, vimp
+    end function
+    ! D(3,NUMNOD) of a restart (wrrestp.F:1188-1194 writes X, D, V, VR)
+    integer(c_int) function or_model_upload_displacement(handle, d) &
+        bind(c, name="or_model_upload_displacement")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle, d
     end function
 
     integer(c_int) function or_model_upload_hex(handle, conn, sig, eint, rho, vol0, pla, epsd, offg, qvis) &

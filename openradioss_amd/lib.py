@@ -24,7 +24,7 @@ OR_PREC_SPLIT = 1
 # Every symbol declared in include/or_gpu.h.
 EXPORTS = (
     "or_version", "or_last_error", "or_device_count", "or_device_memory", "or_model_create", "or_model_destroy",
-    "or_model_set_hex_params", "or_model_set_q4_params", "or_model_upload_nodes", "or_model_upload_hex",
+    "or_model_set_hex_params", "or_model_set_q4_params", "or_model_upload_nodes", "or_model_upload_displacement", "or_model_upload_hex",
     "or_model_upload_q4", "or_model_upload_skyline", "or_model_set_time", "or_model_get_time", "or_model_run",
     "or_model_sync", "or_model_run_output", "or_model_download_nodes", "or_model_download_hex", "or_model_download_q4",
     "or_model_energy", "or_model_kernel_times", "or_model_set_timing", "or_model_footprint", "or_model_footprint_estimate", "or_sforc3_group",
@@ -71,6 +71,7 @@ _SIGS = {
     "or_model_set_hex_params": ([_P, _P], C.c_int),
     "or_model_set_q4_params": ([_P, _P], C.c_int),
     "or_model_upload_nodes": ([_P] * 8, C.c_int),
+    "or_model_upload_displacement": ([_P, _P], C.c_int),
     "or_model_upload_hex": ([_P] * 10, C.c_int),
     "or_model_upload_q4": ([_P] * 14, C.c_int),
     "or_model_upload_skyline": ([_P] * 4, C.c_int),
