@@ -109,6 +109,35 @@ def cpu_baseline(model, budget_s: float = 12.0):
                        f"128) on the same {ne}-element model, {n} cycles after 1 warm-up cycle, {el:.1f} s")
 
 
+def cpu_reference(model, dt1: float, budget_s: float = 6.0):
+    """The reference's own element routines on the CPU, for scale beside the port: SFORC3 and
+    CFORC3 chains compiled from the reference sources (oracle/_ref, one thread, groups of 128
+    as FORINT / FORINTC call them), on the first groups of the bench model from its initial state
+    with time step dt1; element phase only.  None when the reference build is absent."""
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refchain as R
+    if not R.available():
+        return None
+    r = R.RefModel(model)
+    ne_h, ne_q = min(model.ne_hex, 64 * 128), min(model.ne_q4, 32 * 128)
+    n = 0
+    el = 0.0
+    while el < budget_s and n < 1000:
+        dt = C.c_double(1.0e6)
+        t0 = time.perf_counter()
+        if ne_h:
+            r._hex_forces(dt1, dt, ne_h)
+        if ne_q:
+            r._q4_forces(dt1, dt, ne_q)
+        el += time.perf_counter() - t0
+        n += 1
+    return dict(value=(ne_h + ne_q) * n / el, unit="element-updates/s", cores=1, kind="reference",
+                sample=f"element phase of the first {ne_h} hexes and {ne_q} shells of the bench model through the "
+                       f"reference's SFORC3 / CFORC3 routines (amdflang -O2 build of the reference sources, groups "
+                       f"of 128, per-group array staging from Python included), {n} passes, {el:.1f} s")
+
+
 def sweep_leg(precision: str, cycles: int, warmup: int, n: int = 256):
     """The parametric tube sweep (n tubes, element size 1-4 mm, 4.5k-72k shells each) advanced
     together in one batch; a bounded sample of `cycles` batch cycles after `warmup`."""
@@ -365,6 +394,10 @@ def main():
             f"{r['element_updates_per_s']:.3e} element-updates/s")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(model)
+        ref = cpu_reference(model, r64["ts"].dt2)
+        if ref is not None:
+            out["cpu_reference"] = ref
+            log(f"cpu reference routines: {ref['value']:.3e} element-updates/s on one thread")
         log(f"cpu: {out['cpu_baseline']['value']:.3e} element-updates/s")
     print(json.dumps(out))
 

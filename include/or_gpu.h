@@ -89,7 +89,12 @@ typedef struct {
   double dtmin;                                 /* DTMIN1(3) */
 } or_q4_params;
 
-/* Cycle scalars (resol.F: TT, DT1, DT2, DT12, DT2OLD, NCYCLE, MSTOP). */
+/* Cycle scalars (resol.F: TT, DT1, DT2, DT12, DT2OLD, NCYCLE, MSTOP).  MSTOP bits:
+ * OR_MSTOP_JACOBIAN, an element reported a non-positive Jacobian (schkjabt3.F);
+ * OR_MSTOP_TIME_STEP, the cycle's time step came out <= 0 (resol.F:8575, 9599).  A model
+ * with MSTOP set stops cycling. */
+#define OR_MSTOP_JACOBIAN 1
+#define OR_MSTOP_TIME_STEP 2
 typedef struct {
   double tt, dt1, dt2, dt12, dt2old;
   int64_t ncycle;

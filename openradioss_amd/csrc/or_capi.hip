@@ -55,6 +55,9 @@ constexpr int kVariants = 4;
 constexpr int variant_block(int v) { return (v & 1) ? 64 : 256; }
 template <typename R> constexpr int variant_waves(int v) { return 2 + (v >> 1); }
 
+// MSTOP codes (include/or_gpu.h): an element's non-positive Jacobian, a time step <= 0
+constexpr int32_t kMstopJacobian = 1, kMstopTimeStep = 2;
+
 // node code bits (include/or_gpu.h)
 constexpr int32_t kTX = 4, kTY = 2, kTZ = 1, kRX = 32, kRY = 16, kRZ = 8, kIX = 256, kIY = 128, kIZ = 64;
 
@@ -282,7 +285,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
     mstop = o.mstop;
   }
   if (stopped) return;
-  if (mstop) atomicOr(M.flags, 1);
+  if (mstop) atomicOr(M.flags, kMstopJacobian);
   const double m = block_min<TB>(dtx);
   if (threadIdx.x == 0) atomicMin(M.dtmin + parity, dbits(m));
 }
@@ -491,7 +494,8 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
     nx.dt12 = DT12;
     nx.dt2old = DT2;
     nx.ncycle = prev.ncycle + 1;
-    nx.mstop = prev.mstop | *M.flags;
+    // a time step <= 0 stops the run (resol.F:8575, 9599)
+    nx.mstop = prev.mstop | *M.flags | (DT2 <= 0.0 ? kMstopTimeStep : 0);
     nx.done = (nx.mstop != 0 || nx.tt >= M.tend) ? 1 : 0;
     ts[parity ^ 1] = nx;
     dtmin[parity ^ 1] = dbits(kEP06);
@@ -2292,5 +2296,6 @@ int or_nodal_update(int device, int64_t numnod, int64_t lsky, const int64_t *ads
   ts->dt2old = DT2;
   ts->tt = ts->tt + DT2;
   ts->ncycle += 1;
+  if (DT2 <= 0.0) ts->mstop |= kMstopTimeStep; // resol.F:8575, 9599
   return OR_OK;
 }

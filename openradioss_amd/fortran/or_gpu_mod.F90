@@ -27,6 +27,8 @@ module or_gpu_mod
   integer(c_int), parameter, public :: OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_HIP = 2, OR_ERR_NODEV = 3, &
                                        OR_ERR_STATE = 4, OR_ERR_MSTOP = 5
   integer(c_int), parameter, public :: OR_PREC_FP64 = 0, OR_PREC_SPLIT = 1
+  ! MSTOP bits of or_time_state_t: non-positive Jacobian, time step <= 0
+  integer(c_int32_t), parameter, public :: OR_MSTOP_JACOBIAN = 1, OR_MSTOP_TIME_STEP = 2
   integer(c_int32_t), parameter, public :: OR_BC_TX = 4, OR_BC_TY = 2, OR_BC_TZ = 1, OR_BC_RX = 32, &
                                            OR_BC_RY = 16, OR_BC_RZ = 8, OR_IMP_X = 256, OR_IMP_Y = 128, &
                                            OR_IMP_Z = 64

@@ -19,6 +19,8 @@ OR_ERR_NODEV = 3
 OR_ERR_STATE = 4
 OR_ERR_MSTOP = 5
 OR_PREC_FP64 = 0
+OR_MSTOP_JACOBIAN = 1
+OR_MSTOP_TIME_STEP = 2
 OR_PREC_SPLIT = 1
 
 # Every symbol declared in include/or_gpu.h.

@@ -548,6 +548,7 @@ void or_cycle_hex(int64_t nn, int64_t ne, int64_t lsky, const int32_t *conn, dou
   }
   ts->dt2old = ts->dt2;
   ts->dt12 = HALF * (ts->dt1 + ts->dt2);
+  if (ts->dt2 <= ZERO) ts->mstop |= 2; /* time step <= 0 stops the run (resol.F:8575, 9599) */
   /* ACCELE (accele.F:65-80) */
   for (int64_t n = 0; n < nn; ++n) {
     if (ms[n] > ZERO) {

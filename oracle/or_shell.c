@@ -723,6 +723,7 @@ void or_nodes_model(or_model_c *m, or_time_state *ts, double dt2t, int mstop) {
   }
   ts->dt2old = ts->dt2;
   ts->dt12 = HALF * (ts->dt1 + ts->dt2);
+  if (ts->dt2 <= ZERO) ts->mstop |= 2; /* time step <= 0 stops the run (resol.F:8575, 9599) */
   const double DT12 = ts->dt12, DT2 = ts->dt2;
   /* ECRIT output point (sortie_main.F:368 from resol.F:8523, after ACCELE/BCS10/FIXVEL and
    * before VELOCITY): DT05 = HALF*DT1 (ecrit.F:193) */

@@ -116,6 +116,8 @@ class RefHex:
             self.dt2 = min(self.dt2, float(np.float32(1.1)) * self.dt2old, m.dtmx)
             self.dt2old = self.dt2
             self.dt12 = 0.5 * (self.dt1 + self.dt2)
+            if self.dt2 <= 0.0:
+                self.mstop |= 2  # resol.F:8575, 9599
             L = lib()
             ms = np.ascontiguousarray(m.ms)
             L.ref_accele(C.c_int(m.nn), _p(a), _p(self.v), _p(ms))
@@ -182,9 +184,10 @@ class RefModel:
         self.ncycle = 0
         self.mstop = 0
 
-    def _hex_forces(self, dt1, dt2t):
+    def _hex_forces(self, dt1, dt2t, ne=None):
+        """SFORC3 over the groups of the first ne hexes (all by default)."""
         L = lib()
-        ne = self.m.ne_hex
+        ne = self.m.ne_hex if ne is None else ne
         for e0 in range(0, ne, self.GROUP):
             e1 = min(ne, e0 + self.GROUP)
             sig = np.ascontiguousarray(self.hsig[:, e0:e1])
@@ -201,9 +204,10 @@ class RefModel:
                 dst[e0:e1] = src
             self.mstop |= ms.value
 
-    def _q4_forces(self, dt1, dt2t):
+    def _q4_forces(self, dt1, dt2t, ne=None):
+        """CFORC3 over the groups of the first ne shells (all by default)."""
         L = lib()
-        ne = self.m.ne_q4
+        ne = self.m.ne_q4 if ne is None else ne
         q = self.q
         for e0 in range(0, ne, self.GROUP):
             e1 = min(ne, e0 + self.GROUP)
@@ -247,6 +251,8 @@ class RefModel:
             self.dt2 = min(self.dt2, float(np.float32(1.1)) * self.dt2old, m.dtmx)
             self.dt2old = self.dt2
             self.dt12 = 0.5 * (self.dt1 + self.dt2)
+            if self.dt2 <= 0.0:
+                self.mstop |= 2  # resol.F:8575, 9599
             ms = np.ascontiguousarray(m.ms)
             inr = np.ascontiguousarray(m.inr)
             L.ref_accele6(C.c_int(m.nn), _p(a), _p(ar), _p(self.v), _p(ms), _p(inr))

@@ -31,3 +31,18 @@ def test_nodal_dropin_matches_oracle(name, gpu_lib):
     assert np.array_equal(x, o.x) and np.array_equal(v, o.v) and np.array_equal(d, o.d)
     assert np.array_equal(vr, o.vr)
     assert np.array_equal(stifn, o.stifn)
+
+
+def test_nodal_dropin_zero_time_step_stops(gpu_lib):
+    """An element minimum of 0 gives DT2 = 0: the drop-in sets MSTOP's time-step bit like the oracle."""
+    print("case: or_nodal_update with dt2t = 0 on mixed_c120 after 5 cycles", flush=True)
+    m = build_q4_case(Q4_CASES["mixed_c120"])
+    o = O.ModelOracle(m, nthreads=2, math="cr")
+    o.cycle(5)
+    ts = L.OrTimeState()
+    ts.tt, ts.dt1, ts.dt2, ts.dt12, ts.dt2old, ts.ncycle = (o.ts.tt, o.ts.dt1, o.ts.dt2, o.ts.dt12, o.ts.dt2old,
+                                                            o.ts.ncycle)
+    x, v, vr, d = o.x.copy(), o.v.copy(), o.vr.copy(), o.d.copy()
+    E.nodal_update(np.asarray(m.adsky) + 1, o.fsky, m.ms, x, v, d, ts, dt2t=0.0, dtmx=m.dtmx, inr=m.inr,
+                   vr=vr, code=m.code, vimp=m.vimp)
+    assert ts.dt2 == 0.0 and ts.mstop == L.OR_MSTOP_TIME_STEP

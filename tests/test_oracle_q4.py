@@ -249,3 +249,14 @@ def test_ecrit_energy_balance_elastic_free_vibration():
     ecrit, half_step = np.array(ecrit), np.array(half_step)
     assert np.ptp(ecrit) <= 1.5e-2 * ecrit[0]
     assert np.ptp(ecrit) < 0.5 * np.ptp(half_step)
+
+
+def test_zero_time_step_stops_the_run():
+    """A cycle whose time step comes out <= 0 sets MSTOP (resol.F:8575, 9599 stop the run there)."""
+    import ctypes as C
+    m = build_q4_case(Q4_CASES["mixed_c120"])
+    o = O.ModelOracle(m)
+    o.cycle(3)
+    assert o.ts.mstop == 0
+    O.lib().or_nodes_model(C.byref(o.c), C.byref(o.ts), C.c_double(0.0), 0)
+    assert o.ts.dt2 == 0.0 and o.ts.mstop == 2
