@@ -204,11 +204,14 @@ def sweep_full(precision: str, n: int = 256, t_end: float = 5.0e-3, step: int = 
     tts = [s.tt for s in states]
     late = [dict(tube=i, h=p["h"], tt=s.tt, ncycle=s.ncycle, dt2=s.dt2)
             for i, (s, (_, p)) in enumerate(zip(states, sw)) if s.tt < t_end and s.mstop == 0]
+    stopped = [dict(tube=i, h=p["h"], tt=s.tt, ncycle=s.ncycle, dt2=s.dt2, mstop=s.mstop)
+               for i, (s, (_, p)) in enumerate(zip(states, sw)) if s.mstop != 0]
     eu = sum(m.ne_q4 * c for (m, _), c in zip(sw, cyc))
     b.close()
     for g in gms:
         g.close()
     return dict(models=n, precision=precision, t_end=t_end, wall_s=wall, finished=active == 0, running=late,
+                stopped=stopped,
                 element_updates=eu, element_updates_per_s=eu / wall, cycles_min=min(cyc), cycles_max=max(cyc),
                 sim_time_min=min(tts), variants=variants)
 
