@@ -4,7 +4,9 @@
   to the oracle under correctly rounded pow/log/exp;
 * square tube 60x300 (72,000 Q4 shells, 5 points), axial crush at 15 m/s: FP64 GPU bit for bit
   equal to the oracle over its first 2,000 cycles; the split form against the FP64 form up to
-  about 20,000 cycles at matched simulated times, beside the FP64 method's own sensitivity.
+  about 20,000 cycles at matched simulated times, beside the FP64 method's own sensitivity;
+* hex block 160^3 with the 1000x1000 Q4 plate (4,096,000 hexes + 1,000,000 shells, 5,149,361
+  nodes, the bench workload): FP64 GPU bit for bit equal to the oracle over its first cycles.
 """
 import numpy as np
 import pytest
@@ -93,3 +95,22 @@ def test_tube_60x300_split_vs_fp64_to_20000_cycles(gpu_lib):
     b.close()
     for g in runs.values():
         g.close()
+
+
+def test_block_plate_full_size_first_cycles_bitwise(gpu_lib):
+    print("case: block 160^3 + plate 1000^2, 5,096,000 elements, 4 cycles, fp64 vs oracle(cr)", flush=True)
+    m = M.block_with_plate(n=160, plate=1000, seed=20261203)
+    assert (m.ne_hex, m.ne_q4, m.nn) == (4_096_000, 1_000_000, 5_149_361)
+    with E.GpuModel(m, precision="fp64") as g:
+        g.run(4)
+        x, v, vr, d = g.nodes6()
+        ts = g.time_state()
+        hs, qs = g.hex_state(), g.q4_state()
+    o = O.ModelOracle(m, nthreads=THREADS, math="cr")
+    o.cycle(4)
+    assert (ts.tt, ts.dt2, ts.ncycle) == (o.ts.tt, o.ts.dt2, o.ts.ncycle)
+    for name, a, b in (("x", x, o.x), ("v", v, o.v), ("vr", vr, o.vr), ("d", d, o.d),
+                       ("hex sig", hs["sig"], o.hsig), ("q4 sig", qs["sig"], o.q["sig"]),
+                       ("q4 hourg", qs["hourg"], o.q["hourg"])):
+        assert np.array_equal(a, b), name
+    print(f"t = {o.ts.tt:.6e} s after 4 cycles, max |d| = {np.abs(d).max():.4e} mm")
