@@ -330,7 +330,7 @@ def main():
     hex_gbs = tb["hex"]["as_implemented"] / (r64["hex_ms"] * 1e-3) / 1e9
     cyc_ms = r64["hex_ms"] + r64["q4_ms"] + r64["node_ms"]
     out = {
-        "metric": "element-updates/s (explicit hex + Q4 cycle, plain FP64 method)",
+        "metric": "element-updates/s & %HBM peak vs FP64 plain; p50/p99 latency per served solve",
         "value": eu64,
         "unit": "element-updates/s",
         "n_gpus": 1,
