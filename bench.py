@@ -138,6 +138,25 @@ def cpu_reference(model, dt1: float, budget_s: float = 6.0):
                        f"of 128, per-group array staging from Python included), {n} passes, {el:.1f} s")
 
 
+def small_leg(cycles: int = 1000) -> dict:
+    """The small reference workloads (bar 20x20x80, 32,000 hexes; tube 60x300, 72,000 shells), FP64,
+    one run(cycles) call after 20 warm-up cycles: at these sizes a cycle is bound by the three
+    dependent kernel launches, not by the kernels' work."""
+    from openradioss_amd import engine as E
+    from openradioss_amd import model as M
+    out = {}
+    for name, m in (("bar_20x20x80", M.hex_model_to_model(M.cantilever_bar(20, 20, 80))),
+                    ("tube_60x300", M.square_tube())):
+        ne = m.ne_hex + m.ne_q4
+        with E.GpuModel(m, precision="fp64") as g:
+            g.run(20)
+            t0 = time.perf_counter()
+            g.run(cycles)
+            t = time.perf_counter() - t0
+        out[name] = dict(elements=ne, cycles=cycles, us_per_cycle=1e6 * t / cycles, element_updates_per_s=ne * cycles / t)
+    return out
+
+
 def sweep_leg(precision: str, cycles: int, warmup: int, n: int = 256):
     """The parametric tube sweep (n tubes, element size 1-4 mm, 4.5k-72k shells each) advanced
     together in one batch; a bounded sample of `cycles` batch cycles after `warmup`."""
@@ -383,6 +402,9 @@ def main():
             "kernel_variants": {"hex": rs["variants"][0], "q4": rs["variants"][1]},
             "dtype": "f32 element / f64 node"}
         log(f"split: {eus:.3e} element-updates/s (x{eus / eu64:.2f}), disp err {derr:.2e}, energy err {eerr:.2e}")
+    out["small_models"] = small_leg()
+    for k, r in out["small_models"].items():
+        log(f"{k}: {r['element_updates_per_s']:.3e} element-updates/s, {r['us_per_cycle']:.1f} us per cycle")
     if not args.no_sweep:
         out["batch_sweep"] = {}
         for prec in ("fp64", "split"):
