@@ -249,10 +249,17 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
       xd[k] = a; yd[k] = b; zd[k] = cc;
       vx[k] = R(u); vy[k] = R(v); vz[k] = R(w);
     }
+    // GBUF%OFF enters the geometry; the rest of the element state is read after the
+    // kinematics (the fence keeps the compiler from hoisting the loads), so it is not
+    // live through the geometry and the kinematics
+    st.offg = e.offg[i];
+    auto load_rest = [&]() {
+      __asm__ __volatile__("" ::: "memory");
 #pragma unroll
-    for (int k = 0; k < 6; ++k) st.sig[k] = e.sig[(int64_t)k * e.ne + i];
-    st.eint = e.eint[i]; st.rho = e.rho[i]; st.vol0 = e.vol0[i]; st.pla = e.pla[i];
-    st.epsd = e.epsd[i]; st.offg = e.offg[i]; st.qvis = e.qvis[i];
+      for (int k = 0; k < 6; ++k) st.sig[k] = e.sig[(int64_t)k * e.ne + i];
+      st.eint = e.eint[i]; st.rho = e.rho[i]; st.vol0 = e.vol0[i]; st.pla = e.pla[i];
+      st.epsd = e.epsd[i]; st.qvis = e.qvis[i];
+    };
     if constexpr (sizeof(R) == 8) {
       // FP64: each node's force triple goes to its skyline row as soon as it is final,
       // which keeps the chain within two waves' register budget
@@ -261,12 +268,12 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
         const gptr<S> r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
         r[0] = S(fx); r[kBlock] = S(fy); r[2 * kBlock] = S(fz);
       };
-      hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, write_row);
+      hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, write_row, load_rest);
     } else {
       // FP32: the forces are gathered first and stored together (registers are not the limit)
       R f1[8], f2[8], f3[8];
       auto keep = [&](int k, R fx, R fy, R fz) { f1[k] = fx; f2[k] = fy; f3[k] = fz; };
-      hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, keep);
+      hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, keep, load_rest);
       if (!stopped) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -658,7 +665,7 @@ __global__ __launch_bounds__(kBlock) void k_sforc3_group(int nel, int64_t nft, c
     fsky[8 * K + 2] = fz;
     fsky[8 * K + 6] = o.sti4;
   };
-  hex_element<double, double>(xd, yd, zd, vx, vy, vz, st, c, dt1, o, write_row);
+  hex_element<double, double>(xd, yd, zd, vx, vy, vz, st, c, dt1, o, write_row, [] {});
 #pragma unroll
   for (int k = 0; k < 6; ++k) sig[(int64_t)k * nel + i] = st.sig[k];
   eint[i] = st.eint; rho[i] = st.rho; pla[i] = st.pla;

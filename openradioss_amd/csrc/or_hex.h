@@ -574,15 +574,18 @@ __device__ __forceinline__ void hex_material_forces(const HexGeom<R> &g, const H
   }
 }
 
-// The whole chain for one element (all inputs gathered beforehand).
-template <typename R, typename S, typename Write>
+// The whole chain for one element.  st.offg and the gathered nodes are read beforehand;
+// load_rest() reads the rest of st between the kinematics and the material phase.
+template <typename R, typename S, typename Write, typename LoadRest>
 __device__ __forceinline__ void hex_element(const double (&xd)[8], const double (&yd)[8], const double (&zd)[8],
                                             R (&vx)[8], R (&vy)[8], R (&vz)[8], HexState<S> &st,
-                                            const HexConsts &c, const R DT1, HexOut<R> &o, Write &&write) {
+                                            const HexConsts &c, const R DT1, HexOut<R> &o, Write &&write,
+                                            LoadRest &&load_rest) {
   HexGeom<R> g;
   hex_geometry<R>(xd, yd, zd, double(st.offg), g);
   HexKin<R> kn;
   hex_kinematics<R>(vx, vy, vz, g, DT1, kn);
+  load_rest();
   hex_material_forces<R, S>(g, kn, st, c, DT1, o, write);
 }
 
