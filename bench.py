@@ -81,6 +81,8 @@ def measured_traffic(kernel: str):
             cells = [c.strip() for c in line.split("|")]
             if "scaled traffic GB" in cells:
                 col = cells.index("scaled traffic GB")
+            elif len(cells) > 1 and cells[1] == "kernel":
+                col = None  # another table
             elif col is not None and len(cells) > col and cells[1] == f"`{kernel}`":
                 return float(cells[col]) * 1e9
     except (OSError, ValueError):
