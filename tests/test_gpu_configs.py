@@ -3,7 +3,8 @@
 * cantilever bar 20x20x80 (32,000 hexes), law-2 steel, 1,000 cycles: FP64 GPU bit for bit equal
   to the oracle under correctly rounded pow/log/exp;
 * square tube 60x300 (72,000 Q4 shells, 5 points), axial crush at 15 m/s: FP64 GPU bit for bit
-  equal to the oracle over its first 2,000 cycles; the split form against the FP64 form up to
+  equal to the oracle over 12,000 cycles (0.75 ms, past the first fold); the split form against
+  the FP64 form up to
   about 20,000 cycles at matched simulated times, beside the FP64 method's own sensitivity;
 * hex block 160^3 with the 1000x1000 Q4 plate (4,096,000 hexes + 1,000,000 shells, 5,149,361
   nodes, the bench workload): FP64 GPU bit for bit equal to the oracle over its first cycles.
@@ -37,14 +38,17 @@ def test_bar_20x20x80_1000_cycles_bitwise(gpu_lib):
     g.close()
 
 
-def test_tube_60x300_2000_cycles_bitwise(gpu_lib):
-    print("case: tube 60x300, 72,000 shells, 2,000 cycles, fp64 vs oracle(cr)", flush=True)
+def test_tube_60x300_12000_cycles_bitwise(gpu_lib):
+    """The tube crush through its first fold (about 0.65 ms): bit for bit there is what the
+    chaos after the fold cannot hide (a one-ulp difference grows to O(1)).  12,000 cycles keep
+    the oracle's share near a minute; the whole 20,000 were run the same way once."""
+    print("case: tube 60x300, 72,000 shells, 12,000 cycles, fp64 vs oracle(cr)", flush=True)
     m = M.square_tube()
     assert m.ne_q4 == 72_000 and m.nn == 72_240
     g = E.GpuModel(m)
-    g.run(2000)
+    g.run(12000)
     o = O.ModelOracle(m, nthreads=THREADS, math="cr")
-    o.cycle(2000)
+    o.cycle(12000)
     x, v, vr, d = g.nodes6()
     q = g.q4_state()
     assert g.time_state().tt == o.ts.tt
