@@ -9,6 +9,12 @@
 //   k_node_update: ASSPAR4 fixed-order segmented sum per node (hex rows, then Q4 rows,
 //                  the FILLCNE order), DT2 pipeline, ACCELE, BCS10, FIXVEL, VELOCITY,
 //                  DEPLA; thread 0 advances the cycle scalars (double-buffered by parity)
+//
+// The file is compiled twice: once with the compiler's default instruction scheduler
+// (everything), and once with -DOR_ILP_TU and the ILP-first scheduler
+// (-mllvm -amdgpu-sched-strategy=max-ilp), which only provides the element-kernel launches
+// of variants 4..7 (or_ilp_elements).  Scheduling reorders instructions and never changes
+// an operation or its operands, so every variant gives the same bits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,15 +51,16 @@ constexpr int kBlock = 256;
 constexpr double kNoEnd = 1.0e300; // end time of a model advanced by or_model_run
 
 // Launch variants of the element kernels.  Variant v picks the workgroup size
-// (256 elements for even v, 64 for odd v) and the minimum number of waves per SIMD
-// the register allocation is built for (two for v < 2, three for v >= 2, where the
+// (256 elements for even v, 64 for odd v), the minimum number of waves per SIMD
+// the register allocation is built for (two for v % 4 < 2, three otherwise, where the
 // FP64 kernels spill to scratch; a wave's register budget is 512 / waves VGPRs and
-// AGPRs together).
+// AGPRs together), and the instruction scheduler (default for v < 4, ILP-first for
+// v >= 4).
 // Every variant runs the same arithmetic, so the choice (or_model_set_variant,
 // or_model_autotune, or_batch_autotune) changes speed only, never a result bit.
-constexpr int kVariants = 4;
+constexpr int kVariants = 8;
 constexpr int variant_block(int v) { return (v & 1) ? 64 : 256; }
-template <typename R> constexpr int variant_waves(int v) { return 2 + (v >> 1); }
+template <typename R> constexpr int variant_waves(int v) { return 2 + ((v >> 1) & 1); }
 
 // MSTOP codes (include/or_gpu.h): an element's non-positive Jacobian, a time step <= 0
 constexpr int32_t kMstopJacobian = 1, kMstopTimeStep = 2;
@@ -913,6 +920,7 @@ template <typename T> int dalloc(T **p, int64_t n, int64_t &acc) {
 
 } // namespace
 
+#ifndef OR_ILP_TU
 // ============================================================== model object
 struct or_model {
   int device = 0;
@@ -1512,6 +1520,8 @@ static int refresh_dev(or_model *m) {
   return OR_OK;
 }
 
+#endif // OR_ILP_TU
+
 // The three launches of one cycle for a set of models of one precision: element
 // forces (hexes, then shells) and the node update, each over the set's tile list.
 struct LaunchSet {
@@ -1545,16 +1555,40 @@ static void launch_q4(int v, hipStream_t st, const LaunchSet &L, int parity) {
   }
 }
 
+// Element launches of variants 4..7 (variant - 4 in the unit built with the ILP-first
+// scheduler): family 0 hexes, 1 shells; fp64 selects <double,double> or <float,float>.
+void or_ilp_elements(int family, int fp64, int variant, hipStream_t st, const void *md, const void *tiles,
+                     int32_t ntiles, int parity)
+#ifdef OR_ILP_TU
+{
+  LaunchSet L;
+  L.md = (const ModelDev *)md;
+  if (family == 0) {
+    L.th = (const Tile *)tiles; L.nth = ntiles;
+    if (fp64) launch_hex<double, double>(variant, st, L, parity);
+    else launch_hex<float, float>(variant, st, L, parity);
+  } else {
+    L.tq = (const Tile *)tiles; L.ntq = ntiles;
+    if (fp64) launch_q4<double, double>(variant, st, L, parity);
+    else launch_q4<float, float>(variant, st, L, parity);
+  }
+}
+#else
+;
+
 template <typename R, typename S>
 static int launch_cycle(hipStream_t st, const LaunchSet &L, int parity, hipEvent_t after_hex, hipEvent_t after_q4,
                         int output = 0) {
+  constexpr int fp64 = sizeof(R) == 8 ? 1 : 0;
   if (L.nth > 0) {
-    launch_hex<R, S>(L.vh, st, L, parity);
+    if (L.vh >= 4) or_ilp_elements(0, fp64, L.vh - 4, st, L.md, L.th, L.nth, parity);
+    else launch_hex<R, S>(L.vh, st, L, parity);
     OR_HIP(hipGetLastError());
   }
   if (after_hex) OR_HIP(hipEventRecord(after_hex, st));
   if (L.ntq > 0) {
-    launch_q4<R, S>(L.vq, st, L, parity);
+    if (L.vq >= 4) or_ilp_elements(1, fp64, L.vq - 4, st, L.md, L.tq, L.ntq, parity);
+    else launch_q4<R, S>(L.vq, st, L, parity);
     OR_HIP(hipGetLastError());
   }
   if (after_q4) OR_HIP(hipEventRecord(after_q4, st));
@@ -2306,3 +2340,4 @@ int or_nodal_update(int device, int64_t numnod, int64_t lsky, const int64_t *ads
   if (DT2 <= 0.0) ts->mstop |= kMstopTimeStep; // resol.F:8575, 9599
   return OR_OK;
 }
+#endif // OR_ILP_TU
