@@ -44,7 +44,7 @@ def gpu_run(model, precision: str, steps: int, warmup: int):
 
     g = E.GpuModel(model, precision=precision)
     g.run(warmup)
-    variants = g.autotune(3)  # launch variants of the element kernels (results do not depend on them)
+    variants = g.autotune(5)  # launch variants of the element kernels (results do not depend on them)
     g.sync()
     t0 = time.perf_counter()
     g.run(steps, sync=False)
