@@ -1,0 +1,100 @@
+"""Starter decks: the reader against a hand-written deck whose masses, inertias, codes and
+velocities are worked out by hand, a write/read round trip that must give the builders' models
+bit for bit, and the keywords that would take a model off this path."""
+import os
+
+import numpy as np
+import pytest
+
+from openradioss_amd import deck as D
+from openradioss_amd import model as M
+
+FIXTURE = os.path.join(os.path.dirname(__file__), "golden", "deck_brick_shell.rad")
+
+
+def _text():
+    with open(FIXTURE) as f:
+        return f.read()
+
+
+def test_fixture_deck_gives_the_hand_checked_model():
+    m = D.read(FIXTURE)
+    assert (m.nn, m.ne_hex, m.ne_q4) == (12, 2, 2)
+    # nodes in deck order: 101..106 bottom, 201..206 top; x on a unit grid
+    assert np.array_equal(m.x[7], [1.0, 0.0, 1.0]) and np.array_equal(m.x[5], [2.0, 1.0, 0.0])
+    assert m.hexp.conn.tolist() == [[0, 1, 4, 3, 6, 7, 10, 9], [1, 2, 5, 4, 7, 8, 11, 10]]
+    assert m.q4.conn.tolist() == [[6, 7, 10, 9], [7, 8, 11, 10]]
+    rho = 7.8e-9                                  # written "7.8-9" (Fortran exponent without E)
+    brick = rho * 1.0 / 8.0                       # SMASS3: RHO * VOL / 8, unit cubes
+    s11, s12 = rho * 1.5 * 1.0 / 4.0, rho * 2.0 * 1.0 / 4.0   # CINMAS: RHO * THK * AREA / 4
+    want_ms = [brick, 2 * brick, brick, brick, 2 * brick, brick,
+               brick + s11, 2 * brick + s11 + s12, brick + s12, brick + s11, 2 * brick + s11 + s12, brick + s12]
+    assert np.allclose(m.ms, want_ms, rtol=1e-15, atol=0)
+    i11, i12 = s11 * (1.0 / 9.0 + 1.5 ** 2 / 12.0), s12 * (1.0 / 9.0 + 2.0 ** 2 / 12.0)
+    assert np.allclose(m.inr, [0] * 6 + [i11, i11 + i12, i12, i11, i11 + i12, i12], rtol=1e-15, atol=0)
+    assert np.array_equal(m.hexp.vol0, [1.0, 1.0])
+    assert np.array_equal(m.q4.thke, [1.5, 2.0])     # element thickness, else the property's
+    # /BCS 111 000 on the bottom, /IMPVEL X = 2000 * f(t) = 2000 on nodes 203 and 206
+    assert m.code.tolist() == [7] * 6 + [0, 0, M.IMP_X, 0, 0, M.IMP_X]
+    assert np.array_equal(m.vimp[:, 0], [0] * 8 + [2000.0, 0, 0, 2000.0])
+    assert np.array_equal(m.v[:, 2], [0.0] * 6 + [-5000.0] * 6)
+    # blank fields take the Starter's defaults
+    assert (m.hexp.prop.qa, m.hexp.prop.qb, m.hexp.prop.qh) == (1.1, 0.05, 0.1)
+    assert (m.q4.prop.hm, m.q4.prop.hf, m.q4.prop.hr, m.q4.prop.npt) == (0.01, 0.01, 0.02, 5)
+    p = m.hexp.mat.params()
+    assert (p["icc"], p["vp"], p["israte"], p["epmx"]) == (1, 2, 0, M.EP20)
+    # the skyline: solids before shells at a shared node (node 202: two bricks, then two shells)
+    rows = m.adsky[7], m.adsky[8]
+    assert rows[1] - rows[0] == 4
+    got = [m.hexp.iads[0, 5], m.hexp.iads[1, 4], m.q4.iadc[0, 1], m.q4.iadc[1, 0]]
+    assert got == list(range(rows[0], rows[1]))
+
+
+def _same(a: M.Model, b: M.Model):
+    for k in ("x", "v", "vr", "ms", "inr", "code", "vimp", "adsky"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+    for fam, keys in (("hexp", ("conn", "vol0", "iads")), ("q4", ("conn", "thke", "iadc"))):
+        pa, pb = getattr(a, fam), getattr(b, fam)
+        assert (pa is None) == (pb is None), fam
+        if pa is None:
+            continue
+        for k in keys:
+            assert np.array_equal(getattr(pa, k), getattr(pb, k)), (fam, k)
+        assert pa.mat == pb.mat and pa.prop == pb.prop, fam
+        assert pa.params(1e20) == pb.params(1e20) if fam == "hexp" else pa.params() == pb.params()
+
+
+@pytest.mark.parametrize("which", ["bar", "tube", "block_plate"])
+def test_written_deck_reads_back_bit_for_bit(which, tmp_path):
+    if which == "bar":
+        m = M.hex_model_to_model(M.cantilever_bar(4, 4, 10))
+    elif which == "tube":
+        m = M.square_tube(width=8.0, length=12.0)
+    else:
+        m = M.block_with_plate(n=4, plate=6)
+    path = tmp_path / "m_0000.rad"
+    path.write_text(D.write(m, which))
+    _same(D.read(str(path)), m)
+
+
+def _i(v):
+    return str(v).rjust(10)
+
+
+def _r(v):
+    return str(v).rjust(20)
+
+
+@pytest.mark.parametrize("old,new,msg", [
+    (_i(1) + _i(2) + _i(0) * 3, _i(24) + _i(2) + _i(0) * 3, "Ishell = 24"),
+    (_r("1.0e30") + _r("1"), _r("1.0e30") + _r("2"), "varies in time"),
+    ("/BRICK/1\n" + _i(1) + _i(101), "/BRICK/1\n" + _i(1) + _i(999), "node 999"),
+    ("/TITLE", "/RANDOM\n" + _r(1) + "\n/TITLE", "random"),
+    ("/MAT/PLAS_JOHNS/1", "/MAT/ELASTIC/1", "law 2"),
+    ("   111 000" + _i(0), "   111 000" + _i(5), "skew"),
+])
+def test_decks_off_this_path_are_refused(old, new, msg):
+    text = _text()
+    assert text.count(old) == 1, old
+    with pytest.raises(D.DeckError, match=msg):
+        D.to_model(D.parse(text.replace(old, new)))
