@@ -42,7 +42,7 @@ import numpy as np
 
 from . import model as M
 
-__all__ = ["DeckError", "Deck", "parse", "read", "to_model", "write"]
+__all__ = ["DeckError", "Deck", "EngineSettings", "parse", "parse_engine", "read", "to_model", "write"]
 
 
 class DeckError(ValueError):
@@ -425,10 +425,69 @@ def to_model(deck: Deck, dtfac_hex: float = 0.9, dtfac_q4: float = 0.9, dtmx: fl
                          dtfac_hex=dtfac_hex, dtfac_q4=dtfac_q4, dtmx=dtmx)
 
 
-def read(path: str, **kw) -> M.Model:
-    """Read a Starter deck file into a model (`to_model` keywords pass through)."""
+def read(path: str, engine: str | None = None, **kw) -> M.Model:
+    """Read a Starter deck file into a model. With `engine`, the Engine file's /DT/BRICK and
+    /DT/SHELL scale factors are applied; other `to_model` keywords pass through."""
+    if engine is not None:
+        with open(engine, "r", encoding="latin-1") as f:
+            e = parse_engine(f.read())
+        kw = dict(dict(dtfac_hex=e.dtfac_hex, dtfac_q4=e.dtfac_q4), **kw)
     with open(path, "r", encoding="latin-1") as f:
         return to_model(parse(f.read()), **kw)
+
+
+# ------------------------------------------------------------------------------ engine file
+@dataclasses.dataclass
+class EngineSettings:
+    """What an Engine input file (`*_0001.rad`) sets for this path's cycle."""
+
+    tstop: float | None = None            # /RUN end time
+    dtfac_hex: float = 0.9                # DTFAC1(1), /DT/BRICK
+    dtfac_q4: float = 0.9                 # DTFAC1(3), /DT/SHELL
+
+
+# Engine keywords that request output or name the run; they leave the cycle unchanged.
+_ENGINE_IGNORED = {"TITLE", "VERS", "ANIM", "TFILE", "PRINT", "RFILE", "MON", "H3D", "OUTP", "STOP"}
+
+
+def _free(line: str) -> list[str]:
+    """The items of a list-directed (free format) card."""
+    return [t for t in re.split(r"[\s,]+", line.strip()) if t]
+
+
+def parse_engine(text: str) -> EngineSettings:
+    """/RUN and the /DT/BRICK, /DT/SHELL scale factors of an Engine file; the factors are read
+    as freform.F:1076-1115 reads them (list-directed Tsca, Tmin into DTFAC1 / DTMIN1 of the
+    brick and shell types). Keywords that would change the cycle otherwise are refused."""
+    out = EngineSettings()
+    for b in _blocks(text):
+        p = [x.upper() for x in b.parts]
+        where = f"{b.keyword} (line {b.lineno})"
+        if p[0] in _ENGINE_IGNORED:
+            continue
+        if p[0] == "RUN":
+            b.need(1)
+            out.tstop = _real(_free(b.lines[0])[0])
+        elif p[0] == "DT":
+            kind = p[1] if len(p) > 1 else ""
+            extra = [x for x in p[2:] if not x.isdigit()]
+            if kind not in ("BRICK", "SHELL") or extra:
+                raise DeckError(f"{where}: only the /DT/BRICK and /DT/SHELL scale factors apply on this path")
+            b.need(1)
+            vals = _free(b.lines[0])
+            tsca = _real(vals[0]) if vals else 0.0
+            tmin = _real(vals[1]) if len(vals) > 1 else 0.0
+            if tsca <= 0.0:
+                raise DeckError(f"{where}: state a positive time-step scale factor")
+            if tmin != 0.0:
+                raise DeckError(f"{where}: a minimum element time step is not on this path")
+            if kind == "BRICK":
+                out.dtfac_hex = tsca
+            else:
+                out.dtfac_q4 = tsca
+        else:
+            raise DeckError(f"{where}: Engine keyword not applied by this path")
+    return out
 
 
 # ----------------------------------------------------------------------------------- writer

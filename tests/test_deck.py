@@ -98,3 +98,33 @@ def test_decks_off_this_path_are_refused(old, new, msg):
     assert text.count(old) == 1, old
     with pytest.raises(D.DeckError, match=msg):
         D.to_model(D.parse(text.replace(old, new)))
+
+
+ENGINE = """#RADIOSS ENGINE
+/VERS/2025
+/RUN/fixture/1
+   2.0e-4
+/DT/BRICK/0
+ 0.67  0
+/DT/SHELL
+0.5,0
+/ANIM/DT
+0 1e-5
+/END
+"""
+
+
+def test_engine_file_sets_the_time_step_factors(tmp_path):
+    e = D.parse_engine(ENGINE)
+    assert (e.tstop, e.dtfac_hex, e.dtfac_q4) == (2.0e-4, 0.67, 0.5)
+    path = tmp_path / "fixture_0001.rad"
+    path.write_text(ENGINE)
+    m = D.read(FIXTURE, engine=str(path))
+    assert (m.hexp.dtfac, m.q4.dtfac) == (0.67, 0.5)
+
+
+@pytest.mark.parametrize("card,msg", [("/DT/BRICK/CST/0\n0.67 0", "scale factors"),
+                                      ("/DT/SHELL\n0.5 1e-7", "minimum"), ("/DAMP/1\nx", "not applied")])
+def test_engine_keywords_off_this_path_are_refused(card, msg):
+    with pytest.raises(D.DeckError, match=msg):
+        D.parse_engine(ENGINE.replace("/END", card + "\n/END"))
