@@ -64,9 +64,11 @@ def _same(a: M.Model, b: M.Model):
         assert pa.params(1e20) == pb.params(1e20) if fam == "hexp" else pa.params() == pb.params()
 
 
-@pytest.mark.parametrize("which", ["bar", "tube", "block_plate"])
+@pytest.mark.parametrize("which", ["fixture", "bar", "tube", "block_plate"])
 def test_written_deck_reads_back_bit_for_bit(which, tmp_path):
-    if which == "bar":
+    if which == "fixture":
+        m = D.read(FIXTURE)
+    elif which == "bar":
         m = M.hex_model_to_model(M.cantilever_bar(4, 4, 10))
     elif which == "tube":
         m = M.square_tube(width=8.0, length=12.0)
