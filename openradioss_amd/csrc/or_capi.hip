@@ -835,6 +835,9 @@ HexConsts make_hex_consts(const or_hex_params &p) {
 int check_hex_params(const or_hex_params &p) {
   if (p.fisokin != 0.0) return fail(OR_ERR_ARG, "law 2 kinematic hardening (FISOKIN>0) is not on this path");
   if (p.iform != 0) return fail(OR_ERR_ARG, "law 2 Zerilli form (IFORM=1) is not on this path");
+  // VP=1 (plastic strain rate) filters EPSD after the return (m2law.F:536-544) with the Starter's
+  // default cut-off frequency; that update is not on this path
+  if (p.vp != 2 && p.vp != 3) return fail(OR_ERR_ARG, "law 2 hexes: VP must be 2 (total) or 3 (deviatoric strain rate)");
   if (!(p.rho0 > 0.0) || !(p.dtfac > 0.0)) return fail(OR_ERR_ARG, "rho0 and dtfac must be positive");
   return OR_OK;
 }

@@ -42,7 +42,8 @@ import numpy as np
 
 from . import model as M
 
-__all__ = ["DeckError", "Deck", "EngineSettings", "parse", "parse_engine", "read", "to_model", "write"]
+__all__ = ["DeckError", "Deck", "EngineSettings", "parse", "parse_engine", "read", "read_run", "to_model", "write",
+           "write_engine"]
 
 
 class DeckError(ValueError):
@@ -112,7 +113,7 @@ def _blocks(text: str) -> list[_Block]:
     cur = None
     for no, raw in enumerate(text.splitlines(), 1):
         line = raw.rstrip("\r\n")
-        if not line.strip() or line.startswith("#") or line.startswith("$"):
+        if line.startswith("#") or line.startswith("$"):
             continue
         if line.startswith("/"):
             kw = line.split()[0].upper()
@@ -121,8 +122,17 @@ def _blocks(text: str) -> list[_Block]:
             cur = _Block(kw, [], no)
             blocks.append(cur)
         elif cur is not None:
+            # a blank line inside a keyword is a card whose fields all take their defaults
             cur.lines.append(line.expandtabs())
+    for b in blocks:
+        while b.lines and not b.lines[-1].strip():
+            b.lines.pop()
     return blocks
+
+
+def _data(lines) -> list[str]:
+    """The non-blank lines of a list keyword (nodes, elements, function points)."""
+    return [line for line in lines if line.strip()]
 
 
 # ----------------------------------------------------------------------------- deck content
@@ -148,7 +158,7 @@ def _law2(b: _Block) -> M.Law2:
     """hm_read_mat02_jc.F90:107-138, with the card layout of matl2_plas_johns.cfg."""
     b.need(5)
     c = b.lines[1:]
-    rho = _real(_cut(c[0], (20,))[0])
+    rho, rho_o = (_real(x) for x in _cut(c[0], (20, 20)))
     e, nu, iflag, vp, pmin = _cut(c[1], (20, 20, 10, 10, 20))
     e, nu, iflag, vp, pmin = _real(e), _real(nu), _int(iflag), _int(vp), _real(pmin)
     a, bb, n, epsm, sigm = (_real(s) for s in _cut(c[2], (20,) * 5))
@@ -158,6 +168,8 @@ def _law2(b: _Block) -> M.Law2:
     where = f"{b.keyword} (line {b.lineno})"
     if iflag != 0:
         raise DeckError(f"{where}: Iflag = {iflag} (yield and UTS input) is not read here; give a, b, n")
+    if vp == 1:
+        raise DeckError(f"{where}: VP = 1 (plastic strain rate, filtered after the return) is not on this path")
     if pmin != 0.0:
         raise DeckError(f"{where}: a pressure cut-off Pmin is not on this path")
     if chard != 0.0:
@@ -166,6 +178,8 @@ def _law2(b: _Block) -> M.Law2:
         raise DeckError(f"{where}: thermal softening (T_melt) is not on this path")
     if rho <= 0.0 or e <= 0.0:
         raise DeckError(f"{where}: density and Young's modulus must be positive")
+    if rho_o != 0.0 and rho_o != rho:
+        raise DeckError(f"{where}: a reference density RHO_O different from RHO_I is not on this path")
     return M.Law2(rho=rho, young=e, nu=nu, a=a, b=bb, n=n, epsmax=epsm, sigmax=sigm, c=cc, eps0=eps0,
                   icc=icc, vp=vp, fcut=fcut, m_exp=m_exp)
 
@@ -173,13 +187,22 @@ def _law2(b: _Block) -> M.Law2:
 def _solid_prop(b: _Block) -> M.SolidProp:
     """prop_p14_solid.cfg cards; QA/QB defaults of hm_read_prop14.F:355-369 (Isolid=1)."""
     b.need(3)
-    f = _cut(b.lines[1], (10,) * 8)
+    f = _cut(b.lines[1], (10,) * 8 + (20,))
     isolid, ismstr, iale, icpre = (_int(s) for s in f[:4])
+    iframe, dn = _int(f[7]), _real(f[8])
     where = f"{b.keyword} (line {b.lineno})"
     if isolid != 1:
         raise DeckError(f"{where}: Isolid = {isolid}; this path runs Isolid = 1 (state it explicitly)")
     if ismstr != 0 or iale != 0 or icpre != 0:
         raise DeckError(f"{where}: Ismstr/Iale/Icpre must keep their defaults on this path")
+    if iframe not in (0, 1):
+        # Iframe = 2 is the corotational frame (JCVT, hm_read_prop14.F:277-282): not SDEFO3's JCVT=0 branch
+        raise DeckError(f"{where}: Iframe = {iframe}; this path runs the global frame (Iframe 0 or 1)")
+    if dn != 0.0:
+        raise DeckError(f"{where}: numerical damping Dn is not on this path")
+    if len(b.lines) > 4 and len(_cut(b.lines[4], (10, 10, 10))) > 2 and _int(_cut(b.lines[4], (10, 10, 10))[2]):
+        # Icontrol = 1 changes the hourglass default QH from 0.1 to 1 (hm_read_prop14.F:366-367)
+        raise DeckError(f"{where}: Icontrol is not on this path")
     qa, qb, qh, lam, mu = (_real(s) for s in _cut(b.lines[2], (20,) * 5))
     if lam != 0.0 or mu != 0.0:
         raise DeckError(f"{where}: numerical viscosities lambda_v / mu_v are not on this path")
@@ -226,7 +249,22 @@ def _ids(lines) -> list[int]:
 
 # Keywords that do not change the model on this path: titles, output requests, run controls, and
 # defaults that the properties read here must state explicitly anyway.
-_IGNORED = {"TITLE", "ANALY", "IOFLAG", "SPMD", "DEF_SHELL", "DEF_SOLID", "SUBSET", "SHFRA", "TH"}
+_IGNORED = {"TITLE", "ANALY", "IOFLAG", "SPMD", "SUBSET", "SHFRA", "TH"}
+# Global property defaults: a property field left 0 takes their value (hm_read_prop14.F:277-291,
+# hm_read_prop01.F:194-273), so they are read only when they leave every default as it is.
+_DEFAULTS = {"DEF_SHELL", "DEF_SOLID"}
+
+
+def _all_default(lines) -> bool:
+    """True when every field of the cards is blank or zero."""
+    for line in lines:
+        for tok in line.split():
+            try:
+                if float(tok.replace("d", "e").replace("D", "e")) != 0.0:
+                    return False
+            except ValueError:
+                return False
+    return True
 
 
 def parse(text: str) -> Deck:
@@ -241,6 +279,10 @@ def parse(text: str) -> Deck:
         where = f"{b.keyword} (line {b.lineno})"
         if head in _IGNORED:
             continue
+        if head in _DEFAULTS:
+            if not _all_default(b.lines):
+                raise DeckError(f"{where}: /{head} changes property defaults; state the flags in the properties")
+            continue
         if head == "BEGIN":
             if len(b.lines) >= 4 and b.lines[2].split() != b.lines[3].split():
                 raise DeckError(f"{where}: input and work units differ; convert the deck first")
@@ -248,19 +290,19 @@ def parse(text: str) -> Deck:
         if head in ("UNIT", "RANDOM"):
             raise DeckError(f"{where}: keyword changes the model ({head.lower()}) and is not applied here")
         if head == "NODE":
-            for line in b.lines:
+            for line in _data(b.lines):
                 i, x, y, z = _cut(line, (10, 20, 20, 20))
                 node_ids.append(_int(i))
                 xs.append((_real(x), _real(y), _real(z)))
         elif head == "BRICK":
             part = b.ident()
-            rows = [[_int(s) for s in _cut(line, (10,) * 9)] for line in b.lines]
+            rows = [[_int(s) for s in _cut(line, (10,) * 9)] for line in _data(b.lines)]
             a = np.array(rows, dtype=np.int64).reshape(-1, 9)
             bricks.append((part, a[:, 0], a[:, 1:]))
         elif head == "SHELL":
             part = b.ident()
             ids, conn, thk = [], [], []
-            for line in b.lines:
+            for line in _data(b.lines):
                 f = _cut(line, (10,) * 5 + (20, 20))
                 ids.append(_int(f[0]))
                 conn.append([_int(s) for s in f[1:5]])
@@ -307,6 +349,8 @@ def parse(text: str) -> Deck:
             vx, vy, vz, gid, skew = _cut(b.lines[1], (20, 20, 20, 10, 10))
             if _int(skew) != 0:
                 raise DeckError(f"{where}: initial velocities in a skew frame are not on this path")
+            if len(b.lines) > 2 and len(_cut(b.lines[2], (20, 10))) > 1 and _int(_cut(b.lines[2], (20, 10))[1]):
+                raise DeckError(f"{where}: an initial velocity started by a sensor is not on this path")
             if len(b.lines) > 2 and _real(_cut(b.lines[2], (20,))[0]) != 0.0:
                 raise DeckError(f"{where}: a delayed initial velocity (Tstart) is not on this path")
             inivel.append((_real(vx), _real(vy), _real(vz), _int(gid)))
@@ -324,7 +368,7 @@ def parse(text: str) -> Deck:
         elif head == "FUNCT":
             b.need(1)
             ys = []
-            for line in b.lines[1:]:
+            for line in _data(b.lines[1:]):
                 _, y = _cut(line, (20, 20))
                 ys.append(_real(y))
             funct[b.ident()] = ys
@@ -425,9 +469,18 @@ def to_model(deck: Deck, dtfac_hex: float = 0.9, dtfac_q4: float = 0.9, dtmx: fl
                          dtfac_hex=dtfac_hex, dtfac_q4=dtfac_q4, dtmx=dtmx)
 
 
+def read_run(path: str, engine: str, **kw) -> tuple[M.Model, EngineSettings]:
+    """A Starter deck and its Engine file: the model (with the Engine file's /DT/BRICK and /DT/SHELL
+    scale factors applied) and the Engine settings, whose `tstop` is the run's end time."""
+    with open(engine, "r", encoding="latin-1") as f:
+        e = parse_engine(f.read())
+    return read(path, engine, **kw), e
+
+
 def read(path: str, engine: str | None = None, **kw) -> M.Model:
     """Read a Starter deck file into a model. With `engine`, the Engine file's /DT/BRICK and
-    /DT/SHELL scale factors are applied; other `to_model` keywords pass through."""
+    /DT/SHELL scale factors are applied; other `to_model` keywords pass through.  The model has
+    no end time: take it from `read_run` (or `parse_engine(...).tstop`)."""
     if engine is not None:
         with open(engine, "r", encoding="latin-1") as f:
             e = parse_engine(f.read())
@@ -447,7 +500,7 @@ class EngineSettings:
 
 
 # Engine keywords that request output or name the run; they leave the cycle unchanged.
-_ENGINE_IGNORED = {"TITLE", "VERS", "ANIM", "TFILE", "PRINT", "RFILE", "MON", "H3D", "OUTP", "STOP"}
+_ENGINE_IGNORED = {"TITLE", "VERS", "ANIM", "TFILE", "PRINT", "RFILE", "MON", "H3D", "OUTP"}
 
 
 def _free(line: str) -> list[str]:
@@ -468,6 +521,14 @@ def parse_engine(text: str) -> EngineSettings:
         if p[0] == "RUN":
             b.need(1)
             out.tstop = _real(_free(b.lines[0])[0])
+        elif p[0] == "STOP":
+            # /STOP: Emax, Mmax, Nmax (energy / mass error limits that end the run) then output
+            # counts (freform.F:770-786); a limit of 0 means none
+            if len(p) > 1 and p[1] == "LSENS":
+                raise DeckError(f"{where}: a run stopped by sensors is not on this path")
+            vals = _free(b.lines[0]) if b.lines else []
+            if any(_real(v) != 0.0 for v in vals[:3]):
+                raise DeckError(f"{where}: energy or mass error limits that stop the run are not applied here")
         elif p[0] == "DT":
             kind = p[1] if len(p) > 1 else ""
             extra = [x for x in p[2:] if not x.isdigit()]
@@ -508,6 +569,12 @@ def write(m: M.Model, title: str = "model") -> str:
     gives the same model wherever the reals fit 20 characters exactly."""
     if m.hexp is None and m.q4 is None:
         raise DeckError("the model has no elements")
+    if np.any(np.asarray(m.vr) != 0.0):
+        raise DeckError("initial rotational velocities are not written (the reader takes /INIVEL/TRA only)")
+    if m.q4 is not None and (m.q4.prop.ishear != 0 or m.q4.prop.dtmin != 0.0):
+        raise DeckError("a non-default shell Ishear or dt_min is not written")
+    if m.dtmx != M.EP20:
+        raise DeckError("a maximum time step DTMX is not written")
     out = io.StringIO()
     w = out.write
     w(f"#RADIOSS STARTER\n/BEGIN\n{title}\n      2025         0\n"
@@ -576,3 +643,15 @@ def write(m: M.Model, title: str = "model") -> str:
               f"{_r(0.0)}{_r(1.0)}{_r(0.0)}{_r(0.0)}\n")
     w("/END\n")
     return out.getvalue()
+
+
+def write_engine(m: M.Model, tstop: float, title: str = "model") -> str:
+    """An Engine file for a model written by `write`: /RUN with the end time and the /DT/BRICK,
+    /DT/SHELL scale factors of the model's families (read back by `parse_engine` / `read_run`)."""
+    out = [f"#RADIOSS ENGINE\n/RUN/{title}/1\n{_r(tstop)}\n"]
+    if m.hexp is not None:
+        out.append(f"/DT/BRICK\n{_r(m.hexp.dtfac)}{_r(0.0)}\n")
+    if m.q4 is not None:
+        out.append(f"/DT/SHELL\n{_r(m.q4.dtfac)}{_r(0.0)}\n")
+    out.append("/END\n")
+    return "".join(out)

@@ -94,12 +94,54 @@ def _r(v):
     ("/TITLE", "/RANDOM\n" + _r(1) + "\n/TITLE", "random"),
     ("/MAT/PLAS_JOHNS/1", "/MAT/ELASTIC/1", "law 2"),
     ("   111 000" + _i(0), "   111 000" + _i(5), "skew"),
+    # law 2 with VP = 1 (plastic strain rate) and a reference density RHO_O != RHO_I
+    (_r(210000) + _r(".3") + " " * 40, _r(210000) + _r(".3") + _i(0) + _i(1) + _r(0), "VP = 1"),
+    ("               7.8-9\n", "               7.8-9" + _r("7.0-9") + "\n", "RHO_O"),
+    # solid Iframe = 2 (corotational), Icontrol = 1
+    (_i(1) + _i(0) * 7, _i(1) + _i(0) * 6 + _i(2), "Iframe = 2"),
+    # property defaults changed globally, and an initial velocity started by a sensor
+    ("/TITLE", "/DEF_SOLID\n" + _i(0) + _i(2) + "\n/TITLE", "DEF_SOLID"),
+    ("/TITLE", "/DEF_SHELL\n" + _i(0) * 5 + _i(2) + "\n/TITLE", "DEF_SHELL"),
+    (_r("-5.0+3") + _i(2) + _i(0) + "\n", _r("-5.0+3") + _i(2) + _i(0) + "\n" + _r(0) + _i(7) + "\n",
+     "sensor"),
 ])
 def test_decks_off_this_path_are_refused(old, new, msg):
     text = _text()
     assert text.count(old) == 1, old
     with pytest.raises(D.DeckError, match=msg):
         D.to_model(D.parse(text.replace(old, new)))
+
+
+def test_property_defaults_that_change_nothing_are_read():
+    text = _text().replace("/TITLE", "/DEF_SOLID\n" + _i(0) * 8 + "\n/DEF_SHELL\n" + _i(0) * 6 + "\n/TITLE")
+    _same(D.to_model(D.parse(text)), D.read(FIXTURE))
+
+
+def test_solid_icontrol_is_refused():
+    text = _text()
+    card = "#             dt_min\n"
+    i = text.index(card, text.index("/PROP/SOLID/1")) + len(card)
+    j = text.index("\n", i) + 1
+    text = text[:j] + _i(0) + _i(0) + _i(1) + "\n" + text[j:]
+    with pytest.raises(D.DeckError, match="Icontrol"):
+        D.to_model(D.parse(text))
+
+
+def test_writer_refuses_what_the_reader_cannot_take_back():
+    m = M.block_with_plate(n=2, plate=4)
+    m.vr[0, 2] = 1.0
+    with pytest.raises(D.DeckError, match="rotational"):
+        D.write(m)
+
+
+def test_engine_file_round_trip(tmp_path):
+    m = M.block_with_plate(n=2, plate=4)
+    m.hexp.dtfac, m.q4.dtfac = 0.67, 0.5
+    (tmp_path / "m_0000.rad").write_text(D.write(m))
+    (tmp_path / "m_0001.rad").write_text(D.write_engine(m, 3.0e-4))
+    m2, e = D.read_run(str(tmp_path / "m_0000.rad"), str(tmp_path / "m_0001.rad"))
+    assert e.tstop == 3.0e-4 and (m2.hexp.dtfac, m2.q4.dtfac) == (0.67, 0.5)
+    _same(m2, m)
 
 
 ENGINE = """#RADIOSS ENGINE
@@ -126,7 +168,9 @@ def test_engine_file_sets_the_time_step_factors(tmp_path):
 
 
 @pytest.mark.parametrize("card,msg", [("/DT/BRICK/CST/0\n0.67 0", "scale factors"),
-                                      ("/DT/SHELL\n0.5 1e-7", "minimum"), ("/DAMP/1\nx", "not applied")])
+                                      ("/DT/SHELL\n0.5 1e-7", "minimum"), ("/DAMP/1\nx", "not applied"),
+                                      ("/STOP\n0.1 0 0 0 0", "error limits"), ("/STOP/LSENS/1\n3", "sensors")])
 def test_engine_keywords_off_this_path_are_refused(card, msg):
+    assert D.parse_engine(ENGINE.replace("/END", "/STOP\n0 0 0 1 1\n/END")).tstop == 2.0e-4
     with pytest.raises(D.DeckError, match=msg):
         D.parse_engine(ENGINE.replace("/END", card + "\n/END"))
