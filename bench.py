@@ -66,9 +66,9 @@ def gpu_run(model, precision: str, steps: int, warmup: int):
 
 
 def kernel_name(family: str, real: str, variant: int) -> str:
-    """Name of an element-kernel launch variant as the profiles list it (workgroup size 256 for
-    even variants, 64 for odd ones; register budget for 2 or 3 waves per SIMD)."""
-    return f"k_{family}_forces<{real}, {real}, {64 if variant & 1 else 256}, {2 + (variant >> 1)}>"
+    """Name of an element-kernel launch variant as the profiles list it (one 256-element tile per
+    workgroup; register budget for 2 waves per SIMD for even variants, 3 for odd ones)."""
+    return f"k_{family}_forces<{real}, {real}, 256, {2 + (variant & 1)}>"
 
 
 def measured_traffic(kernel: str):

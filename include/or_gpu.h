@@ -130,7 +130,12 @@ int or_model_upload_nodes(or_model *m, const double *x, const double *v, const d
  * (wrrestp.F:1188-1194) with the element buffers. */
 int or_model_upload_displacement(or_model *m, const double *d);
 /* conn: (ne,8) 0-based node ids; state arrays: sig SIG(NEL,6) component-major over all hexes,
- * eint, rho, vol0, pla, epsd, offg, qvis (ne each); NULL state arrays start at zero (offg at one). */
+ * eint, rho, vol0, pla, epsd, offg, qvis (ne each); NULL state arrays start at zero (offg at one).
+ * Element arrays are always given and returned in the caller's element order.  The first
+ * element upload of a family fixes its storage order on the device: tiles of 256 elements
+ * ordered along a space-filling curve of the element centroids when the nodes are already
+ * uploaded (the order the LDS assembly profits from), the caller's order otherwise.  Later
+ * uploads may change the state, not the connectivity. */
 int or_model_upload_hex(or_model *m, const int32_t *conn, const double *sig, const double *eint,
                         const double *rho, const double *vol0, const double *pla, const double *epsd,
                         const double *offg, const double *qvis);
@@ -170,8 +175,9 @@ int or_model_kernel_times(or_model *m, int reset, double *hex_ms, double *q4_ms,
 int or_model_set_timing(or_model *m, int enable);
 /* Bytes of device memory held by the model. */
 int or_model_footprint(const or_model *m, int64_t *bytes);
-/* Bytes a model will hold once created and uploaded, computed without allocating
- * anything (admission by HBM footprint).  strain_rate_filter: the shells' law has
+/* An upper bound of the bytes a model will hold once created and uploaded, computed
+ * without allocating anything (admission by HBM footprint); it counts every skyline
+ * row as a stored row, where the element tiles sum most of them in LDS.  strain_rate_filter: the shells' law has
  * the strain-rate filter on (ISRATE > 0); rows_hex / rows_q4: (nn) number of hex /
  * shell skyline rows of each node (required when the family is present). */
 int or_model_footprint_estimate(int64_t nn, int64_t ne_hex, int64_t ne_q4, int32_t npt, int precision,
@@ -234,9 +240,10 @@ int or_batch_active(or_batch *b, int32_t *nactive);
 
 /* ---- launch variants of the element kernels and their autotuning ----
  * No counterpart in the reference.  Variant v (0..3) of the hex (family 0) or shell
- * (family 1) kernel sets its workgroup size (256 elements for even v, 64 for odd v) and
- * the register budget it is compiled for (one more wave per SIMD for v >= 2).  All
- * variants compute bit-identical results.  The autotuners time cycles_per_variant real
+ * (family 1) kernel sets the register budget it is compiled for (two waves per SIMD for
+ * even v, three for odd v) and the compiler's instruction scheduler (default for v < 2,
+ * ILP-first for v >= 2); every workgroup is one tile of 256 elements.  All variants
+ * compute bit-identical results.  The autotuners time cycles_per_variant real
  * cycles per candidate (the models advance meanwhile) and keep the fastest:
  * chosen[family] for a model, chosen[2*precision + family] for a batch. */
 int or_model_set_variant(or_model *m, int family, int variant);

@@ -20,12 +20,14 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <numeric>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/or_gpu.h"
+#include "or_assembly.h"
 #include "or_hex.h"
 #include "or_q4.h"
 
@@ -50,17 +52,17 @@ int fail(int code, const std::string &msg) {
 constexpr int kBlock = 256;
 constexpr double kNoEnd = 1.0e300; // end time of a model advanced by or_model_run
 
-// Launch variants of the element kernels.  Variant v picks the workgroup size
-// (256 elements for even v, 64 for odd v), the minimum number of waves per SIMD
-// the register allocation is built for (two for v % 4 < 2, three otherwise, where the
-// FP64 kernels spill to scratch; a wave's register budget is 512 / waves VGPRs and
-// AGPRs together), and the instruction scheduler (default for v < 4, ILP-first for
-// v >= 4).
+// Launch variants of the element kernels.  Variant v picks the minimum number of waves
+// per SIMD the register allocation is built for (two for even v, three for odd v, where
+// the FP64 kernels spill to scratch; a wave's register budget is 512 / waves VGPRs and
+// AGPRs together) and the instruction scheduler (default for v < 2, ILP-first for v >= 2).
+// Every element workgroup is one tile of kTile elements, the unit of the LDS assembly.
 // Every variant runs the same arithmetic, so the choice (or_model_set_variant,
 // or_model_autotune, or_batch_autotune) changes speed only, never a result bit.
-constexpr int kVariants = 8;
-constexpr int variant_block(int v) { return (v & 1) ? 64 : 256; }
-template <typename R> constexpr int variant_waves(int v) { return 2 + ((v >> 1) & 1); }
+constexpr int kVariants = 4;
+constexpr int kTile = orasm::kTile;
+static_assert(kTile == kBlock, "element tiles and node blocks share the workgroup size");
+template <typename R> constexpr int variant_waves(int v) { return 2 + (v & 1); }
 
 // MSTOP codes (include/or_gpu.h): an element's non-positive Jacobian, a time step <= 0
 constexpr int32_t kMstopJacobian = 1, kMstopTimeStep = 2;
@@ -68,24 +70,20 @@ constexpr int32_t kMstopJacobian = 1, kMstopTimeStep = 2;
 // node code bits (include/or_gpu.h)
 constexpr int32_t kTX = 4, kTY = 2, kTZ = 1, kRX = 32, kRY = 16, kRZ = 8, kIX = 256, kIY = 128, kIZ = 64;
 
-// Skyline rows of the resident model hold the force (and moment) words of FSKY.
-// The stiffness words (FSKY 7, 8) have no reader on this path (element time
-// step, NODADT=0, no interfaces), so the resident rows leave them out; the group
-// drop-ins write them.
-//
-// Layout (sliced ELL with component planes): nodes are taken in slices of kBlock,
-// the node kernel's block.  Slice b owns slots slab[b] .. slab[b] + D_b - 1, D_b the
-// largest row count of its nodes; row j of node n is slot slab[n / kBlock] + j in
-// lane n % kBlock, and its component c is word (slot * W + c) * kBlock + lane
-// (W = 3 words for hex rows, 6 for shell rows).  The node kernel then reads
-// row j of 64 consecutive nodes as 64 consecutive words per component, and
-// neighbouring elements write neighbouring words.  Element kernels hold each of
-// their rows as the position slot * kBlock + lane.
+// Assembly of the resident model (plan in or_assembly.h).  The rows of a node that
+// lead its ASSPAR4 order and whose elements share the tile of its first element are
+// summed in that tile's LDS from zero and stored once, in FP64, as the node's prefix
+// (pre[n][W]); the node kernel starts from the prefix and adds the node's remaining
+// ("rest") rows in order.  Rest rows are stored node after node (CSR, rs[n] ..
+// rs[n + 1] - 1), W words each in the storage type: W = 3 (fx, fy, fz) for hexes,
+// 6 (plus mx, my, mz) for shells.  Every node with hex rows has a hex prefix; a node
+// with shell rows has a shell prefix only when it has no hex rows (its sum does not
+// start with a shell row otherwise).  The stiffness words (FSKY 7, 8) have no reader
+// on this path (element time step, NODADT=0, no interfaces), so the resident rows
+// leave them out; the group drop-ins write them.
 constexpr int kHexRowWords = 3; // fx, fy, fz
 constexpr int kQ4RowWords = 6;  // fx, fy, fz, mx, my, mz
-__device__ __forceinline__ int64_t row_word(int32_t pos, int W) {
-  return ((int64_t)(pos / kBlock) * W) * kBlock + (pos % kBlock);
-}
+typedef uint32_t v4u __attribute__((ext_vector_type(4))); // a prefix's 8 LDS sources
 
 // Workgroups of a launch are dispatched to the 8 XCDs in turn (workgroup b runs on
 // XCD b mod 8), each XCD with its own L2.  Element kernels renumber their workgroups
@@ -125,7 +123,7 @@ __device__ __forceinline__ int32_t time_done(const DevTime *t) { return to_globa
 
 template <typename S> struct HexPtrs {
   gptr<const int32_t> conn;  // [8][ne]
-  gptr<const int32_t> iads;  // [8][ne] row position
+  gptr<const int32_t> rest;  // [8][ne] rest row of the corner, or -1 (summed in LDS)
   gptr<S> sig;               // [6][ne]
   gptr<S> eint, rho, pla, epsd, offg, qvis;
   gptr<const S> vol0;
@@ -134,7 +132,7 @@ template <typename S> struct HexPtrs {
 
 template <typename S> struct Q4Ptrs {
   gptr<const int32_t> conn;  // [4][ne]
-  gptr<const int32_t> iadc;  // [4][ne] row position
+  gptr<const int32_t> rest;  // [4][ne] rest row of the corner, or -1 (summed in LDS)
   gptr<S> offg, thk;
   gptr<const S> thke;
   gptr<S> forc, mom, gstr, hourg, eint, epsd; // [5][ne] [3][ne] [8][ne] [5][ne] [2][ne] [ne]
@@ -151,7 +149,8 @@ struct NodePtrs {
   const int32_t *code;   // [nn]
   const double *vimp;    // [nn][3]
   const int32_t *deg_h, *deg_q;    // [nn] rows of each node per family (null when the family is absent)
-  const int32_t *slab_h, *slab_q;  // [ceil(nn / kBlock)] first slot of each node slice
+  const int32_t *rs_h, *rs_q;      // [nn + 1] first rest row of each node per family
+  const double *pre_h, *pre_q;     // [nn][3], [nn][6] FP64 prefixes
   int64_t nn;
 };
 
@@ -160,17 +159,22 @@ struct NodePtrs {
 // per slot of a batch); a launch covers any number of models through a tile list.
 struct ModelDev {
   // hexes; element state in the model's storage type (double, or float in the split form)
-  const int32_t *hconn, *hiads;
+  const int32_t *hconn, *hrest;
+  const int32_t *hfr_start, *hfr_node; // prefixes of each hex tile: [ntiles + 1], [nfr]
+  const uint16_t *hfr_src;              // [nfr][8] LDS sources of each prefix
   void *hsig, *heint, *hrho, *hpla, *hepsd, *hoffg, *hqvis;
   const void *hvol0;
   int64_t neh;
   // Q4 shells
-  const int32_t *qconn, *qiadc;
+  const int32_t *qconn, *qrest;
+  const int32_t *qfr_start, *qfr_node;
+  const uint16_t *qfr_src;
   void *qoffg, *qthk, *qforc, *qmom, *qgstr, *qhourg, *qeint, *qepsd, *qsig, *qpla, *qepsdip;
   const void *qthke;
   int64_t neq;
   NodePtrs nd;
-  void *skh, *skq;            // skyline rows (storage type)
+  void *rowh, *rowq;          // rest rows (storage type)
+  double *preh, *preq;        // prefixes (written by the element kernels)
   DevTime *ts;                // [2] cycle scalars, by cycle parity
   unsigned long long *dtmin;  // [2] element time-step minimum (FP64 bits), by parity
   int *flags;                 // bit 0: an element reported MSTOP
@@ -189,12 +193,12 @@ struct Tile {
 };
 
 template <typename S> __device__ __forceinline__ HexPtrs<S> hex_ptrs(const ModelDev &M) {
-  return HexPtrs<S>{to_global(M.hconn), to_global(M.hiads), to_global<S>(M.hsig), to_global<S>(M.heint),
+  return HexPtrs<S>{to_global(M.hconn), to_global(M.hrest), to_global<S>(M.hsig), to_global<S>(M.heint),
                     to_global<S>(M.hrho), to_global<S>(M.hpla), to_global<S>(M.hepsd), to_global<S>(M.hoffg),
                     to_global<S>(M.hqvis), to_global<const S>(M.hvol0), M.neh};
 }
 template <typename S> __device__ __forceinline__ Q4Ptrs<S> q4_ptrs(const ModelDev &M) {
-  return Q4Ptrs<S>{to_global(M.qconn), to_global(M.qiadc), to_global<S>(M.qoffg), to_global<S>(M.qthk),
+  return Q4Ptrs<S>{to_global(M.qconn), to_global(M.qrest), to_global<S>(M.qoffg), to_global<S>(M.qthk),
                    to_global<const S>(M.qthke), to_global<S>(M.qforc), to_global<S>(M.qmom), to_global<S>(M.qgstr),
                    to_global<S>(M.qhourg), to_global<S>(M.qeint), to_global<S>(M.qepsd), to_global<S>(M.qsig),
                    to_global<S>(M.qpla), to_global<S>(M.qepsdip), M.neq};
@@ -226,10 +230,64 @@ __device__ __forceinline__ void load_node(gptr<const double> xv, int64_t n, doub
   x = a.x; y = a.y; z = b.x; vx = b.y; vy = q.x; vz = q.y;
 }
 
+// ---------------------------------------------------------------- tile assembly
+// The element chains leave every corner's W row words in LDS, in the storage type (the
+// value a row holds): corner k, word w of the tile's element t at lds[(k * W + w) * kTile + t].
+// After a barrier the tile writes its rest rows from there and sums its prefixes.
+template <typename S, int W>
+__device__ __forceinline__ void lds_corner(S *lds, int k, const S (&w)[W]) {
+#pragma unroll
+  for (int c = 0; c < W; ++c) lds[(k * W + c) * kTile + threadIdx.x] = w[c];
+}
+
+// The rest rows of this thread's element (corners whose rest row is >= 0).
+template <typename S, int W, int NK>
+__device__ __forceinline__ void tile_rest_rows(const S *lds, gptr<const int32_t> rest, int64_t ne, int64_t i,
+                                               gptr<S> rows) {
+  if (i >= ne) return;
+#pragma unroll
+  for (int k = 0; k < NK; ++k) {
+    const int32_t r = rest[(int64_t)k * ne + i];
+    if (r >= 0) {
+      const gptr<S> w = rows + (int64_t)r * W;
+#pragma unroll
+      for (int c = 0; c < W; ++c) w[c] = lds[(k * W + c) * kTile + threadIdx.x];
+    }
+  }
+}
+
+// After the tile's corners are in LDS (and a barrier): every prefix of the tile, summed
+// in row order from zero in FP64 (ASSPAR4's running sum: A = A + FSKY), stored in pre[n][W].
+template <typename S, int W>
+__device__ __forceinline__ void tile_prefixes(const S *lds, gptr<const int32_t> fr_start, gptr<const int32_t> fr_node,
+                                              gptr<const v4u> fr_src, int64_t tile, gptr<double> pre) {
+  const int32_t j0 = fr_start[tile], j1 = fr_start[tile + 1];
+  for (int32_t j = j0 + (int32_t)threadIdx.x; j < j1; j += kTile) {
+    const int32_t n = fr_node[j];
+    const v4u src = fr_src[j];
+    double a[W];
+#pragma unroll
+    for (int c = 0; c < W; ++c) a[c] = 0.0;
+#pragma unroll
+    for (int r = 0; r < orasm::kRunCap; ++r) {
+      const uint32_t u = (src[r >> 1] >> (16 * (r & 1))) & 0xFFFFu;
+      if (u == orasm::kNoSrc) break;
+      const int base = (int)(u >> 8) * W * kTile + (int)(u & 0xFFu);
+#pragma unroll
+      for (int c = 0; c < W; ++c) a[c] = a[c] + double(lds[base + c * kTile]);
+    }
+    const gptr<double> p = pre + (int64_t)n * W;
+#pragma unroll
+    for (int c = 0; c < W; ++c) p[c] = a[c];
+  }
+}
+
 // ---------------------------------------------------------------- hex element kernel
 template <typename R, typename S, int TB, int W>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_hex_forces(
     const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity) {
+  static_assert(TB == kTile, "a hex workgroup is one assembly tile");
+  __shared__ S lds[8 * kHexRowWords * kTile];
   const Tile tl = tiles[xcd_contiguous_block(blockIdx.x, gridDim.x)];
   const ModelDev &M = md[tl.model];
   const DevTime *ts = M.ts;
@@ -238,7 +296,6 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   const int stopped = time_done(ts + parity);
   const HexPtrs<S> e = hex_ptrs<S>(M);
   const gptr<const double> xv = to_global<const double>(M.nd.xv);
-  const gptr<S> sky = to_global<S>(M.skh);
   const HexConsts &c = M.hc;
   const int64_t i = (int64_t)tl.first + threadIdx.x;
   double dtx = kEP06;
@@ -268,12 +325,11 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
       st.epsd = e.epsd[i]; st.qvis = e.qvis[i];
     };
     if constexpr (sizeof(R) == 8) {
-      // FP64: each node's force triple goes to its skyline row as soon as it is final,
-      // which keeps the chain within two waves' register budget
+      // FP64: each node's force triple leaves for LDS as soon as it is final, which keeps
+      // the chain within two waves' register budget
       auto write_row = [&](int k, R fx, R fy, R fz) {
-        if (stopped) return;
-        const gptr<S> r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
-        r[0] = S(fx); r[kBlock] = S(fy); r[2 * kBlock] = S(fz);
+        const S w[3] = {S(fx), S(fy), S(fz)};
+        lds_corner<S, 3>(lds, k, w);
       };
       hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, write_row, load_rest);
     } else {
@@ -281,12 +337,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
       R f1[8], f2[8], f3[8];
       auto keep = [&](int k, R fx, R fy, R fz) { f1[k] = fx; f2[k] = fy; f3[k] = fz; };
       hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, keep, load_rest);
-      if (!stopped) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const gptr<S> r = sky + row_word(e.iads[(int64_t)k * e.ne + i], kHexRowWords);
-          r[0] = S(f1[k]); r[kBlock] = S(f2[k]); r[2 * kBlock] = S(f3[k]);
-        }
+      for (int k = 0; k < 8; ++k) {
+        const S w[3] = {S(f1[k]), S(f2[k]), S(f3[k])};
+        lds_corner<S, 3>(lds, k, w);
       }
     }
     if (!stopped) {
@@ -298,7 +352,11 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
     dtx = o.dtx;
     mstop = o.mstop;
   }
-  if (stopped) return;
+  if (stopped) return; // uniform over the workgroup
+  __syncthreads();
+  tile_rest_rows<S, kHexRowWords, 8>(lds, e.rest, e.ne, i, to_global<S>(M.rowh));
+  tile_prefixes<S, kHexRowWords>(lds, to_global(M.hfr_start), to_global(M.hfr_node),
+                                 to_global<const v4u>(M.hfr_src), tl.first / kTile, to_global(M.preh));
   if (mstop) atomicOr(M.flags, kMstopJacobian);
   const double m = block_min<TB>(dtx);
   if (threadIdx.x == 0) atomicMin(M.dtmin + parity, dbits(m));
@@ -357,6 +415,8 @@ template <typename R, typename S> struct Q4IpIO {
 template <typename R, typename S, int TB, int W>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_q4_forces(
     const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity) {
+  static_assert(TB == kTile, "a shell workgroup is one assembly tile");
+  __shared__ S lds[4 * kQ4RowWords * kTile];
   const Tile tl = tiles[xcd_contiguous_block(blockIdx.x, gridDim.x)];
   const ModelDev &M = md[tl.model];
   const DevTime *ts = M.ts;
@@ -364,7 +424,6 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   const Q4Ptrs<S> q = q4_ptrs<S>(M);
   const gptr<const double> xv = to_global<const double>(M.nd.xv);
   const gptr<const double> vr = to_global<const double>(M.nd.vr);
-  const gptr<S> sky = to_global<S>(M.skq);
   const Q4Consts &c = M.qc;
   const int64_t i = (int64_t)tl.first + threadIdx.x;
   double dt = kEP06;
@@ -400,14 +459,17 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
       q.offg[i] = S(st.offg); q.thk[i] = S(st.thk); q.epsd[i] = S(st.epsd);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const gptr<S> r = sky + row_word(q.iadc[(int64_t)k * q.ne + i], kQ4RowWords);
-        r[0] = S(-o.f[0][k]); r[kBlock] = S(-o.f[1][k]); r[2 * kBlock] = S(-o.f[2][k]);
-        r[3 * kBlock] = S(-o.m[0][k]); r[4 * kBlock] = S(-o.m[1][k]); r[5 * kBlock] = S(-o.m[2][k]);
+        const S w[6] = {S(-o.f[0][k]), S(-o.f[1][k]), S(-o.f[2][k]), S(-o.m[0][k]), S(-o.m[1][k]), S(-o.m[2][k])};
+        lds_corner<S, 6>(lds, k, w);
       }
     }
     dt = o.dt;
   }
-  if (stopped) return;
+  if (stopped) return; // uniform over the workgroup
+  __syncthreads();
+  tile_rest_rows<S, kQ4RowWords, 4>(lds, q.rest, q.ne, i, to_global<S>(M.rowq));
+  tile_prefixes<S, kQ4RowWords>(lds, to_global(M.qfr_start), to_global(M.qfr_node), to_global<const v4u>(M.qfr_src),
+                                tl.first / kTile, to_global(M.preq));
   const double m = block_min<TB>(dt);
   if (threadIdx.x == 0) atomicMin(M.dtmin + parity, dbits(m));
 }
@@ -459,32 +521,48 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
   const Tile tl = tiles[blockIdx.x];
   const ModelDev &M = md[tl.model];
   const NodePtrs &nd = M.nd;
-  const gptr<const S> skh = to_global<const S>(M.skh);
-  const gptr<const S> skq = to_global<const S>(M.skq);
+  const gptr<const S> rowh = to_global<const S>(M.rowh);
+  const gptr<const S> rowq = to_global<const S>(M.rowq);
   DevTime *ts = M.ts;
   unsigned long long *dtmin = M.dtmin;
   const double dtmx = M.dtmx;
   const int64_t n = (int64_t)tl.first + threadIdx.x;
   const DevTime prev = load_time(ts + parity);
   const bool in_range = n < nd.nn;
-  // ASSPAR4: rows of this node in skyline order (solids, then shells), from zero
+  // ASSPAR4: rows of this node in skyline order (solids, then shells), from zero: the
+  // prefix the element tiles summed, then the rest rows in order
   double ax = 0.0, ay = 0.0, az = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
-  const int lane = (int)(n % kBlock);
-  if (skh && in_range) {
-    const int32_t deg = to_global(nd.deg_h)[n];
-    gptr<const S> r = skh + (int64_t)to_global(nd.slab_h)[n / kBlock] * kHexRowWords * kBlock + lane;
-    for (int32_t j = 0; j < deg; ++j, r += kHexRowWords * kBlock) {
-      ax = ax + double(r[0]); ay = ay + double(r[kBlock]); az = az + double(r[2 * kBlock]);
+  int32_t dh = 0;
+  if (nd.deg_h && in_range) {
+    dh = to_global(nd.deg_h)[n];
+    if (dh > 0) {
+      const gptr<const double> p = to_global(nd.pre_h) + 3 * n;
+      ax = p[0]; ay = p[1]; az = p[2];
+      const gptr<const int32_t> rs = to_global(nd.rs_h);
+      const int32_t r1 = rs[n + 1];
+      for (int32_t r = rs[n]; r < r1; ++r) {
+        const gptr<const S> w = rowh + (int64_t)r * kHexRowWords;
+        ax = ax + double(w[0]); ay = ay + double(w[1]); az = az + double(w[2]);
+      }
     }
   }
   bool rot = false; // node carries rotational DOF work this cycle
-  if (skq && in_range) {
-    const int32_t deg = to_global(nd.deg_q)[n];
-    rot = deg > 0;
-    gptr<const S> r = skq + (int64_t)to_global(nd.slab_q)[n / kBlock] * kQ4RowWords * kBlock + lane;
-    for (int32_t j = 0; j < deg; ++j, r += kQ4RowWords * kBlock) {
-      ax = ax + double(r[0]); ay = ay + double(r[kBlock]); az = az + double(r[2 * kBlock]);
-      rx = rx + double(r[3 * kBlock]); ry = ry + double(r[4 * kBlock]); rz = rz + double(r[5 * kBlock]);
+  if (nd.deg_q && in_range) {
+    const int32_t dq = to_global(nd.deg_q)[n];
+    rot = dq > 0;
+    if (dq > 0) {
+      if (dh == 0) { // the node's sum starts with its shell rows: their prefix
+        const gptr<const v2d> p = (gptr<const v2d>)(to_global(nd.pre_q) + 6 * n);
+        const v2d p0 = p[0], p1 = p[1], p2 = p[2];
+        ax = p0.x; ay = p0.y; az = p1.x; rx = p1.y; ry = p2.x; rz = p2.y;
+      }
+      const gptr<const int32_t> rs = to_global(nd.rs_q);
+      const int32_t r1 = rs[n + 1];
+      for (int32_t r = rs[n]; r < r1; ++r) {
+        const gptr<const S> w = rowq + (int64_t)r * kQ4RowWords;
+        ax = ax + double(w[0]); ay = ay + double(w[1]); az = az + double(w[2]);
+        rx = rx + double(w[3]); ry = ry + double(w[4]); rz = rz + double(w[5]);
+      }
     }
   }
   if (prev.done) {
@@ -939,17 +1017,23 @@ struct or_model {
   Q4Consts qc{};
   // nodes
   double *xv = nullptr, *vr = nullptr, *d = nullptr, *ms = nullptr, *in = nullptr, *vimp = nullptr;
-  int32_t *code = nullptr, *deg_h = nullptr, *deg_q = nullptr, *slab_h = nullptr, *slab_q = nullptr;
-  int64_t sky_bytes = 0;               // skyline rows, sized at upload_skyline
+  int32_t *code = nullptr, *deg_h = nullptr, *deg_q = nullptr, *rs_h = nullptr, *rs_q = nullptr;
+  double *preh = nullptr, *preq = nullptr;
+  int64_t sky_bytes = 0;               // rest rows and prefix lists, sized at upload_skyline
+  // element storage order of each family: slot -> element (or_assembly.h tile_order); empty
+  // until the first element upload, fixed after it
+  std::vector<int32_t> hperm, qperm;
   // hex (storage type depends on precision)
-  int32_t *hconn = nullptr, *hiads = nullptr;
+  int32_t *hconn = nullptr, *hrest = nullptr, *hfr_start = nullptr, *hfr_node = nullptr;
+  uint16_t *hfr_src = nullptr;
   void *hsig = nullptr, *heint = nullptr, *hrho = nullptr, *hvol0 = nullptr, *hpla = nullptr, *hepsd = nullptr,
-       *hoffg = nullptr, *hqvis = nullptr, *skh = nullptr;
+       *hoffg = nullptr, *hqvis = nullptr, *rowh = nullptr;
   // Q4
-  int32_t *qconn = nullptr, *qiadc = nullptr;
+  int32_t *qconn = nullptr, *qrest = nullptr, *qfr_start = nullptr, *qfr_node = nullptr;
+  uint16_t *qfr_src = nullptr;
   void *qoffg = nullptr, *qthk = nullptr, *qthke = nullptr, *qforc = nullptr, *qmom = nullptr, *qgstr = nullptr,
        *qhourg = nullptr, *qeint = nullptr, *qepsd = nullptr, *qsig = nullptr, *qpla = nullptr, *qepsdip = nullptr,
-       *skq = nullptr;
+       *rowq = nullptr;
   // scalars
   DevTime *ts = nullptr;               // [2]
   unsigned long long *dtmin = nullptr; // [2]
@@ -1014,9 +1098,8 @@ static int put_elem_tiles(or_model *m) {
   for (int f = 0; f < 2; ++f) {
     const int64_t ne = f == 0 ? m->neh : m->neq;
     if (ne == 0) continue;
-    const int tb = variant_block(f == 0 ? m->var_h : m->var_q);
     std::vector<Tile> tl;
-    for (int64_t first = 0; first < ne; first += tb) tl.push_back(Tile{0, (int32_t)first});
+    for (int64_t first = 0; first < ne; first += kTile) tl.push_back(Tile{0, (int32_t)first});
     OR_HIP(hipMemcpy(f == 0 ? m->tiles_h : m->tiles_q, tl.data(), sizeof(Tile) * tl.size(), hipMemcpyHostToDevice));
     (f == 0 ? m->nt_h : m->nt_q) = (int32_t)tl.size();
   }
@@ -1092,18 +1175,22 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int6
   }
   if (ne_hex > 0) {
     A(&m->deg_h, nn);
-    A(&m->slab_h, (nn + kBlock - 1) / kBlock);
+    A(&m->rs_h, nn + 1);
+    A(&m->preh, 3 * nn);
+    A(&m->hfr_start, (ne_hex + kTile - 1) / kTile + 1);
     A(&m->hconn, 8 * ne_hex);
-    A(&m->hiads, 8 * ne_hex);
+    A(&m->hrest, 8 * ne_hex);
     AS(&m->hsig, 6 * ne_hex);
     AS(&m->heint, ne_hex); AS(&m->hrho, ne_hex); AS(&m->hvol0, ne_hex); AS(&m->hpla, ne_hex);
     AS(&m->hepsd, ne_hex); AS(&m->hoffg, ne_hex); AS(&m->hqvis, ne_hex);
   }
   if (ne_q4 > 0) {
     A(&m->deg_q, nn);
-    A(&m->slab_q, (nn + kBlock - 1) / kBlock);
+    A(&m->rs_q, nn + 1);
+    A(&m->preq, 6 * nn);
+    A(&m->qfr_start, (ne_q4 + kTile - 1) / kTile + 1);
     A(&m->qconn, 4 * ne_q4);
-    A(&m->qiadc, 4 * ne_q4);
+    A(&m->qrest, 4 * ne_q4);
     AS(&m->qoffg, ne_q4); AS(&m->qthk, ne_q4); AS(&m->qthke, ne_q4); AS(&m->qepsd, ne_q4);
     AS(&m->qforc, 5 * ne_q4); AS(&m->qmom, 3 * ne_q4); AS(&m->qgstr, 8 * ne_q4); AS(&m->qhourg, 5 * ne_q4);
     AS(&m->qeint, 2 * ne_q4);
@@ -1115,8 +1202,8 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int6
   A(&m->flags, 1);
   A(&m->dev, 1);
   m->nt_n = (int32_t)((nn + kBlock - 1) / kBlock);
-  A(&m->tiles_h, (ne_hex + 63) / 64); // room for the smallest element block
-  A(&m->tiles_q, (ne_q4 + 63) / 64);
+  A(&m->tiles_h, (ne_hex + kTile - 1) / kTile);
+  A(&m->tiles_q, (ne_q4 + kTile - 1) / kTile);
   A(&m->tiles_n, m->nt_n);
   const int64_t nb = 1024;
   A(&m->energy, 3 + 3 * nb);
@@ -1166,10 +1253,12 @@ int or_model_destroy(or_model *m) {
   if (m->batch) (void)or_batch_remove(m->batch, m);
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
-  void *ptrs[] = {m->xv, m->vr, m->d, m->ms, m->in, m->vimp, m->code, m->deg_h, m->deg_q, m->slab_h, m->slab_q, m->hconn, m->hiads,
-                  m->hsig, m->heint, m->hrho, m->hvol0, m->hpla, m->hepsd, m->hoffg, m->hqvis, m->skh, m->qconn,
-                  m->qiadc, m->qoffg, m->qthk, m->qthke, m->qforc, m->qmom, m->qgstr, m->qhourg, m->qeint, m->qepsd,
-                  m->qsig, m->qpla, m->qepsdip, m->skq, m->ts, m->dtmin, m->flags, m->energy, m->dev,
+  void *ptrs[] = {m->xv, m->vr, m->d, m->ms, m->in, m->vimp, m->code, m->deg_h, m->deg_q, m->rs_h, m->rs_q,
+                  m->preh, m->preq, m->hconn, m->hrest, m->hfr_start, m->hfr_node, m->hfr_src,
+                  m->hsig, m->heint, m->hrho, m->hvol0, m->hpla, m->hepsd, m->hoffg, m->hqvis, m->rowh, m->qconn,
+                  m->qrest, m->qfr_start, m->qfr_node, m->qfr_src, m->qoffg, m->qthk, m->qthke, m->qforc, m->qmom,
+                  m->qgstr, m->qhourg, m->qeint, m->qepsd,
+                  m->qsig, m->qpla, m->qepsdip, m->rowq, m->ts, m->dtmin, m->flags, m->energy, m->dev,
                   m->tiles_h, m->tiles_q, m->tiles_n, m->ecrit};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -1254,28 +1343,63 @@ int or_model_upload_displacement(or_model *m, const double *d) {
   return OR_OK;
 }
 
-template <typename S> static int put(or_model *m, void *dst, const double *src, int64_t n, double dflt) {
+// Element arrays [ncomp][ne] move between the caller's element order and the storage
+// order of slots (perm: slot -> element; empty: the same order).
+static inline int64_t elem_index(const std::vector<int32_t> &perm, int64_t i) {
+  if (perm.empty()) return i;
+  const int64_t ne = (int64_t)perm.size();
+  return (i / ne) * ne + perm[(size_t)(i % ne)];
+}
+template <typename S>
+static int put(or_model *m, void *dst, const double *src, int64_t n, double dflt, const std::vector<int32_t> &perm) {
   std::vector<S> t((size_t)n);
-  for (int64_t i = 0; i < n; ++i) t[i] = (S)(src ? src[i] : dflt);
+  for (int64_t i = 0; i < n; ++i) t[i] = (S)(src ? src[elem_index(perm, i)] : dflt);
   OR_HIP(hipMemcpyAsync(dst, t.data(), sizeof(S) * n, hipMemcpyHostToDevice, m->stream));
   OR_HIP(quiesce(m));
   return OR_OK;
 }
-static int put_state(or_model *m, void *dst, const double *src, int64_t n, double dflt) {
+static int put_state(or_model *m, void *dst, const double *src, int64_t n, double dflt,
+                     const std::vector<int32_t> &perm) {
   if (!dst || n <= 0) return OR_OK;
-  return m->prec == OR_PREC_FP64 ? put<double>(m, dst, src, n, dflt) : put<float>(m, dst, src, n, dflt);
+  return m->prec == OR_PREC_FP64 ? put<double>(m, dst, src, n, dflt, perm) : put<float>(m, dst, src, n, dflt, perm);
 }
-template <typename S> static int get(or_model *m, double *dst, const void *src, int64_t n) {
+template <typename S>
+static int get(or_model *m, double *dst, const void *src, int64_t n, const std::vector<int32_t> &perm) {
   std::vector<S> t((size_t)n);
   OR_HIP(hipMemcpyAsync(t.data(), src, sizeof(S) * n, hipMemcpyDeviceToHost, m->stream));
   OR_HIP(quiesce(m));
-  for (int64_t i = 0; i < n; ++i) dst[i] = (double)t[i];
+  for (int64_t i = 0; i < n; ++i) dst[elem_index(perm, i)] = (double)t[i];
   return OR_OK;
 }
-static int get_state(or_model *m, double *dst, const void *src, int64_t n) {
+static int get_state(or_model *m, double *dst, const void *src, int64_t n, const std::vector<int32_t> &perm) {
   if (!dst || n <= 0) return OR_OK;
   if (!src) return fail(OR_ERR_STATE, "state array not held by this model");
-  return m->prec == OR_PREC_FP64 ? get<double>(m, dst, src, n) : get<float>(m, dst, src, n);
+  return m->prec == OR_PREC_FP64 ? get<double>(m, dst, src, n, perm) : get<float>(m, dst, src, n, perm);
+}
+
+// Node coordinates on the host ([nn][3]), or empty before the nodes are uploaded.
+static int host_coords(or_model *m, std::vector<double> &x) {
+  x.clear();
+  if (!m->have_nodes) return OR_OK;
+  const int64_t nn = m->nn;
+  std::vector<double> xv((size_t)(6 * nn));
+  OR_HIP(quiesce(m));
+  OR_HIP(hipMemcpy(xv.data(), m->xv, sizeof(double) * 6 * nn, hipMemcpyDeviceToHost));
+  x.resize((size_t)(3 * nn));
+  for (int64_t n = 0; n < nn; ++n)
+    for (int d = 0; d < 3; ++d) x[(size_t)(3 * n + d)] = xv[(size_t)(6 * n + d)];
+  return OR_OK;
+}
+
+// The storage order of a family is chosen at its first element upload: tiles from the
+// node coordinates when the nodes are already uploaded, the caller's order otherwise.
+static int choose_order(or_model *m, int nk, int64_t ne, const int32_t *conn, std::vector<int32_t> &perm) {
+  if (!perm.empty() || ne == 0) return OR_OK;
+  std::vector<double> x;
+  int rc = host_coords(m, x);
+  if (rc) return rc;
+  perm = orasm::tile_order(nk, ne, conn, x.empty() ? nullptr : x.data());
+  return OR_OK;
 }
 
 int or_model_upload_hex(or_model *m, const int32_t *conn, const double *sig, const double *eint, const double *rho,
@@ -1286,24 +1410,33 @@ int or_model_upload_hex(or_model *m, const int32_t *conn, const double *sig, con
   if (ne == 0) { m->have_hex = true; return OR_OK; }
   if (!conn || !vol0 || !rho) return fail(OR_ERR_ARG, "conn, rho and vol0 are required");
   OR_HIP(hipSetDevice(m->device));
+  for (int64_t j = 0; j < 8 * ne; ++j)
+    if (conn[j] < 0 || conn[j] >= m->nn) return fail(OR_ERR_ARG, "connectivity node id out of range");
+  if (m->have_hex && m->have_sky) {
+    // the assembly plan holds this connectivity: a later upload may change the state only
+    std::vector<int32_t> cur((size_t)(8 * ne));
+    OR_HIP(hipMemcpy(cur.data(), m->hconn, sizeof(int32_t) * 8 * ne, hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < ne; ++s)
+      for (int k = 0; k < 8; ++k)
+        if (cur[(size_t)(k * ne + s)] != conn[8 * (int64_t)m->hperm[(size_t)s] + k])
+          return fail(OR_ERR_ARG, "hex connectivity differs from the one the skyline was uploaded for");
+  }
+  int rc = choose_order(m, 8, ne, conn, m->hperm);
+  if (rc) return rc;
+  const std::vector<int32_t> &perm = m->hperm;
   std::vector<int32_t> soa((size_t)(8 * ne));
-  for (int64_t e = 0; e < ne; ++e)
-    for (int k = 0; k < 8; ++k) {
-      const int32_t n = conn[8 * e + k];
-      if (n < 0 || n >= m->nn) return fail(OR_ERR_ARG, "connectivity node id out of range");
-      soa[(size_t)(k * ne + e)] = n;
-    }
+  for (int64_t s = 0; s < ne; ++s)
+    for (int k = 0; k < 8; ++k) soa[(size_t)(k * ne + s)] = conn[8 * (int64_t)perm[(size_t)s] + k];
   OR_HIP(hipMemcpyAsync(m->hconn, soa.data(), sizeof(int32_t) * 8 * ne, hipMemcpyHostToDevice, m->stream));
   OR_HIP(quiesce(m));
-  int rc;
-  if ((rc = put_state(m, m->hsig, sig, 6 * ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->heint, eint, ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->hrho, rho, ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->hvol0, vol0, ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->hpla, pla, ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->hepsd, epsd, ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->hoffg, offg, ne, 1.0))) return rc;
-  if ((rc = put_state(m, m->hqvis, qvis, ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->hsig, sig, 6 * ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->heint, eint, ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->hrho, rho, ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->hvol0, vol0, ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->hpla, pla, ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->hepsd, epsd, ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->hoffg, offg, ne, 1.0, perm))) return rc;
+  if ((rc = put_state(m, m->hqvis, qvis, ne, 0.0, perm))) return rc;
   m->have_hex = true;
   return OR_OK;
 }
@@ -1317,31 +1450,40 @@ int or_model_upload_q4(or_model *m, const int32_t *conn, const double *thke, con
   if (ne == 0) { m->have_q4 = true; return OR_OK; }
   if (!conn || !thke) return fail(OR_ERR_ARG, "conn and thke are required");
   OR_HIP(hipSetDevice(m->device));
-  std::vector<int32_t> soa((size_t)(4 * ne));
   for (int64_t e = 0; e < ne; ++e) {
     if (!(thke[e] > 0.0)) return fail(OR_ERR_ARG, "shell thickness must be positive");
-    for (int k = 0; k < 4; ++k) {
-      const int32_t n = conn[4 * e + k];
-      if (n < 0 || n >= m->nn) return fail(OR_ERR_ARG, "connectivity node id out of range");
-      soa[(size_t)(k * ne + e)] = n;
-    }
+    for (int k = 0; k < 4; ++k)
+      if (conn[4 * e + k] < 0 || conn[4 * e + k] >= m->nn) return fail(OR_ERR_ARG, "connectivity node id out of range");
   }
+  if (m->have_q4 && m->have_sky) {
+    std::vector<int32_t> cur((size_t)(4 * ne));
+    OR_HIP(hipMemcpy(cur.data(), m->qconn, sizeof(int32_t) * 4 * ne, hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < ne; ++s)
+      for (int k = 0; k < 4; ++k)
+        if (cur[(size_t)(k * ne + s)] != conn[4 * (int64_t)m->qperm[(size_t)s] + k])
+          return fail(OR_ERR_ARG, "shell connectivity differs from the one the skyline was uploaded for");
+  }
+  int rc = choose_order(m, 4, ne, conn, m->qperm);
+  if (rc) return rc;
+  const std::vector<int32_t> &perm = m->qperm;
+  std::vector<int32_t> soa((size_t)(4 * ne));
+  for (int64_t s = 0; s < ne; ++s)
+    for (int k = 0; k < 4; ++k) soa[(size_t)(k * ne + s)] = conn[4 * (int64_t)perm[(size_t)s] + k];
   OR_HIP(hipMemcpyAsync(m->qconn, soa.data(), sizeof(int32_t) * 4 * ne, hipMemcpyHostToDevice, m->stream));
   OR_HIP(quiesce(m));
   const int64_t np = m->npt;
-  int rc;
-  if ((rc = put_state(m, m->qthke, thke, ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->qoffg, offg, ne, 1.0))) return rc;
-  if ((rc = put_state(m, m->qthk, thk ? thk : thke, ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->qforc, forc, 5 * ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->qmom, mom, 3 * ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->qgstr, gstr, 8 * ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->qhourg, hourg, 5 * ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->qeint, eint, 2 * ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->qepsd, epsd, ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->qsig, sig, np * 5 * ne, 0.0))) return rc;
-  if ((rc = put_state(m, m->qpla, pla, np * ne, 0.0))) return rc;
-  if (m->qepsdip && (rc = put_state(m, m->qepsdip, epsd_ip, np * ne, 0.0))) return rc;
+  if ((rc = put_state(m, m->qthke, thke, ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->qoffg, offg, ne, 1.0, perm))) return rc;
+  if ((rc = put_state(m, m->qthk, thk ? thk : thke, ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->qforc, forc, 5 * ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->qmom, mom, 3 * ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->qgstr, gstr, 8 * ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->qhourg, hourg, 5 * ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->qeint, eint, 2 * ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->qepsd, epsd, ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->qsig, sig, np * 5 * ne, 0.0, perm))) return rc;
+  if ((rc = put_state(m, m->qpla, pla, np * ne, 0.0, perm))) return rc;
+  if (m->qepsdip && (rc = put_state(m, m->qepsdip, epsd_ip, np * ne, 0.0, perm))) return rc;
   m->have_q4 = true;
   return OR_OK;
 }
@@ -1385,57 +1527,64 @@ int or_model_upload_skyline(or_model *m, const int64_t *adsky, const int64_t *ia
     for (int64_t k = adsky[n]; k < adsky[n + 1]; ++k)
       if ((k - adsky[n] < nh[(size_t)n]) != (fam[(size_t)k] == 0))
         return fail(OR_ERR_ARG, "skyline rows of a node must list solid rows before shell rows (FILLCNE order)");
-  // sliced-ELL slots: slice b of kBlock nodes gets max(rows of its nodes) slots per family
-  const int64_t nslice = (nn + kBlock - 1) / kBlock;
-  std::vector<int32_t> slab_h((size_t)nslice, 0), slab_q((size_t)nslice, 0);
-  int64_t th = 0, tq = 0;
-  for (int64_t b = 0; b < nslice; ++b) {
-    int32_t mh = 0, mq = 0;
-    for (int64_t n = b * kBlock; n < std::min(nn, (b + 1) * kBlock); ++n) {
-      mh = std::max(mh, nh[(size_t)n]);
-      mq = std::max(mq, nq[(size_t)n]);
-    }
-    slab_h[(size_t)b] = (int32_t)th;
-    slab_q[(size_t)b] = (int32_t)tq;
-    th += mh;
-    tq += mq;
-  }
-  if (th * kBlock >= (int64_t(1) << 31) || tq * kBlock >= (int64_t(1) << 31))
-    return fail(OR_ERR_ARG, "skyline too large for 32-bit row positions");
-  auto position = [&](int64_t K) -> int32_t {
-    const int32_t n = owner[(size_t)K];
-    const int64_t j = fam[(size_t)K] == 0 ? K - adsky[n] : K - adsky[n] - nh[(size_t)n];
-    const int64_t slot = (fam[(size_t)K] == 0 ? slab_h : slab_q)[(size_t)(n / kBlock)] + j;
-    return (int32_t)(slot * kBlock + n % kBlock);
-  };
-  // (re)allocate the rows
-  const size_t ss = ssize(m);
-  if (m->skh) { (void)hipFree(m->skh); m->skh = nullptr; }
-  if (m->skq) { (void)hipFree(m->skq); m->skq = nullptr; }
+  // element and corner of every row
+  std::vector<int32_t> relem((size_t)lsky);
+  std::vector<int8_t> rcorner((size_t)lsky);
+  for (int64_t e = 0; e < neh; ++e)
+    for (int k = 0; k < 8; ++k) { relem[(size_t)iads[8 * e + k]] = (int32_t)e; rcorner[(size_t)iads[8 * e + k]] = (int8_t)k; }
+  for (int64_t e = 0; e < neq; ++e)
+    for (int k = 0; k < 4; ++k) { relem[(size_t)iadc[4 * e + k]] = (int32_t)e; rcorner[(size_t)iadc[4 * e + k]] = (int8_t)k; }
+  // the storage orders (chosen here if no element upload has chosen them yet)
+  if (neh > 0 && m->hperm.empty()) { m->hperm.resize((size_t)neh); std::iota(m->hperm.begin(), m->hperm.end(), 0); }
+  if (neq > 0 && m->qperm.empty()) { m->qperm.resize((size_t)neq); std::iota(m->qperm.begin(), m->qperm.end(), 0); }
+  // release the previous plan's arrays
+  for (void **q : {&m->rowh, &m->rowq, (void **)&m->hfr_node, (void **)&m->hfr_src, (void **)&m->qfr_node,
+                   (void **)&m->qfr_src})
+    if (*q) { (void)hipFree(*q); *q = nullptr; }
   m->bytes -= m->sky_bytes;
   m->sky_bytes = 0;
-  if (neh > 0) {
-    const size_t nb = ss * (size_t)(th * kHexRowWords * kBlock);
-    OR_HIP(hipMalloc(&m->skh, nb ? nb : ss));
-    m->sky_bytes += (int64_t)nb;
-    std::vector<int32_t> soa((size_t)(8 * neh));
-    for (int64_t e = 0; e < neh; ++e)
-      for (int k = 0; k < 8; ++k) soa[(size_t)(k * neh + e)] = position(iads[8 * e + k]);
-    OR_HIP(hipMemcpyAsync(m->deg_h, nh.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, m->stream));
-    OR_HIP(hipMemcpyAsync(m->slab_h, slab_h.data(), sizeof(int32_t) * nslice, hipMemcpyHostToDevice, m->stream));
-    OR_HIP(hipMemcpyAsync(m->hiads, soa.data(), sizeof(int32_t) * 8 * neh, hipMemcpyHostToDevice, m->stream));
-    OR_HIP(quiesce(m));
-  }
-  if (neq > 0) {
-    const size_t nb = ss * (size_t)(tq * kQ4RowWords * kBlock);
-    OR_HIP(hipMalloc(&m->skq, nb ? nb : ss));
-    m->sky_bytes += (int64_t)nb;
-    std::vector<int32_t> soa((size_t)(4 * neq));
-    for (int64_t e = 0; e < neq; ++e)
-      for (int k = 0; k < 4; ++k) soa[(size_t)(k * neq + e)] = position(iadc[4 * e + k]);
-    OR_HIP(hipMemcpyAsync(m->deg_q, nq.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, m->stream));
-    OR_HIP(hipMemcpyAsync(m->slab_q, slab_q.data(), sizeof(int32_t) * nslice, hipMemcpyHostToDevice, m->stream));
-    OR_HIP(hipMemcpyAsync(m->qiadc, soa.data(), sizeof(int32_t) * 4 * neq, hipMemcpyHostToDevice, m->stream));
+  const size_t ss = ssize(m);
+  for (int f = 0; f < 2; ++f) {
+    const int64_t ne = f == 0 ? neh : neq;
+    if (ne == 0) continue;
+    const int nk = f == 0 ? 8 : 4, W = f == 0 ? kHexRowWords : kQ4RowWords;
+    // the family's rows of every node, in the node's row order
+    std::vector<int64_t> rptr((size_t)nn + 1, 0);
+    std::vector<int32_t> fe;
+    std::vector<int8_t> fk;
+    std::vector<char> ok((size_t)nn);
+    fe.reserve((size_t)(nk * ne));
+    fk.reserve((size_t)(nk * ne));
+    for (int64_t n = 0; n < nn; ++n) {
+      const int64_t k0 = f == 0 ? adsky[n] : adsky[n] + nh[(size_t)n];
+      const int64_t k1 = f == 0 ? adsky[n] + nh[(size_t)n] : adsky[n + 1];
+      for (int64_t K = k0; K < k1; ++K) { fe.push_back(relem[(size_t)K]); fk.push_back(rcorner[(size_t)K]); }
+      rptr[(size_t)n + 1] = (int64_t)fe.size();
+      ok[(size_t)n] = (f == 0 || nh[(size_t)n] == 0) ? 1 : 0;
+    }
+    orasm::FamilyPlan plan;
+    orasm::plan_family(plan, nk, ne, nn, f == 0 ? m->hperm : m->qperm, rptr, fe, fk, ok);
+    const int64_t nfr = (int64_t)plan.fr_node.size();
+    void **rows = f == 0 ? &m->rowh : &m->rowq;
+    int32_t **frn = f == 0 ? &m->hfr_node : &m->qfr_node;
+    uint16_t **frs = f == 0 ? &m->hfr_src : &m->qfr_src;
+    const size_t nrow = ss * (size_t)(plan.nrest * W);
+    OR_HIP(hipMalloc(rows, nrow ? nrow : ss));
+    OR_HIP(hipMalloc((void **)frn, sizeof(int32_t) * (size_t)std::max<int64_t>(nfr, 1)));
+    OR_HIP(hipMalloc((void **)frs, sizeof(uint16_t) * orasm::kRunCap * (size_t)std::max<int64_t>(nfr, 1)));
+    m->sky_bytes += (int64_t)(nrow + (4 + 2 * orasm::kRunCap) * (size_t)nfr);
+    const std::vector<int32_t> &deg = f == 0 ? nh : nq;
+    hipStream_t st = m->stream;
+    OR_HIP(hipMemcpyAsync(f == 0 ? m->deg_h : m->deg_q, deg.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, st));
+    OR_HIP(hipMemcpyAsync(f == 0 ? m->rs_h : m->rs_q, plan.rs.data(), sizeof(int32_t) * (nn + 1), hipMemcpyHostToDevice, st));
+    OR_HIP(hipMemcpyAsync(f == 0 ? m->hfr_start : m->qfr_start, plan.fr_start.data(),
+                          sizeof(int32_t) * plan.fr_start.size(), hipMemcpyHostToDevice, st));
+    if (nfr > 0) {
+      OR_HIP(hipMemcpyAsync(*frn, plan.fr_node.data(), sizeof(int32_t) * nfr, hipMemcpyHostToDevice, st));
+      OR_HIP(hipMemcpyAsync(*frs, plan.fr_src.data(), sizeof(uint16_t) * orasm::kRunCap * nfr, hipMemcpyHostToDevice, st));
+    }
+    OR_HIP(hipMemcpyAsync(f == 0 ? m->hrest : m->qrest, plan.rest.data(), sizeof(int32_t) * nk * ne,
+                          hipMemcpyHostToDevice, st));
     OR_HIP(quiesce(m));
   }
   m->bytes += m->sky_bytes;
@@ -1497,16 +1646,18 @@ static int flush_timing(or_model *m) {
 // Refresh the device copy of a model's kernel view after uploads or parameter changes.
 static void fill_dev(or_model *m, ModelDev &d, double tend) {
   d = ModelDev{};
-  d.hconn = m->hconn; d.hiads = m->hiads; d.hsig = m->hsig; d.heint = m->heint; d.hrho = m->hrho;
+  d.hconn = m->hconn; d.hrest = m->hrest; d.hfr_start = m->hfr_start; d.hfr_node = m->hfr_node;
+  d.hfr_src = m->hfr_src; d.hsig = m->hsig; d.heint = m->heint; d.hrho = m->hrho;
   d.hpla = m->hpla; d.hepsd = m->hepsd; d.hoffg = m->hoffg; d.hqvis = m->hqvis; d.hvol0 = m->hvol0;
   d.neh = m->neh;
-  d.qconn = m->qconn; d.qiadc = m->qiadc; d.qoffg = m->qoffg; d.qthk = m->qthk; d.qforc = m->qforc;
+  d.qconn = m->qconn; d.qrest = m->qrest; d.qfr_start = m->qfr_start; d.qfr_node = m->qfr_node;
+  d.qfr_src = m->qfr_src; d.qoffg = m->qoffg; d.qthk = m->qthk; d.qforc = m->qforc;
   d.qmom = m->qmom; d.qgstr = m->qgstr; d.qhourg = m->qhourg; d.qeint = m->qeint; d.qepsd = m->qepsd;
   d.qsig = m->qsig; d.qpla = m->qpla; d.qepsdip = m->qepsdip; d.qthke = m->qthke; d.neq = m->neq;
-  d.nd = NodePtrs{m->xv, m->vr, m->d, m->ms, m->in, m->code, m->vimp, m->deg_h, m->deg_q, m->slab_h, m->slab_q,
-                  m->nn};
-  d.skh = m->neh > 0 ? m->skh : nullptr;
-  d.skq = m->neq > 0 ? m->skq : nullptr;
+  d.nd = NodePtrs{m->xv, m->vr, m->d, m->ms, m->in, m->code, m->vimp, m->deg_h, m->deg_q, m->rs_h, m->rs_q,
+                  m->preh, m->preq, m->nn};
+  d.rowh = m->rowh; d.rowq = m->rowq;
+  d.preh = m->preh; d.preq = m->preq;
   d.ts = m->ts; d.dtmin = m->dtmin; d.flags = m->flags; d.ecrit = m->ecrit;
   d.dtmx = m->neh > 0 ? m->hc.dtmx : 1.0e20;
   d.tend = tend;
@@ -1534,31 +1685,25 @@ struct LaunchSet {
   int vh = 0, vq = 0; // element kernel variants (their block sizes match th / tq)
 };
 
+// v & 1 selects the register budget (2 or 3 waves per SIMD); the caller routes v >= 2 to
+// the unit built with the ILP-first scheduler.
 template <typename R, typename S>
 static void launch_hex(int v, hipStream_t st, const LaunchSet &L, int parity) {
-  constexpr int W0 = variant_waves<R>(0), W1 = variant_waves<R>(2);
+  constexpr int W0 = variant_waves<R>(0), W1 = variant_waves<R>(1);
   const dim3 g((unsigned)L.nth);
-  switch (v) {
-  case 0: hipLaunchKernelGGL((k_hex_forces<R, S, 256, W0>), g, dim3(256), 0, st, L.md, L.th, parity); break;
-  case 1: hipLaunchKernelGGL((k_hex_forces<R, S, 64, W0>), g, dim3(64), 0, st, L.md, L.th, parity); break;
-  case 2: hipLaunchKernelGGL((k_hex_forces<R, S, 256, W1>), g, dim3(256), 0, st, L.md, L.th, parity); break;
-  default: hipLaunchKernelGGL((k_hex_forces<R, S, 64, W1>), g, dim3(64), 0, st, L.md, L.th, parity); break;
-  }
+  if ((v & 1) == 0) hipLaunchKernelGGL((k_hex_forces<R, S, kTile, W0>), g, dim3(kTile), 0, st, L.md, L.th, parity);
+  else hipLaunchKernelGGL((k_hex_forces<R, S, kTile, W1>), g, dim3(kTile), 0, st, L.md, L.th, parity);
 }
 
 template <typename R, typename S>
 static void launch_q4(int v, hipStream_t st, const LaunchSet &L, int parity) {
-  constexpr int W0 = variant_waves<R>(0), W1 = variant_waves<R>(2);
+  constexpr int W0 = variant_waves<R>(0), W1 = variant_waves<R>(1);
   const dim3 g((unsigned)L.ntq);
-  switch (v) {
-  case 0: hipLaunchKernelGGL((k_q4_forces<R, S, 256, W0>), g, dim3(256), 0, st, L.md, L.tq, parity); break;
-  case 1: hipLaunchKernelGGL((k_q4_forces<R, S, 64, W0>), g, dim3(64), 0, st, L.md, L.tq, parity); break;
-  case 2: hipLaunchKernelGGL((k_q4_forces<R, S, 256, W1>), g, dim3(256), 0, st, L.md, L.tq, parity); break;
-  default: hipLaunchKernelGGL((k_q4_forces<R, S, 64, W1>), g, dim3(64), 0, st, L.md, L.tq, parity); break;
-  }
+  if ((v & 1) == 0) hipLaunchKernelGGL((k_q4_forces<R, S, kTile, W0>), g, dim3(kTile), 0, st, L.md, L.tq, parity);
+  else hipLaunchKernelGGL((k_q4_forces<R, S, kTile, W1>), g, dim3(kTile), 0, st, L.md, L.tq, parity);
 }
 
-// Element launches of variants 4..7 (variant - 4 in the unit built with the ILP-first
+// Element launches of variants 2..3 (variant - 2 in the unit built with the ILP-first
 // scheduler): family 0 hexes, 1 shells; fp64 selects <double,double> or <float,float>.
 void or_ilp_elements(int family, int fp64, int variant, hipStream_t st, const void *md, const void *tiles,
                      int32_t ntiles, int parity)
@@ -1584,13 +1729,13 @@ static int launch_cycle(hipStream_t st, const LaunchSet &L, int parity, hipEvent
                         int output = 0) {
   constexpr int fp64 = sizeof(R) == 8 ? 1 : 0;
   if (L.nth > 0) {
-    if (L.vh >= 4) or_ilp_elements(0, fp64, L.vh - 4, st, L.md, L.th, L.nth, parity);
+    if (L.vh >= 2) or_ilp_elements(0, fp64, L.vh - 2, st, L.md, L.th, L.nth, parity);
     else launch_hex<R, S>(L.vh, st, L, parity);
     OR_HIP(hipGetLastError());
   }
   if (after_hex) OR_HIP(hipEventRecord(after_hex, st));
   if (L.ntq > 0) {
-    if (L.vq >= 4) or_ilp_elements(1, fp64, L.vq - 4, st, L.md, L.tq, L.ntq, parity);
+    if (L.vq >= 2) or_ilp_elements(1, fp64, L.vq - 2, st, L.md, L.tq, L.ntq, parity);
     else launch_q4<R, S>(L.vq, st, L, parity);
     OR_HIP(hipGetLastError());
   }
@@ -1737,13 +1882,13 @@ int or_model_download_hex(or_model *m, double *sig, double *eint, double *rho, d
   OR_HIP(quiesce(m));
   const int64_t ne = m->neh;
   int rc;
-  if ((rc = get_state(m, sig, m->hsig, 6 * ne))) return rc;
-  if ((rc = get_state(m, eint, m->heint, ne))) return rc;
-  if ((rc = get_state(m, rho, m->hrho, ne))) return rc;
-  if ((rc = get_state(m, pla, m->hpla, ne))) return rc;
-  if ((rc = get_state(m, epsd, m->hepsd, ne))) return rc;
-  if ((rc = get_state(m, offg, m->hoffg, ne))) return rc;
-  if ((rc = get_state(m, qvis, m->hqvis, ne))) return rc;
+  if ((rc = get_state(m, sig, m->hsig, 6 * ne, m->hperm))) return rc;
+  if ((rc = get_state(m, eint, m->heint, ne, m->hperm))) return rc;
+  if ((rc = get_state(m, rho, m->hrho, ne, m->hperm))) return rc;
+  if ((rc = get_state(m, pla, m->hpla, ne, m->hperm))) return rc;
+  if ((rc = get_state(m, epsd, m->hepsd, ne, m->hperm))) return rc;
+  if ((rc = get_state(m, offg, m->hoffg, ne, m->hperm))) return rc;
+  if ((rc = get_state(m, qvis, m->hqvis, ne, m->hperm))) return rc;
   return OR_OK;
 }
 
@@ -1754,23 +1899,23 @@ int or_model_download_q4(or_model *m, double *offg, double *thk, double *forc, d
   OR_HIP(quiesce(m));
   const int64_t ne = m->neq, np = m->npt;
   int rc;
-  if ((rc = get_state(m, offg, m->qoffg, ne))) return rc;
-  if ((rc = get_state(m, thk, m->qthk, ne))) return rc;
-  if ((rc = get_state(m, forc, m->qforc, 5 * ne))) return rc;
-  if ((rc = get_state(m, mom, m->qmom, 3 * ne))) return rc;
-  if ((rc = get_state(m, gstr, m->qgstr, 8 * ne))) return rc;
-  if ((rc = get_state(m, hourg, m->qhourg, 5 * ne))) return rc;
-  if ((rc = get_state(m, eint, m->qeint, 2 * ne))) return rc;
-  if ((rc = get_state(m, epsd, m->qepsd, ne))) return rc;
-  if ((rc = get_state(m, sig, m->qsig, np * 5 * ne))) return rc;
-  if ((rc = get_state(m, pla, m->qpla, np * ne))) return rc;
+  if ((rc = get_state(m, offg, m->qoffg, ne, m->qperm))) return rc;
+  if ((rc = get_state(m, thk, m->qthk, ne, m->qperm))) return rc;
+  if ((rc = get_state(m, forc, m->qforc, 5 * ne, m->qperm))) return rc;
+  if ((rc = get_state(m, mom, m->qmom, 3 * ne, m->qperm))) return rc;
+  if ((rc = get_state(m, gstr, m->qgstr, 8 * ne, m->qperm))) return rc;
+  if ((rc = get_state(m, hourg, m->qhourg, 5 * ne, m->qperm))) return rc;
+  if ((rc = get_state(m, eint, m->qeint, 2 * ne, m->qperm))) return rc;
+  if ((rc = get_state(m, epsd, m->qepsd, ne, m->qperm))) return rc;
+  if ((rc = get_state(m, sig, m->qsig, np * 5 * ne, m->qperm))) return rc;
+  if ((rc = get_state(m, pla, m->qpla, np * ne, m->qperm))) return rc;
   if (epsd_ip) {
     if (m->qepsdip) {
-      if ((rc = get_state(m, epsd_ip, m->qepsdip, np * ne))) return rc;
+      if ((rc = get_state(m, epsd_ip, m->qepsdip, np * ne, m->qperm))) return rc;
     } else {
       // ASRATE = 1: LBUF%EPSD = 1*EPSD_PG + 0*LBUF%EPSD equals GBUF%EPSD at every point
       std::vector<double> g((size_t)ne);
-      if ((rc = get_state(m, g.data(), m->qepsd, ne))) return rc;
+      if ((rc = get_state(m, g.data(), m->qepsd, ne, m->qperm))) return rc;
       for (int64_t ip = 0; ip < np; ++ip) std::memcpy(epsd_ip + ip * ne, g.data(), sizeof(double) * ne);
     }
   }
@@ -1821,27 +1966,27 @@ int or_model_footprint_estimate(int64_t nn, int64_t ne_hex, int64_t ne_q4, int32
   if ((ne_hex > 0 && !rows_hex) || (ne_q4 > 0 && !rows_q4)) return fail(OR_ERR_ARG, "missing row counts");
   const int64_t ss = precision == OR_PREC_FP64 ? 8 : 4;
   const int64_t nsl = (nn + kBlock - 1) / kBlock;
+  const int64_t th = (ne_hex + kTile - 1) / kTile, tq = (ne_q4 + kTile - 1) / kTile;
   // the allocations of or_model_create, in the same order
-  int64_t b = 8 * (6 * nn + 3 * nn + nn + 3 * nn) + 4 * nn;           // xv, d, ms, vimp, code
-  if (ne_q4 > 0) b += 8 * (4 * nn + nn);                               // vr, in
-  if (ne_hex > 0) b += 4 * (nn + nsl + 16 * ne_hex) + ss * 13 * ne_hex; // deg, slab, conn, iads, state
+  int64_t b = 8 * (6 * nn + 3 * nn + nn + 3 * nn) + 4 * nn;             // xv, d, ms, vimp, code
+  if (ne_q4 > 0) b += 8 * (4 * nn + nn);                                 // vr, in
+  if (ne_hex > 0)                                                        // deg, rs, prefix, tiles, conn, rest, state
+    b += 4 * (nn + nn + 1) + 8 * 3 * nn + 4 * (th + 1) + 4 * 16 * ne_hex + ss * 13 * ne_hex;
   if (ne_q4 > 0) {
-    b += 4 * (nn + nsl + 8 * ne_q4) + ss * (4 + 5 + 3 + 8 + 5 + 2) * ne_q4 + ss * 6 * (int64_t)npt * ne_q4;
-    if (strain_rate_filter) b += ss * (int64_t)npt * ne_q4;            // LBUF EPSD (or_model_set_q4_params)
+    b += 4 * (nn + nn + 1) + 8 * 6 * nn + 4 * (tq + 1) + 4 * 8 * ne_q4 + ss * (4 + 5 + 3 + 8 + 5 + 2) * ne_q4 +
+         ss * 6 * (int64_t)npt * ne_q4;
+    if (strain_rate_filter) b += ss * (int64_t)npt * ne_q4;              // LBUF EPSD (or_model_set_q4_params)
   }
   b += (int64_t)sizeof(DevTime) * 2 + 8 * 2 + 4 + 8 * (3 + 3 * 1024) + (int64_t)sizeof(ModelDev);
-  b += (int64_t)sizeof(Tile) * ((ne_hex + 63) / 64 + (ne_q4 + 63) / 64 + nsl); // element tiles sized for 64
-  // skyline rows in sliced-ELL form (or_model_upload_skyline): per slice, its largest row count
+  b += (int64_t)sizeof(Tile) * (th + tq + nsl);
+  // or_model_upload_skyline: rest rows (at most every row of the family) and the prefix lists
+  // (at most one prefix per node and family)
   for (int f = 0; f < 2; ++f) {
     const int32_t *rows = f == 0 ? rows_hex : rows_q4;
     if ((f == 0 ? ne_hex : ne_q4) == 0) continue;
-    int64_t slots = 0;
-    for (int64_t s0 = 0; s0 < nn; s0 += kBlock) {
-      int32_t mx = 0;
-      for (int64_t n = s0; n < std::min(nn, s0 + kBlock); ++n) mx = std::max(mx, rows[n]);
-      slots += mx;
-    }
-    b += ss * slots * kBlock * (f == 0 ? kHexRowWords : kQ4RowWords);
+    int64_t nrow = 0, nodes = 0;
+    for (int64_t n = 0; n < nn; ++n) { nrow += rows[n]; nodes += rows[n] > 0; }
+    b += ss * nrow * (f == 0 ? kHexRowWords : kQ4RowWords) + (4 + 2 * orasm::kRunCap) * nodes;
   }
   *bytes = b;
   return OR_OK;
@@ -2048,7 +2193,7 @@ int batch_rebuild(or_batch *b) {
     if (!b->live[(size_t)i]) continue;
     const int c = m->prec == OR_PREC_FP64 ? 0 : 1;
     const int64_t cnt[3] = {m->neh, m->neq, m->nn};
-    const int step[3] = {variant_block(b->var[c][0]), variant_block(b->var[c][1]), kBlock};
+    const int step[3] = {kTile, kTile, kBlock};
     for (int f = 0; f < 3; ++f)
       for (int64_t first = 0; first < cnt[f]; first += step[f]) tl[c][f].push_back(Tile{(int32_t)i, (int32_t)first});
   }

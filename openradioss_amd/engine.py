@@ -211,7 +211,7 @@ class GpuModel:
         return cls.restore(model, ck, precision=precision, device=device)
 
     def set_variant(self, family: str, variant: int):
-        """Launch variant (0..7) of the 'hex' or 'q4' element kernel; results do not depend on it."""
+        """Launch variant (0..3) of the 'hex' or 'q4' element kernel; results do not depend on it."""
         L.check(self._lib.or_model_set_variant(self._h, {"hex": 0, "q4": 1}[family], int(variant)))
 
     def autotune(self, cycles_per_variant: int = 5) -> tuple[int, int]:

@@ -25,7 +25,9 @@ def test_served_results_equal_alone_runs(gpu_lib):
         assert r.result["tt"] >= r.t_end
         g = E.GpuModel(r.model, precision=r.precision)
         fp = g.footprint()
-        assert fp == S.footprint(r.model, r.precision)
+        # the estimate bounds the rest rows by every row of the model (the tiles sum most of
+        # them in LDS), so it is an upper bound of the allocation
+        assert fp <= S.footprint(r.model, r.precision) <= 2 * fp
         g.run(r.ncycle)
         x, v, vr, d = g.nodes6()
         assert np.array_equal(d, r.result["d"]) and np.array_equal(v, r.result["v"])

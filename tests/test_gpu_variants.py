@@ -1,4 +1,4 @@
-"""Launch variants of the element kernels (workgroup size, register budget, instruction scheduler)
+"""Launch variants of the element kernels (register budget, instruction scheduler)
 and their autotuners:
 every variant computes bit-identical results, and tuned runs equal untuned ones."""
 import numpy as np
@@ -14,14 +14,14 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("precision", ["fp64", "split"])
 def test_every_variant_gives_the_same_bits(precision, gpu_lib):
-    print(f"case: mixed_c120, 8 variants per element kernel, {precision}", flush=True)
+    print(f"case: mixed_c120, 4 variants per element kernel, {precision}", flush=True)
     c = Q4_CASES["mixed_c120"]
     m = build_q4_case(c)
     ref = None
-    for v in range(8):
+    for v in range(4):
         g = E.GpuModel(m, precision=precision)
         g.set_variant("hex", v)
-        g.set_variant("q4", 7 - v)
+        g.set_variant("q4", 3 - v)
         g.run(c["cycles"])
         out = g.nodes6() + (g.q4_state()["sig"], g.hex_state()["sig"])
         if ref is None:
@@ -40,11 +40,11 @@ def test_model_autotune_keeps_results(gpu_lib):
     print("case: autotune on the 8x20 tube", flush=True)
     m = M.square_tube(width=8.0, length=20.0)
     g = E.GpuModel(m)
-    chosen = g.autotune(2)                 # 8 variants x 2 cycles of the shell kernel
-    assert 0 <= chosen[1] < 8
+    chosen = g.autotune(2)                 # 4 variants x 2 cycles of the shell kernel
+    assert 0 <= chosen[1] < 4
     g.run(12)
     n = g.time_state().ncycle
-    assert n == 8 * 2 + 12
+    assert n == 4 * 2 + 12
     o = O.ModelOracle(m, nthreads=8, math="cr")
     o.cycle(n)
     assert np.array_equal(g.nodes6()[3], o.d)
@@ -61,7 +61,7 @@ def test_batch_autotune_keeps_results(gpu_lib):
         b.add(g, 1.0)
         gms.append(g)
     chosen = b.autotune(2)
-    assert all(0 <= v < 8 for v in chosen)
+    assert all(0 <= v < 4 for v in chosen)
     b.run(10)
     for g, (m, _), k in zip(gms, sw, range(4)):
         n = g.time_state().ncycle
