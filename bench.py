@@ -232,11 +232,14 @@ def serve_leg(nreq: int, rate: float, time_scale: float):
     reqs = S.request_stream(nreq, rate=rate, seed=SEED, time_scale=time_scale)
     free, total = L.device_memory(0)
     budget = int(0.8 * free)
-    srv = S.Server(hbm_budget=budget, cycles_per_step=50, keep_results=False)
+    srv = S.Server(hbm_budget=budget, cycles_per_step=200, keep_results=False)
     st = srv.serve(reqs)
     srv.close()
-    st.update(rate_per_s=rate, time_scale=time_scale, hbm_budget_gb=budget / 1e9, max_used_gb=srv.max_used / 1e9,
-              elements=[min(r.ne for r in reqs), max(r.ne for r in reqs)])
+    st.update(rate_per_s=rate, seed=SEED, time_scale=time_scale, hbm_budget_gb=budget / 1e9,
+              max_used_gb=srv.max_used / 1e9, elements=[min(r.ne for r in reqs), max(r.ne for r in reqs)],
+              trace=[dict(arrival_s=round(r.arrival, 4), kind=r.kind, elements=r.ne, precision=r.precision,
+                          t_end_ms=round(1e3 * r.t_end, 4), cycles=r.ncycle, latency_s=round(r.latency, 4))
+                     for r in reqs])
     return st
 
 
@@ -416,10 +419,13 @@ def main():
             log(f"sweep {prec}: {r['element_updates_per_s']:.3e} element-updates/s over {r['models']} tubes "
                 f"({r['shells']} shells), {r['ms_per_cycle']:.3f} ms per batch cycle")
     if not args.no_serve:
-        r = serve_leg(32, rate=40.0, time_scale=0.02)
-        out["serving"] = r
-        log(f"serve: {r['requests']} requests, p50 {r['p50_s']:.3f} s, p99 {r['p99_s']:.3f} s, "
-            f"{r['element_updates_per_s']:.3e} element-updates/s")
+        # the served workload at its full end times (1-10 ms), at three arrival rates
+        out["serving"] = []
+        for rate in (2.0, 8.0, 32.0):
+            r = serve_leg(16, rate=rate, time_scale=1.0)
+            out["serving"].append(r)
+            log(f"serve at {rate:g}/s: {r['requests']} requests, p50 {r['p50_s']:.3f} s, p99 {r['p99_s']:.3f} s, "
+                f"{r['element_updates_per_s']:.3e} element-updates/s")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(model)
         log(f"cpu: {out['cpu_baseline']['value']:.3e} element-updates/s")

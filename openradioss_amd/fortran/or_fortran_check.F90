@@ -11,6 +11,15 @@
 !        QVIS (NEL each); DT1.
 ! <out>: int32 status, MSTOP; DT2T; FSKY(8,LSKY); SIG(NEL*6), EINT, RHO, PLA, EPSD,
 !        OFF, QVIS.
+!
+! With <mode> <in> <out>, the other calls an Engine makes (layouts in the subroutines below):
+!   q4group  FORINTC's call of one CFORC3 group: the group's IPARG column, PM and GEO words and
+!            the law's parameters go through OR_GPU_ELIGIBLE_SHELL and OR_PACK_Q4_PARAMS, and
+!            the group through OR_CFORC3_GROUP;
+!   hexgroup the same for one SFORC3 group (OR_GPU_ELIGIBLE_SOLID, OR_PACK_HEX_PARAMS,
+!            OR_SFORC3_GROUP);
+!   nodal    RESOL's nodal half of a cycle through OR_NODAL_UPDATE;
+!   model    a resident model: create, upload, OR_MODEL_RUN, download.
 program or_fortran_check
   use, intrinsic :: iso_c_binding
   use or_gpu_mod
@@ -22,23 +31,38 @@ program or_fortran_check
   integer(c_int32_t), allocatable :: ixs(:, :), iads(:, :)
   real(c_double), allocatable :: sig(:), eint(:), rho(:), vol(:), pla(:), epsd(:), off(:), qvis(:)
   real(c_double) :: dt1, dt2t
-  character(len=4096) :: fin, fout
+  character(len=4096) :: fin, fout, mode
   integer :: u
 
   if (command_argument_count() < 2) then
     ierr = or_device_count(ndev)
     write (*, '(a,i0)') 'or_version ', or_version()
     write (*, '(a,i0)') 'devices ', ndev
-  else
+  else if (command_argument_count() == 2) then
+    call get_command_argument(1, fin)
+    call get_command_argument(2, fout)
     call run_group()
+  else
+    call get_command_argument(1, mode)
+    call get_command_argument(2, fin)
+    call get_command_argument(3, fout)
+    select case (trim(mode))
+    case ('q4group')
+      call run_q4group()
+    case ('hexgroup')
+      call run_hexgroup()
+    case ('nodal')
+      call run_nodal()
+    case ('model')
+      call run_model()
+    case default
+      write (*, '(2a)') 'unknown mode ', trim(mode)
+    end select
   end if
 
 contains
 
   subroutine run_group()
-    call get_command_argument(1, fin)
-    call get_command_argument(2, fout)
-
     open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', action='read')
     read (u) numnod, nel, lsky
     read (u) hp
@@ -58,4 +82,155 @@ contains
     write (u) int(ierr, c_int32_t), mstop, dt2t, fsky, sig, eint, rho, pla, epsd, off, qvis
     close (u)
   end subroutine run_group
+
+  ! <in>:  int32 NUMNOD, NEL, LSKY; IPARG(100) int32, IPARAM(4) int32; mat RHO, BULK, SHEAR,
+  !        UPARAM(10); GEO(40); DTFAC1(1), DTMX; X, V (3,NUMNOD); IXS(11,NEL), IADS(8,NEL) int32;
+  !        SIG(NEL*6), EINT, RHO, VOL, PLA, EPSD, OFF, QVIS; DT1.
+  ! <out>: int32 status, eligible, MSTOP; DT2T; FSKY(8,LSKY); SIG, EINT, RHO, PLA, EPSD, OFF, QVIS.
+  subroutine run_hexgroup()
+    integer(c_int32_t) :: iparg(100), iparam(4), ok
+    real(c_double) :: mrho, mbulk, mshear, uparam(10), geo(40), dtfac, dtmx
+    open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', action='read')
+    read (u) numnod, nel, lsky, iparg, iparam, mrho, mbulk, mshear, uparam, geo, dtfac, dtmx
+    allocate (x(3, numnod), v(3, numnod), ixs(11, nel), iads(8, nel), fsky(8, lsky))
+    allocate (sig(6*nel), eint(nel), rho(nel), vol(nel), pla(nel), epsd(nel), off(nel), qvis(nel))
+    read (u) x, v, ixs, iads, sig, eint, rho, vol, pla, epsd, off, qvis, dt1
+    close (u)
+    fsky = 0.0d0
+    dt2t = 1.0d6
+    mstop = 0
+    ierr = OR_ERR_ARG
+    ok = 0
+    if (or_gpu_eligible_solid(iparg, iparam, uparam(8))) then
+      ok = 1
+      call or_pack_hex_params(mrho, mbulk, mshear, uparam, iparam, geo, dtfac, dtmx, hp)
+      ierr = or_sforc3_group(0_c_int, hp, int(numnod, c_int64_t), nel, int(iparg(3), c_int64_t), x, v, ixs, &
+                             sig, eint, rho, vol, pla, epsd, off, qvis, int(lsky, c_int64_t), fsky, iads, dt1, &
+                             dt2t, mstop)
+      if (ierr /= OR_OK) write (*, '(a,i0,2a)') 'or_sforc3_group status ', ierr, ': ', or_error_message()
+    end if
+    open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
+    write (u) int(ierr, c_int32_t), ok, mstop, dt2t, fsky, sig, eint, rho, pla, epsd, off, qvis
+    close (u)
+  end subroutine run_hexgroup
+
+  ! <in>:  int32 NUMNOD, NEL, LSKY; IPARG(100), IPARAM(4) int32; PM(30), UPARAM(10), GEO(40),
+  !        DTFAC1(3), DTMIN1(3); X, V, VR (3,NUMNOD); IXC(7,NEL), IADC(4,NEL) int32; GBUF OFF, THK,
+  !        THKE (NEL), FOR(NEL*5), MOM(NEL*3), STRA(NEL*8), HOURG(NEL*5), EINT(NEL*2), EPSD(NEL);
+  !        LBUF SIG(NPT*5*NEL), PLA(NPT*NEL), EPSD(NPT*NEL) with NPT = IPARG(6); DT1.
+  ! <out>: int32 status, eligible; DT2T; FSKY(8,LSKY); OFF, THK, FOR, MOM, STRA, HOURG, EINT,
+  !        EPSD, SIG, PLA, EPSD per point.
+  subroutine run_q4group()
+    integer(c_int32_t) :: iparg(100), iparam(4), ok, npt
+    real(c_double) :: pm(30), uparam(10), geo(40), dtfac, dtmin
+    real(c_double), allocatable :: vr(:, :), thk(:), thke(:), forc(:), mom(:), gstr(:), hourg(:), qeint(:)
+    real(c_double), allocatable :: qepsd(:), lsig(:), lpla(:), lepsd(:)
+    integer(c_int32_t), allocatable :: ixc(:, :), iadc(:, :)
+    type(or_q4_params_t) :: qp
+    open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', action='read')
+    read (u) numnod, nel, lsky, iparg, iparam, pm, uparam, geo, dtfac, dtmin
+    npt = iparg(6)
+    allocate (x(3, numnod), v(3, numnod), vr(3, numnod), ixc(7, nel), iadc(4, nel), fsky(8, lsky))
+    allocate (off(nel), thk(nel), thke(nel), forc(5*nel), mom(3*nel), gstr(8*nel), hourg(5*nel), qeint(2*nel))
+    allocate (qepsd(nel), lsig(npt*5*nel), lpla(npt*nel), lepsd(npt*nel))
+    read (u) x, v, vr, ixc, iadc, off, thk, thke, forc, mom, gstr, hourg, qeint, qepsd, lsig, lpla, lepsd, dt1
+    close (u)
+    fsky = 0.0d0
+    dt2t = 1.0d6
+    ierr = OR_ERR_ARG
+    ok = 0
+    if (or_gpu_eligible_shell(iparg, iparam)) then
+      ok = 1
+      call or_pack_q4_params(pm, uparam, iparam, geo, iparg, dtfac, dtmin, qp)
+      ierr = or_cforc3_group(0_c_int, qp, int(numnod, c_int64_t), nel, int(iparg(3), c_int64_t), x, v, vr, ixc, &
+                             off, thk, thke, forc, mom, gstr, hourg, qeint, qepsd, lsig, lpla, lepsd, &
+                             int(lsky, c_int64_t), fsky, iadc, dt1, dt2t)
+      if (ierr /= OR_OK) write (*, '(a,i0,2a)') 'or_cforc3_group status ', ierr, ': ', or_error_message()
+    end if
+    open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
+    write (u) int(ierr, c_int32_t), ok, dt2t, fsky, off, thk, forc, mom, gstr, hourg, qeint, qepsd, lsig, lpla, lepsd
+    close (u)
+  end subroutine run_q4group
+
+  ! <in>:  int64 NUMNOD, LSKY; ADSKY(NUMNOD+1) int64 (1-based); FSKY(8,LSKY); MS, IN (NUMNOD);
+  !        node codes (NUMNOD) int32; VIMP, X, V, VR, D (3,NUMNOD); DT2T, DTMX; or_time_state_t.
+  ! <out>: int32 status; X, V, VR, D (3,NUMNOD); STIFN, STIFR (NUMNOD); or_time_state_t.
+  subroutine run_nodal()
+    integer(c_int64_t) :: nn, ls
+    integer(c_int64_t), allocatable, target :: adsky(:)
+    real(c_double), allocatable, target :: fs(:, :), ms(:), in(:), vimp(:, :), xx(:, :), vv(:, :), vr(:, :), d(:, :)
+    real(c_double), allocatable, target :: stifn(:), stifr(:)
+    integer(c_int32_t), allocatable, target :: code(:)
+    real(c_double) :: dtmx
+    type(or_time_state_t) :: ts
+    open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', action='read')
+    read (u) nn, ls
+    allocate (adsky(nn + 1), fs(8, ls), ms(nn), in(nn), code(nn), vimp(3, nn), xx(3, nn), vv(3, nn), vr(3, nn))
+    allocate (d(3, nn), stifn(nn), stifr(nn))
+    read (u) adsky, fs, ms, in, code, vimp, xx, vv, vr, d, dt2t, dtmx, ts
+    close (u)
+    stifn = 0.0d0
+    stifr = 0.0d0
+    ierr = or_nodal_update(0_c_int, nn, ls, c_loc(adsky), c_loc(fs), c_loc(ms), c_loc(in), c_loc(code), &
+                           c_loc(vimp), c_loc(xx), c_loc(vv), c_loc(vr), c_loc(d), c_loc(stifn), c_loc(stifr), &
+                           dt2t, dtmx, ts)
+    if (ierr /= OR_OK) write (*, '(a,i0,2a)') 'or_nodal_update status ', ierr, ': ', or_error_message()
+    open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
+    write (u) int(ierr, c_int32_t), xx, vv, vr, d, stifn, stifr, ts
+    close (u)
+  end subroutine run_nodal
+
+  ! <in>:  int64 NN, NEH, NEQ; int32 NPT, PRECISION, NCYCLES; or_hex_params_t; or_q4_params_t;
+  !        X, V, VR (3,NN); MS, IN (NN); node codes (NN) int32; VIMP (3,NN); hex connectivity
+  !        (8,NEH) int32 0-based, VOL0 and RHO (NEH); shell connectivity (4,NEQ) int32 0-based,
+  !        THKE (NEQ); ADSKY (NN+1), IADS (8,NEH), IADC (4,NEQ) int64 0-based.
+  ! <out>: int32 status; X, V, VR, D (3,NN); or_time_state_t.
+  subroutine run_model()
+    integer(c_int64_t) :: nn, neh, neq
+    integer(c_int32_t) :: npt, prec, ncyc
+    type(or_q4_params_t) :: qp
+    type(c_ptr) :: h
+    real(c_double), allocatable, target :: xx(:, :), vv(:, :), vr(:, :), d(:, :), ms(:), in(:), vimp(:, :)
+    real(c_double), allocatable, target :: vol0(:), hrho(:), thke(:)
+    integer(c_int32_t), allocatable, target :: code(:), connh(:, :), connq(:, :)
+    integer(c_int64_t), allocatable, target :: adsky(:), iadsh(:, :), iadcq(:, :)
+    type(or_time_state_t) :: ts
+    open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', action='read')
+    read (u) nn, neh, neq, npt, prec, ncyc, hp, qp
+    allocate (xx(3, nn), vv(3, nn), vr(3, nn), d(3, nn), ms(nn), in(nn), code(nn), vimp(3, nn))
+    allocate (connh(8, max(neh, 1_c_int64_t)), vol0(max(neh, 1_c_int64_t)), hrho(max(neh, 1_c_int64_t)))
+    allocate (connq(4, max(neq, 1_c_int64_t)), thke(max(neq, 1_c_int64_t)))
+    allocate (adsky(nn + 1), iadsh(8, max(neh, 1_c_int64_t)), iadcq(4, max(neq, 1_c_int64_t)))
+    read (u) xx, vv, vr, ms, in, code, vimp
+    if (neh > 0) read (u) connh(:, 1:neh), vol0(1:neh), hrho(1:neh)
+    if (neq > 0) read (u) connq(:, 1:neq), thke(1:neq)
+    read (u) adsky
+    if (neh > 0) read (u) iadsh(:, 1:neh)
+    if (neq > 0) read (u) iadcq(:, 1:neq)
+    close (u)
+    ierr = or_model_create(h, 0_c_int, nn, neh, neq, npt, int(prec, c_int))
+    if (ierr == OR_OK .and. neh > 0) ierr = or_model_set_hex_params(h, hp)
+    if (ierr == OR_OK .and. neq > 0) ierr = or_model_set_q4_params(h, qp)
+    if (ierr == OR_OK) ierr = or_model_upload_nodes(h, c_loc(xx), c_loc(vv), c_loc(vr), c_loc(ms), c_loc(in), &
+                                                    c_loc(code), c_loc(vimp))
+    if (ierr == OR_OK .and. neh > 0) ierr = or_model_upload_hex(h, c_loc(connh), c_null_ptr, c_null_ptr, &
+                                                                c_loc(hrho), c_loc(vol0), c_null_ptr, &
+                                                                c_null_ptr, c_null_ptr, c_null_ptr)
+    if (ierr == OR_OK .and. neq > 0) ierr = or_model_upload_q4(h, c_loc(connq), c_loc(thke), c_null_ptr, &
+                                                               c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                                                               c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                                                               c_null_ptr, c_null_ptr)
+    if (ierr == OR_OK) ierr = or_model_upload_skyline(h, c_loc(adsky), c_loc(iadsh), c_loc(iadcq))
+    if (ierr == OR_OK) ierr = or_model_run(h, int(ncyc, c_int64_t))
+    if (ierr == OR_OK) ierr = or_model_sync(h)
+    if (ierr == OR_OK) ierr = or_model_download_nodes(h, c_loc(xx), c_loc(vv), c_loc(vr), c_loc(d))
+    if (ierr == OR_OK) ierr = or_model_get_time(h, ts)
+    if (ierr /= OR_OK) write (*, '(a,i0,2a)') 'resident model status ', ierr, ': ', or_error_message()
+    if (c_associated(h)) then
+      if (or_model_destroy(h) /= OR_OK) write (*, '(a)') 'or_model_destroy failed'
+    end if
+    open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
+    write (u) int(ierr, c_int32_t), xx, vv, vr, d, ts
+    close (u)
+  end subroutine run_model
 end program or_fortran_check

@@ -23,6 +23,7 @@ module or_gpu_mod
   public :: or_batch_create, or_batch_destroy, or_batch_add, or_batch_remove, or_batch_run, or_batch_sync
   public :: or_batch_active, or_model_set_variant, or_model_autotune, or_batch_autotune
   public :: or_sforc3_group, or_cforc3_group, or_nodal_update, or_math_eval
+  public :: or_gpu_eligible_solid, or_gpu_eligible_shell, or_pack_hex_params, or_pack_q4_params
 
   integer(c_int), parameter, public :: OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_HIP = 2, OR_ERR_NODEV = 3, &
                                        OR_ERR_STATE = 4, OR_ERR_MSTOP = 5
@@ -346,6 +347,117 @@ module or_gpu_mod
   end interface
 
 contains
+
+  ! Engine glue.  The predicates decide, per element group, whether the group runs on this
+  ! path; they are written from the path's scope (the SFORC3 / CFORC3 branches this library
+  ! implements), after the pattern of the reference's own GPU predicate (forintc.F:301-318).
+  ! Their arguments are the group's IPARG column (IPARG(NPARG,NG)), the material law's
+  ! integer parameters (mat_param%iparam(1:4): IFORM, ICC, VP, ISRATE) and the group's
+  ! property words the branches depend on.
+
+  ! An 8-node solid group of law 2 on the SFORC3 branch of this path: ITY = 1 (IPARG(5)),
+  ! MLW = 2 (IPARG(1)), group active (IPARG(8) /= 1), Isolid = 1 (JHBE = IPARG(23) = 1),
+  ! ISMSTR = 4 (IPARG(9)), Lagrangian (JLAG = IPARG(14) = 1, JALE = IPARG(7) = 0, JEUL =
+  ! IPARG(11) = 0), no failure
+  ! model (IPARG(43) = 0), no thermal coupling (IPARG(13) = 0), global frame (JCVT = IPARG(37)
+  ! = 0), and a Johnson-Cook form with isotropic hardening (IFORM = 0, FISOKIN = 0) whose
+  ! strain rate is the total or deviatoric one (VP = 2 or 3).
+  logical function or_gpu_eligible_solid(iparg, iparam, fisokin) result(ok)
+    integer, intent(in) :: iparg(*), iparam(4)
+    real(c_double), intent(in) :: fisokin
+    ok = iparg(5) == 1 .and. iparg(1) == 2 .and. iparg(8) /= 1 .and. iparg(23) == 1 .and. iparg(9) == 4 &
+         .and. iparg(14) == 1 .and. iparg(7) == 0 .and. iparg(11) == 0 .and. iparg(43) == 0 &
+         .and. iparg(13) == 0 &
+         .and. iparg(37) == 0 .and. iparam(1) == 0 .and. fisokin == 0.0d0 &
+         .and. (iparam(3) == 2 .or. iparam(3) == 3)
+  end function
+
+  ! A 4-node shell group of law 2 on the CFORC3 branch of this path: ITY = 3, MLW = 2, group
+  ! active, Ishell = 1 (IHBE = IPARG(23) = 1), 2..10 points through the thickness (NPT =
+  ! IPARG(6)), Iplas 0..2 (IPARG(29)), ISMSTR = 2 (IPARG(9)), property type 1 (IGTYP =
+  ! IPARG(38)), no failure model (IPARG(43) = 0), no sub-stack (IPARG(71) = 0), no XFEM
+  ! (IPARG(54) = 0), no seat belt (IPARG(91) = 0), and a Johnson-Cook form with the total
+  ! strain rate (IFORM = 0, VP = 2).
+  logical function or_gpu_eligible_shell(iparg, iparam) result(ok)
+    integer, intent(in) :: iparg(*), iparam(4)
+    ok = iparg(5) == 3 .and. iparg(1) == 2 .and. iparg(8) /= 1 .and. iparg(23) == 1 &
+         .and. iparg(6) >= 2 .and. iparg(6) <= 10 .and. iparg(29) >= 0 .and. iparg(29) <= 2 &
+         .and. iparg(9) == 2 .and. iparg(38) == 1 .and. iparg(43) == 0 .and. iparg(71) == 0 &
+         .and. iparg(54) == 0 .and. iparg(91) == 0 .and. iparam(1) == 0 .and. iparam(3) == 2
+  end function
+
+  ! or_hex_params_t from the values SFORC3's callees read: the law-2 material (mat_param%rho,
+  ! %bulk, %shear; %uparam(1:10) = CA, CB, CN, EPMX, SIGMX, CC, EPDR, FISOKIN, ASRATE, M_EXP
+  ! and %iparam(1:4) = IFORM, ICC, VP, ISRATE, m2law.F:135-155), the /PROP/SOLID words
+  ! GEO(13:17) = QH, QA, QB, VNS1, VNS2 (shvis3.F:167, mqviscb.F:201-204), DTFAC1(1) and DTMX.
+  subroutine or_pack_hex_params(rho, bulk, shear, uparam, iparam, geo, dtfac, dtmx, p)
+    real(c_double), intent(in) :: rho, bulk, shear, uparam(10), geo(*), dtfac, dtmx
+    integer, intent(in) :: iparam(4)
+    type(or_hex_params_t), intent(out) :: p
+    p%rho0 = rho
+    p%bulk = bulk
+    p%shear = shear
+    p%ca = uparam(1)
+    p%cb = uparam(2)
+    p%cn = uparam(3)
+    p%epmx = uparam(4)
+    p%sigmx = uparam(5)
+    p%cc = uparam(6)
+    p%epdr = uparam(7)
+    p%fisokin = uparam(8)
+    p%asrate = uparam(9)
+    p%m_exp = uparam(10)
+    p%iform = iparam(1)
+    p%icc = iparam(2)
+    p%vp = iparam(3)
+    p%israte = iparam(4)
+    p%qh = geo(13)
+    p%qa = geo(14)
+    p%qb = geo(15)
+    p%vns1 = geo(16)
+    p%vns2 = geo(17)
+    p%dtfac = dtfac
+    p%dtmx = dtmx
+  end subroutine
+
+  ! or_q4_params_t from the values CFORC3's callees read: PM(1) RHO, PM(20) YOUNG, PM(21) NU,
+  ! PM(22) G, PM(27) SSP (ccoef3.F:125-131); the law's %uparam(1:10) and %iparam(1:4) as for
+  ! solids (sigeps02c.F:91-123); the /PROP/SHELL words GEO(13:15) = Hm, Hf, Hr (chvis3),
+  ! GEO(37) = ISHEAR and GEO(38) = shear factor (ccoef3.F:161-163); the group's NPT, ITHK,
+  ! IPLA (IPARG(6), IPARG(28), IPARG(29)); DTFAC1(3) and DTMIN1(3).
+  subroutine or_pack_q4_params(pm, uparam, iparam, geo, iparg, dtfac, dtmin, p)
+    real(c_double), intent(in) :: pm(*), uparam(10), geo(*), dtfac, dtmin
+    integer, intent(in) :: iparam(4), iparg(*)
+    type(or_q4_params_t), intent(out) :: p
+    p%rho = pm(1)
+    p%young = pm(20)
+    p%nu = pm(21)
+    p%shear = pm(22)
+    p%ssp = pm(27)
+    p%ca = uparam(1)
+    p%cb = uparam(2)
+    p%cn = uparam(3)
+    p%epmx = uparam(4)
+    p%ymax = uparam(5)
+    p%cc = uparam(6)
+    p%epdr = uparam(7)
+    p%asrate = uparam(9)
+    p%m_exp = uparam(10)
+    p%iform = iparam(1)
+    p%icc = iparam(2)
+    p%vp = iparam(3)
+    p%israte = iparam(4)
+    p%npt = iparg(6)
+    p%ithk = iparg(28)
+    p%ipla = iparg(29)
+    p%ishear = nint(geo(37))
+    p%hm = geo(13)
+    p%hf = geo(14)
+    p%hr = geo(15)
+    p%fsh = geo(38)
+    p%dtfac = dtfac
+    p%dtmin = dtmin
+  end subroutine
 
   ! Text of the last failure of the calling thread.
   function or_error_message() result(msg)

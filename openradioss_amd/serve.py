@@ -9,9 +9,10 @@ a time; a request whose time reached its end leaves the batch, its results are
 downloaded and its memory is freed, which makes room for the next admissions.
 
 Latency of a request = completion time - arrival time (queueing, upload,
-compute and download included).  The batch runs on its own stream; the
-admission of a new request happens between batch steps, while the next step
-of the others is queued behind it on the device.
+compute and download included).  The batch runs on its own stream.  Each pass
+of the loop queues the next step of the running requests first and admits new
+ones after it, so that a new request's upload (on its own stream) overlaps the
+cycles of the others; the admitted request joins the launches of the next step.
 """
 from __future__ import annotations
 
@@ -132,11 +133,13 @@ class Server:
                 raise TimeoutError("serving loop exceeded its wall-clock limit")
             while pending and pending[0].arrival <= t:
                 self.waiting.append(pending.pop(0))
-            self._admit(t)
-            if self.running:
+            stepped = bool(self.running)
+            if stepped:
                 self.batch.run(self.cycles_per_step, sync=False)
+            self._admit(t)             # uploads overlap the step just queued
+            if stepped:
                 self._retire(now)
-            elif pending:
+            elif not self.running and pending:
                 time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
         wall = now()
         lat = np.array([r.latency for r in self.done])

@@ -35,6 +35,7 @@ FLAGS = ["-cpp", "-DMYREAL8", "-DCPP_mach=CPP_p4linux964", "-DCPP_rel=00", "-ffi
 LEAVES = [
     "engine/source/elements/solid/solide/sderi3.F",
     "engine/source/elements/solid/solide4/schkjabt3.F",
+    "engine/source/output/message/message.F",  # ANCMSG, called by SCHKJABT3 on a negative Jacobian
     "engine/source/elements/solid/solide/sdlen3.F",
     "engine/source/elements/solid/solide/slen.F",
     "engine/source/elements/solid/solide/sdefo3.F",

@@ -16,6 +16,25 @@ double or_cr_pow(double x, double y) { return ormath::or_pow(x, y); }
 double or_cr_log(double x) { return ormath::or_log(x); }
 double or_cr_exp(double x) { return ormath::or_exp(x); }
 
+// Counting build of the oracle (OR_COUNT_MATH): the host library's value is returned, and each
+// call site counts its calls and those whose correctly rounded value differs.
+static int64_t g_site[16][2];
+static inline double count_site(int site, double host, double cr) {
+  __atomic_fetch_add(&g_site[site][0], 1, __ATOMIC_RELAXED);
+  if (memcmp(&host, &cr, sizeof(double)) != 0) __atomic_fetch_add(&g_site[site][1], 1, __ATOMIC_RELAXED);
+  return host;
+}
+double or_count_pow(int site, double x, double y) { return count_site(site, pow(x, y), or_cr_pow(x, y)); }
+double or_count_log(int site, double x) { return count_site(site, log(x), or_cr_log(x)); }
+double or_count_exp(int site, double x) { return count_site(site, exp(x), or_cr_exp(x)); }
+void or_math_site_counts(int64_t *out) {
+  for (int s = 0; s < 16; ++s) {
+    out[2 * s] = __atomic_load_n(&g_site[s][0], __ATOMIC_RELAXED);
+    out[2 * s + 1] = __atomic_load_n(&g_site[s][1], __ATOMIC_RELAXED);
+  }
+}
+void or_math_site_reset(void) { memset(g_site, 0, sizeof(g_site)); }
+
 // Vector form: out[i] = log(x[i]) (fn 0), exp(x[i]) (fn 1) or pow(x[i], y[i]) (fn 2).
 void or_cr_eval(int fn, int64_t n, const double *x, const double *y, double *out) {
   for (int64_t i = 0; i < n; ++i)

@@ -275,18 +275,18 @@ void or_hex_forces(int64_t e0, int64_t e1, int64_t ne, const int32_t *conn,
     if (p->cc != ZERO) {
       if (p->vp == 1) {
         EPD = dmax(epsd, p->epdr);
-        EPD = OR_LOG(EPD / p->epdr);
+        EPD = OR_LOGS(OR_SITE_M2LAW_RATE, EPD / p->epdr);
       } else {
         EPD = dmax(epsd, EM15);
-        EPD = OR_LOG(EPD / p->epdr);
+        EPD = OR_LOGS(OR_SITE_M2LAW_RATE, EPD / p->epdr);
       }
       double MT = dmax(EM15, p->m_exp);
       EPD = dmax(ZERO, EPD);
-      EPD = (ONE + p->cc * EPD) * (ONE - OR_POW(tstar, MT));
+      EPD = (ONE + p->cc * EPD) * (ONE - OR_POWS(OR_SITE_M2LAW_RATE, tstar, MT));
       if (p->icc == 1) SIGMX = p->sigmx * EPD;
     } else {
       double MT = dmax(EM15, p->m_exp);
-      EPD = ONE - OR_POW(tstar, MT);
+      EPD = ONE - OR_POWS(OR_SITE_M2LAW_RATE, tstar, MT);
       if (p->icc == 1) SIGMX = p->sigmx * EPD;
     }
     double epxe = PLA[e];
@@ -295,9 +295,9 @@ void or_hex_forces(int64_t e0, int64_t e1, int64_t ne, const int32_t *conn,
       AK = CA + p->cb * epxe;
       QH = p->cb * EPD;
     } else if (epxe > ZERO) {
-      AK = CA + p->cb * OR_POW(epxe, p->cn);
-      if (p->cn > ONE) QH = (p->cb * p->cn * OR_POW(epxe, p->cn - ONE)) * EPD;
-      else QH = (p->cb * p->cn / OR_POW(epxe, ONE - p->cn)) * EPD;
+      AK = CA + p->cb * OR_POWS(OR_SITE_M2LAW_AK, epxe, p->cn);
+      if (p->cn > ONE) QH = (p->cb * p->cn * OR_POWS(OR_SITE_M2LAW_QH, epxe, p->cn - ONE)) * EPD;
+      else QH = (p->cb * p->cn / OR_POWS(OR_SITE_M2LAW_QH, epxe, ONE - p->cn)) * EPD;
     } else {
       AK = CA;
       QH = ZERO;
@@ -320,7 +320,7 @@ void or_hex_forces(int64_t e0, int64_t e1, int64_t ne, const int32_t *conn,
       double AD = ZERO, AL = ZERO;
       double CX = SSP + sqrt(VD2);
       if (off == ONE) {
-        if (VOLN > ZERO) AL = OR_POW(VOLN, ONE / THREE);
+        if (VOLN > ZERO) AL = OR_POWS(OR_SITE_MQVISCB_AL, VOLN, ONE / THREE);
         else AL = ZERO;
         AD = dmax(ZERO, DD);
       }
@@ -386,7 +386,7 @@ void or_hex_forces(int64_t e0, int64_t e1, int64_t ne, const int32_t *conn,
     double EHOU_e;
     {
       double CAQ = FOURTH * off * p->qh;
-      double FCL = CAQ * rhon * OR_POW(VOLN, TWO_THIRD);
+      double FCL = CAQ * rhon * OR_POWS(OR_SITE_SHVIS3_FCL, VOLN, TWO_THIRD);
       double FCQ = FCL * CAQ * HUNDRED;
       FCL = FCL * SSP;
       double G11 = ONE - PX1H1, G21 = -ONE - PX2H1, G31 = ONE - PX3H1, G41 = -ONE - PX4H1;

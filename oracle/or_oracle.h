@@ -40,6 +40,33 @@ double or_cr_exp(double x);
 #define OR_EXP exp
 #endif
 
+/* Call sites of the elementary functions.  The counting build (-DOR_COUNT_MATH) returns
+ * the host library's value, so it follows the reference's trajectory, and counts per site
+ * the calls whose correctly rounded value differs from it (or_math_site_counts). */
+enum {
+  OR_SITE_MQVISCB_AL = 0, /* mqviscb.F: AL = VOL**THIRD */
+  OR_SITE_SHVIS3_FCL,     /* shvis3.F: VOL**TWO_THIRD */
+  OR_SITE_M2LAW_AK,       /* m2law.F: EPXE**CN (yield stress) */
+  OR_SITE_M2LAW_QH,       /* m2law.F: EPXE**(CN-1) or EPXE**(1-CN) (hardening modulus) */
+  OR_SITE_M2LAW_RATE,     /* m2law.F: LOG(EPSD/EPDR), TSTAR**MT */
+  OR_SITE_M2CPLR,         /* m2cplr.F: EXP(CN*LOG(PLA)), LOG of the rate ratio */
+  OR_NSITE
+};
+#ifdef OR_COUNT_MATH
+double or_count_pow(int site, double x, double y);
+double or_count_log(int site, double x);
+double or_count_exp(int site, double x);
+void or_math_site_counts(int64_t *out); /* [OR_NSITE][2]: calls, calls whose results differ */
+void or_math_site_reset(void);
+#define OR_POWS(s, x, y) or_count_pow(s, x, y)
+#define OR_LOGS(s, x) or_count_log(s, x)
+#define OR_EXPS(s, x) or_count_exp(s, x)
+#else
+#define OR_POWS(s, x, y) OR_POW(x, y)
+#define OR_LOGS(s, x) OR_LOG(x)
+#define OR_EXPS(s, x) OR_EXP(x)
+#endif
+
 /* Material (law 2, Johnson-Cook form) + solid property + run control. */
 typedef struct {
   double rho0;     /* mat_param%rho        (hm_read_mat02_jc.F90: pm(1)=rhor) */

@@ -162,7 +162,7 @@ static void m2cplr_point(const m2c_t *c, double off, double gs, double epsp_in, 
     double epsp = epsp_in;
     if (c->israte == 0 && c->vp == 2) epsp = dmax(dmax(fabs(dxx), fabs(dyy)), HALF * fabs(dxy));
     epsp = dmax(epsp, c->epdr);
-    double logep = OR_LOG(epsp / c->epdr);
+    double logep = OR_LOGS(OR_SITE_M2CPLR, epsp / c->epdr);
     double q = (ONE + c->cc * logep);
     q = dmax(q, EM20);
     CA = CA * q;
@@ -175,7 +175,7 @@ static void m2cplr_point(const m2c_t *c, double off, double gs, double epsp_in, 
     YLD = CA;
   } else {
     double beta = CB * (ONE - ZERO);
-    YLD = CA + beta * OR_EXP(c->cn * OR_LOG(*pla));
+    YLD = CA + beta * OR_EXPS(OR_SITE_M2CPLR, c->cn * OR_LOGS(OR_SITE_M2CPLR, *pla));
   }
   YLD = dmin(YLD, YMAX);
   if (c->ipla == 0) {
@@ -191,7 +191,7 @@ static void m2cplr_point(const m2c_t *c, double off, double gs, double epsp_in, 
       *pla = *pla + DPLA;
       *epchk = dmax(*pla, *epchk);
       if (YLD >= YMAX) H = ZERO;
-      else H = c->cn * CB * OR_EXP((c->cn - ONE) * OR_LOG(*pla + SMALL));
+      else H = c->cn * CB * OR_EXPS(OR_SITE_M2CPLR, (c->cn - ONE) * OR_LOGS(OR_SITE_M2CPLR, *pla + SMALL));
     }
   } else if (c->ipla == 1) {
     double s1 = sxx + syy, s2 = sxx - syy, s3 = sxy;
@@ -202,7 +202,7 @@ static void m2cplr_point(const m2c_t *c, double off, double gs, double epsp_in, 
       double nu1 = ONE / (ONE - c->nu);
       double nu2 = ONE / (ONE + c->nu);
       if (YLD >= YMAX) H = ZERO;
-      else H = c->cn * CB * OR_EXP((c->cn - ONE) * OR_LOG(*pla + SMALL));
+      else H = c->cn * CB * OR_EXPS(OR_SITE_M2CPLR, (c->cn - ONE) * OR_LOGS(OR_SITE_M2CPLR, *pla + SMALL));
       double dpla_j = (svm - YLD) / (THREE * c->g + H);
       double anu1 = A * nu1;
       double bnu2 = THREE * B * nu2;
@@ -214,7 +214,7 @@ static void m2cplr_point(const m2c_t *c, double off, double gs, double epsp_in, 
         DPLA = dpla_j;
         double yld_i;
         if (pla_i == ZERO) yld_i = dmin(YMAX, CA);
-        else yld_i = dmin(YMAX, CA + CB * OR_EXP(c->cn * OR_LOG(pla_i)));
+        else yld_i = dmin(YMAX, CA + CB * OR_EXPS(OR_SITE_M2CPLR, c->cn * OR_LOGS(OR_SITE_M2CPLR, pla_i)));
         DR = HALF * c->young * dpla_i / yld_i;
         P = ONE / (ONE + DR * nu1);
         Q = ONE / (ONE + THREE * DR * nu2);
@@ -239,7 +239,7 @@ static void m2cplr_point(const m2c_t *c, double off, double gs, double epsp_in, 
     double yld2 = YLD * YLD;
     if (svm2 > yld2 && off == ONE) {
       if (YLD >= YMAX) H = ZERO;
-      else H = c->cn * CB * OR_EXP((c->cn - ONE) * OR_LOG(*pla + SMALL));
+      else H = c->cn * CB * OR_EXPS(OR_SITE_M2CPLR, (c->cn - ONE) * OR_LOGS(OR_SITE_M2CPLR, *pla + SMALL));
       double aa = (svm2 - yld2) / (5.0 * svm2 + THREE * (-sxx * syy + sxy * sxy));
       double s1 = (ONE - TWO * aa) * sxx + aa * syy;
       double s2 = aa * sxx + (ONE - TWO * aa) * syy;

@@ -20,7 +20,12 @@ _CR_DEPS = [os.path.join(_HERE, "..", "openradioss_amd", "csrc", f) for f in ("o
 # the reference's compiled routines).  "cr": the same restatement under correctly rounded
 # pow/log/exp, the elementary functions of the FP64 device path.
 MATH_MODES = ("libm", "cr")
-_LIBS = {"libm": os.path.join(_HERE, "liboracle.so"), "cr": os.path.join(_HERE, "liboracle_cr.so")}
+# "count": the libm build that also counts, per call site, the calls where the correctly rounded
+# value differs from the host library's (or_math_site_counts); a diagnostic of the libm spread.
+_LIBS = {"libm": os.path.join(_HERE, "liboracle.so"), "cr": os.path.join(_HERE, "liboracle_cr.so"),
+         "count": os.path.join(_HERE, "liboracle_count.so")}
+MATH_SITES = ("MQVISCB VOL**THIRD", "SHVIS3 VOL**TWO_THIRD", "M2LAW EPXE**CN", "M2LAW EPXE**(CN-1)",
+              "M2LAW rate LOG / TSTAR**MT", "M2CPLR EXP(CN*LOG(PLA)) and LOG")
 _LIB = _LIBS["libm"]
 
 
@@ -40,8 +45,8 @@ def build(force: bool = False, math: str = "libm") -> str:
         tmp = out + ".tmp.o"
         subprocess.check_call(["g++", *cflags, "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
                                "-c", _CR_SRC, "-o", tmp])
-        subprocess.check_call(["gcc", *cflags, "-DOR_CR_MATH", "-shared", *_SRCS, tmp, "-o", out, "-lm",
-                               "-lstdc++"])
+        flag = "-DOR_CR_MATH" if math == "cr" else "-DOR_COUNT_MATH"
+        subprocess.check_call(["gcc", *cflags, flag, "-shared", *_SRCS, tmp, "-o", out, "-lm", "-lstdc++"])
         os.remove(tmp)
     return out
 
@@ -70,6 +75,9 @@ def lib(math: str = "libm"):
         L.or_shell_const.argtypes = [C.c_int]
         L.or_shell_table.restype = C.c_double
         L.or_shell_table.argtypes = [C.c_int, C.c_int, C.c_int]
+        if math == "count":
+            L.or_math_site_counts.restype = None
+            L.or_math_site_reset.restype = None
         _libs[math] = L
     return _libs[math]
 
@@ -282,3 +290,15 @@ class ModelOracle:
         finally:
             self.c.ecrit = None
         return out
+
+
+def math_site_counts() -> dict:
+    """Per call site of the elementary functions: (calls, calls whose correctly rounded value
+    differs from the host library's), accumulated by runs with math="count" since the last reset."""
+    out = np.zeros(32, np.int64)
+    lib("count").or_math_site_counts(out.ctypes.data_as(C.c_void_p))
+    return {name: (int(out[2 * k]), int(out[2 * k + 1])) for k, name in enumerate(MATH_SITES)}
+
+
+def math_site_reset():
+    lib("count").or_math_site_reset()

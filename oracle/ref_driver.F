@@ -15,6 +15,7 @@ C  and the law-2 glue of MMAIN (mmain.F90:668-745: VOL_AVG, AMU, RHOREF).
       USE MATPARAM_DEF_MOD
       USE GLOB_THERM_MOD
       USE ALEANIM_MOD, ONLY : FANI_CELL_
+      USE MESSAGE_MOD
 #include "implicit_f.inc"
 #include "mvsiz_p.inc"
 #include "param_c.inc"
@@ -98,6 +99,9 @@ C-----switches of the restated branch (common blocks)
       IDYNA = 0
       ISMDISP = 0
       IVECTOR = 0
+C-----message table as build_msg.F allocates it (2 types x message ids), without the texts
+C     of the Engine's message file: ANCMSG (SCHKJABT3 on a negative Jacobian) prints the id only
+      IF (.NOT. ALLOCATED(MESSAGES)) ALLOCATE(MESSAGES(2, 5000))
       INVSTR = 100
       NUMMAT = 1
       NUMGEO = 1

@@ -30,12 +30,16 @@ def main():
             continue
         m = build_case(c)
         r = R.RefHex(m)
-        r.cycle(c["cycles"])
-        assert r.mstop == 0
+        # cycle by cycle: a cycle that sets MSTOP is the run's last (resol.F ends the run there)
+        for _ in range(c["cycles"]):
+            r.cycle(1)
+            if r.mstop:
+                break
+        assert (r.mstop != 0) == (c["kind"] == "crush")
         np.savez_compressed(os.path.join(HERE, name + ".npz"), x=r.x, v=r.v, d=r.d, sig=r.sig, eint=r.eint,
                             rho=r.rho, pla=r.pla, epsd=r.epsd, offg=r.offg, qvis=r.qvis,
                             time=np.array([r.tt, r.dt1, r.dt2, r.dt12, r.dt2old]),
-                            ncycle=np.array([r.ncycle]))
+                            ncycle=np.array([r.ncycle]), mstop=np.array([r.mstop]))
         print(name, m.ne, "elements", c["cycles"], "cycles, tt", r.tt)
     for name, c in Q4_CASES.items():
         if only and name not in only:

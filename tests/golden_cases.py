@@ -10,12 +10,34 @@ CASES = {
     "impact_4x4x16_c120": dict(kind="bar", nx=4, ny=4, nz=16, v0=6.0e4, cycles=120),
     # distorted free block with random velocities and linear hardening (CN=1 branch)
     "distorted_linhard_c80": dict(kind="distorted", n=4, seed=9, cycles=80),
+    # plastic impact with Johnson-Cook rate hardening (CC != 0, ISRATE = 1): total strain rate
+    # (VP = 2) filtered at 100 kHz (m2law.F:227-245, mstrain_rate.F)
+    "impact_rate_c120": dict(kind="bar", nx=4, ny=4, nz=16, v0=6.0e4, cycles=120,
+                             mat=dict(c=0.05, eps0=1.0, fcut=1.0e5)),
+    # the same with the deviatoric strain rate (VP = 3, IDEV = 1) and no filter
+    "impact_vp3_c120": dict(kind="bar", nx=4, ny=4, nz=16, v0=6.0e4, cycles=120,
+                            mat=dict(c=0.05, eps0=1.0, vp=3)),
+    # a block crushed at 1 km/s until an element's Jacobian turns negative: SCHKJABT3 deletes it
+    # and sets MSTOP, which ends the run after that cycle (sderi3.F:147-153, schkjabt3.F)
+    "inverted_mstop": dict(kind="crush", n=2, v_top=-1.0e6, cycles=400),
 }
 
 
 def build_case(c):
     if c["kind"] == "bar":
-        return M.cantilever_bar(c["nx"], c["ny"], c["nz"], v0=c["v0"])
+        mat = M.STEEL if "mat" not in c else M.Law2(rho=7.85e-9, young=210000.0, nu=0.3, a=350.0, b=275.0,
+                                                     n=0.36, **c["mat"])
+        return M.cantilever_bar(c["nx"], c["ny"], c["nz"], v0=c["v0"], mat=mat)
+    if c["kind"] == "crush":
+        n = c["n"]
+        x, conn = M.hex_block(n, n, n, 1.0)
+        top = np.nonzero(x[:, 2] == float(n))[0]
+        bcs = np.where(x[:, 2] == 0.0, 7, 0).astype(np.int32)
+        # one top corner driven down faster than the rest, so that one element inverts first
+        vals = np.full(top.size, c["v_top"])
+        vals[0] = 3.0 * c["v_top"]
+        imp = (top.astype(np.int32), np.full(top.size, 2, np.int32), vals)
+        return M.build_hex_model(x, conn, M.STEEL, bcs=bcs, imp=imp)
     rng = np.random.default_rng(c["seed"])
     x, conn = M.hex_block(c["n"], c["n"], c["n"], 1.0)
     x = x + 0.08 * rng.uniform(-1.0, 1.0, x.shape)
@@ -33,6 +55,10 @@ Q4_CASES = {
     # distorted plate with random translational and rotational velocities, one triangle,
     # radial-return plasticity (IPLA=0) and constant thickness (ITHK=0)
     "plate_bend_c120": dict(kind="plate", seed=5, ipla=0, ithk=0, cycles=120),
+    # the same plate with a shell time-step floor DTMIN1(3) just above its smallest element time
+    # step: CDT3 with IDTMIN(3) = 2 deletes the elements whose time step falls below it
+    # (cdt3.F:147-165), and the run continues without them
+    "plate_dtmin_c120": dict(kind="plate", seed=5, ipla=0, ithk=0, cycles=120, dtmin=1.4e-7),
     # hex block with a shell plate tied on its top face (configs[2] family)
     "mixed_c120": dict(kind="mixed", seed=7, cycles=120),
 }
@@ -54,7 +80,7 @@ def build_q4_case(c):
         nn = x.shape[0]
         v = 2.0e3 * rng.standard_normal((nn, 3))
         vr = 500.0 * rng.standard_normal((nn, 3))
-        prop = M.ShellProp(thick=1.2, ipla=c["ipla"], ithk=c["ithk"])
+        prop = M.ShellProp(thick=1.2, ipla=c["ipla"], ithk=c["ithk"], dtmin=c.get("dtmin", 0.0))
         return M.build_model(x, q4_conn=conn, q4_mat=M.STEEL, q4_prop=prop, v0=v, vr0=vr)
     # mixed: 3x3x3 hex block, 5x5 plate on top sharing the block's top nodes
     xh, ch = M.hex_block(3, 3, 3, 1.0)

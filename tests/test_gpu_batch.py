@@ -99,3 +99,41 @@ def test_removed_member_runs_alone_again(gpu_lib):
     b.close()
     g.close()
     ref.close()
+
+
+def test_full_tube_sweep_members_equal_the_oracle(gpu_lib):
+    """The 256-tube parametric sweep (60 x 300 mm tubes, element size 1-4 mm, 4.5k-72k shells each,
+    6,032,500 shells) as one batch for 200 batch cycles; 8 members spanning every element size,
+    each at its own time step, equal the oracle (correctly rounded pow / log / exp) bit for bit."""
+    import oracle as O
+    print("case: 256-tube sweep, one fp64 batch, 200 cycles; 8 members against the oracle", flush=True)
+    sw = M.tube_sweep(256, seed=20261203)
+    b = E.GpuBatch()
+    gms = []
+    for m, _ in sw:
+        g = E.GpuModel(m, precision="fp64")
+        b.add(g, 5.0e-3)
+        gms.append(g)
+    b.run(200)
+    assert b.active() == 256
+    picks = []
+    for h in M.SWEEP_SIZES:                 # two members of every element size but the largest
+        idx = [i for i, (_, p) in enumerate(sw) if p["h"] == h]
+        picks += idx[:2] if h != 1.0 else idx[:1]
+    picks = picks[:8]
+    dts = set()
+    for i in picks:
+        m, p = sw[i]
+        ts = gms[i].time_state()
+        assert ts.ncycle == 200 and ts.mstop == 0
+        o = O.ModelOracle(m, nthreads=16, math="cr")
+        o.cycle(200)
+        x, v, vr, d = gms[i].nodes6()
+        print(f"tube {i}: h {p['h']} mm, {m.ne_q4} shells, dt {ts.dt2:.4e} s", flush=True)
+        assert (ts.tt, ts.dt2) == (o.ts.tt, o.ts.dt2)
+        assert np.array_equal(d, o.d) and np.array_equal(v, o.v) and np.array_equal(vr, o.vr)
+        dts.add(ts.dt2)
+    assert len(dts) == len(picks)           # each model at its own time step
+    b.close()
+    for g in gms:
+        g.close()

@@ -66,6 +66,14 @@ def test_fp64_bitwise_vs_cr_oracle_and_golden(name, gpu_lib):
         assert np.array_equal(hs["pla"], o.hpla)
     e_gpu, e_ora = g.energy(), o.energy()
     assert abs(e_gpu.sum() - e_ora.sum()) <= 1e-13 * abs(e_ora.sum())
+    # total energy against the reference's state: internal (shell EINT, hex EINT*VOL0) + kinetic
+    ie = float(np.sum(gold["q_eint"])) if "q_eint" in gold else 0.0
+    if m.hexp is not None:
+        ie += float(np.sum(gold["h_eint"] * m.hexp.vol0))
+    ke = 0.5 * float(np.sum(m.ms * np.sum(gold["v"] ** 2, axis=1)) + np.sum(m.inr * np.sum(gold["vr"] ** 2, axis=1)))
+    e_ref = ie + ke
+    print(f"total energy rel err vs reference golden {abs(e_gpu.sum() - e_ref) / abs(e_ref):.3e}", flush=True)
+    assert abs(e_gpu.sum() - e_ref) <= 1e-12 * abs(e_ref)
     g.close()
 
 

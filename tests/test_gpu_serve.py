@@ -33,3 +33,37 @@ def test_served_results_equal_alone_runs(gpu_lib):
         assert np.array_equal(d, r.result["d"]) and np.array_equal(v, r.result["v"])
         assert g.time_state().tt == r.result["tt"]
         g.close()
+
+
+def test_served_crash_solves_at_full_size(gpu_lib):
+    """Six requests of the served workload: bars and tubes of 5k-500k elements (lognormal),
+    FP64 and split, end times of 1-2 ms, Poisson arrivals, admitted by footprint into one batch.
+    The FP64 request of the fewest elements equals the oracle bit for bit; every request equals
+    its model advanced alone to the same cycle."""
+    import oracle as O
+    print("case: serving 6 requests of 5k-500k elements, end times 1-2 ms", flush=True)
+    reqs = S.request_stream(6, rate=20.0, seed=11, t_end_range=(1.0e-3, 2.0e-3))
+    sizes = sorted(r.ne for r in reqs)
+    print("request sizes", sizes, [r.precision for r in reqs], flush=True)
+    assert sizes[0] >= 5_000 and sizes[-1] > 100_000
+    from openradioss_amd import lib as L
+    free, _ = L.device_memory(0)
+    srv = S.Server(hbm_budget=int(0.5 * free), cycles_per_step=200)
+    stats = srv.serve(reqs)
+    srv.close()
+    print(stats, flush=True)
+    assert stats["requests"] == 6
+    fp64 = [r for r in reqs if r.precision == "fp64"]
+    small = min(fp64, key=lambda r: r.ne)
+    for r in reqs:
+        assert r.result["tt"] >= r.t_end
+        g = E.GpuModel(r.model, precision=r.precision)
+        g.run(r.ncycle)
+        x, v, vr, d = g.nodes6()
+        assert np.array_equal(d, r.result["d"]) and np.array_equal(v, r.result["v"])
+        assert g.time_state().tt == r.result["tt"]
+        g.close()
+    o = O.ModelOracle(small.model, nthreads=16, math="cr")
+    o.cycle(small.ncycle)
+    print(f"oracle check: request {small.rid}, {small.kind}, {small.ne} elements, {small.ncycle} cycles", flush=True)
+    assert o.ts.tt == small.result["tt"] and np.array_equal(o.d, small.result["d"])

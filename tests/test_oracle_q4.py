@@ -44,6 +44,9 @@ def test_golden_cases_exercise_their_branches():
     g = np.load(os.path.join(HERE, "golden", "plate_bend_c120.npz"))
     assert np.abs(g["vr"]).max() > 0.0 and np.abs(g["q_mom"]).max() > 0.0 and g["q_pla"].max() > 0.0
     assert np.abs(g["q_hourg"][3:5]).max() > 0.0
+    # the shell time-step floor deleted elements (OFF 0) and the run went on without them
+    g = np.load(os.path.join(HERE, "golden", "plate_dtmin_c120.npz"))
+    assert 0 < (g["q_offg"] == 0.0).sum() < g["q_offg"].size
 
 
 @pytest.mark.skipif(not R.available(), reason="oracle/_ref not built (needs the reference tree)")
