@@ -195,6 +195,19 @@ int or_sforc3_group(int device, const or_hex_params *p, int64_t numnod, int32_t 
                     double *qvis, int64_t lsky, double *fsky, const int32_t *iads, double dt1,
                     double *dt2t, int32_t *mstop);
 
+/* Allocation-free variant of or_sforc3_group: no counterpart in the reference.  It is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation): it calls no hipMalloc / hipFree and stages the
+ * group through one file-scope device array of 139,264 bytes per device, held until the
+ * process ends (the group's own nodes, at most 8 per element, its GBUF arrays and its FSKY
+ * rows).  Same arguments, results and errors as or_sforc3_group; nel is at most 128.  Every
+ * call on a device uses that device's one array: calls that overlap on one device race. */
+int or_sforc3_group_static(int device, const or_hex_params *p, int64_t numnod, int32_t nel, int64_t nft,
+                           const double *x, const double *v, const int32_t *ixs, double *sig, double *eint,
+                           double *rho, const double *vol, double *pla, double *epsd, double *off,
+                           double *qvis, int64_t lsky, double *fsky, const int32_t *iads, double dt1,
+                           double *dt2t, int32_t *mstop);
+
 /* ---- group-level drop-in for CFORC3 (one group of nel <= 128 Q4 shells) ----
  * x, v, vr X/V/VR(3,numnod); ixc IXC(7,*) from column nft+1; GBUF arrays of the group;
  * LBUF sig (npt,5,nel), pla (npt,nel), epsd_ip (npt,nel); fsky FSKY(8,lsky);

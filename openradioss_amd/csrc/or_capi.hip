@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <numeric>
 #include <cstdlib>
 #include <cstring>
@@ -2065,6 +2066,134 @@ int or_sforc3_group(int device, const or_hex_params *p, int64_t numnod, int32_t 
   for (int i = 0; i < nel; ++i) {
     if (dtx[i] < *dt2t) *dt2t = dtx[i];
     if (msv[i]) *mstop = 1;
+  }
+  return OR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SFORC3 group drop-in without allocation.  The group is staged through one file-scope
+// device array instead of per-call device buffers: its own nodes (at most 8 per element,
+// renumbered in order of first appearance), its GBUF arrays and its FSKY rows (8 per
+// element, gathered from the caller's FSKY and scattered back, so that words the kernel
+// does not write keep their values).  k_sforc3_group runs on the staged copy with
+// nft = 0.  The array holds kStaticGroupBytes per device, for the whole process.
+namespace {
+constexpr int kStaticNel = 128;                                   // MVSIZ - 1
+constexpr size_t kStaticGroupDoubles = 6 * 8 * kStaticNel         // x, v of the group's nodes
+                                       + 13 * kStaticNel           // GBUF (SIG 6, 7 scalars)
+                                       + 64 * kStaticNel           // FSKY rows (8 words x 8)
+                                       + kStaticNel;               // element time steps
+constexpr size_t kStaticGroupInts = 11 * kStaticNel + 8 * kStaticNel + kStaticNel; // IXS, IADS, MSTOP
+constexpr size_t kStaticGroupBytes = 8 * kStaticGroupDoubles + 4 * kStaticGroupInts;
+
+static __device__ __align__(256) unsigned char scratch[kStaticGroupBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char *g_scratch[kMaxDevices] = {};
+
+// The device address of this device's copy of `scratch` (host code never uses the array's
+// name as an address): looked up once per device, with that device current.
+int scratch_of(int device, unsigned char **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(OR_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[device]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(OR_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = (unsigned char *)ptr;
+  }
+  *out = g_scratch[device];
+  return OR_OK;
+}
+} // namespace
+
+int or_sforc3_group_static(int device, const or_hex_params *p, int64_t numnod, int32_t nel, int64_t nft,
+                           const double *x, const double *v, const int32_t *ixs, double *sig, double *eint,
+                           double *rho, const double *vol, double *pla, double *epsd, double *off, double *qvis,
+                           int64_t lsky, double *fsky, const int32_t *iads, double dt1, double *dt2t,
+                           int32_t *mstop) {
+  if (!p || !x || !v || !ixs || !sig || !eint || !rho || !vol || !pla || !epsd || !off || !qvis || !fsky || !iads ||
+      !dt2t || !mstop)
+    return fail(OR_ERR_ARG, "null argument");
+  if (nel <= 0 || nel > kStaticNel)
+    return fail(OR_ERR_ARG, "nel must be in 1..128: the staging array holds one group of at most 128 hexes");
+  if (numnod <= 0 || lsky <= 0 || nft < 0) return fail(OR_ERR_ARG, "bad sizes");
+  int rc = check_hex_params(*p);
+  if (rc) return rc;
+  for (int i = 0; i < nel; ++i)
+    for (int k = 0; k < 8; ++k) {
+      const int32_t nid = ixs[11 * (nft + i) + 1 + k];
+      if (nid < 1 || nid > numnod) return fail(OR_ERR_ARG, "IXS node id out of range");
+      const int32_t K = iads[8 * (nft + i) + k];
+      if (K < 1 || K > lsky) return fail(OR_ERR_ARG, "IADS row out of range");
+    }
+  OR_HIP(hipSetDevice(device));
+  unsigned char *base = nullptr;
+  if ((rc = scratch_of(device, &base))) return rc;
+  const HexConsts c = make_hex_consts(*p);
+  // the staged group on the host, laid out as in the device array
+  double hd[kStaticGroupDoubles];
+  int32_t hi[kStaticGroupInts];
+  double *lx = hd, *lv = hd + 24 * kStaticNel, *lsig = hd + 48 * kStaticNel;
+  double *leint = lsig + 6 * nel, *lrho = leint + nel, *lvol = lrho + nel, *lpla = lvol + nel, *lepsd = lpla + nel,
+         *loff = lepsd + nel, *lqvis = loff + nel;
+  double *lfsky = hd + 61 * kStaticNel, *ldtx = hd + 125 * kStaticNel;
+  int32_t *lixs = hi, *liads = hi + 11 * kStaticNel, *lms = hi + 19 * kStaticNel;
+  int32_t nloc = 0;
+  int32_t gid[8 * kStaticNel]; // global (1-based) id of each local node
+  for (int i = 0; i < nel; ++i) {
+    for (int k = 0; k < 8; ++k) {
+      const int32_t g = ixs[11 * (nft + i) + 1 + k];
+      int32_t l = 0;
+      while (l < nloc && gid[l] != g) ++l;
+      if (l == nloc) {
+        gid[nloc++] = g;
+        for (int d = 0; d < 3; ++d) {
+          lx[3 * l + d] = x[3 * (int64_t)(g - 1) + d];
+          lv[3 * l + d] = v[3 * (int64_t)(g - 1) + d];
+        }
+      }
+      lixs[11 * i + 1 + k] = l + 1;
+      liads[8 * i + k] = 8 * i + k + 1;
+      const int64_t K = iads[8 * (nft + i) + k] - 1;
+      for (int w = 0; w < 8; ++w) lfsky[8 * (8 * i + k) + w] = fsky[8 * K + w];
+    }
+    lixs[11 * i] = ixs[11 * (nft + i)];
+    for (int w = 9; w < 11; ++w) lixs[11 * i + w] = ixs[11 * (nft + i) + w];
+  }
+  for (int k = 0; k < 6; ++k)
+    for (int i = 0; i < nel; ++i) lsig[k * nel + i] = sig[(int64_t)k * nel + i];
+  for (int i = 0; i < nel; ++i) {
+    leint[i] = eint[i]; lrho[i] = rho[i]; lvol[i] = vol[i]; lpla[i] = pla[i];
+    lepsd[i] = epsd[i]; loff[i] = off[i]; lqvis[i] = qvis[i];
+  }
+  double *dd = (double *)base;
+  int32_t *di = (int32_t *)(base + 8 * kStaticGroupDoubles);
+  OR_HIP(hipMemcpy(dd, hd, sizeof(hd), hipMemcpyHostToDevice));
+  OR_HIP(hipMemcpy(di, hi, sizeof(hi), hipMemcpyHostToDevice));
+  double *dsig = dd + 48 * kStaticNel;
+  double *deint = dsig + 6 * nel, *drho = deint + nel, *dvol = drho + nel, *dpla = dvol + nel, *depsd = dpla + nel,
+         *doff = depsd + nel, *dq = doff + nel;
+  hipLaunchKernelGGL(k_sforc3_group, dim3(1), dim3(kBlock), 0, 0, nel, (int64_t)0, (const double *)dd,
+                     (const double *)(dd + 24 * kStaticNel), (const int32_t *)di, dsig, deint, drho,
+                     (const double *)dvol, dpla, depsd, doff, dq, dd + 61 * kStaticNel,
+                     (const int32_t *)(di + 11 * kStaticNel), c, dt1, dd + 125 * kStaticNel, (int *)(di + 19 * kStaticNel));
+  OR_HIP(hipGetLastError());
+  OR_HIP(hipMemcpy(hd, dd, sizeof(hd), hipMemcpyDeviceToHost)); // synchronous after the launch
+  OR_HIP(hipMemcpy(hi, di, sizeof(hi), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 6; ++k)
+    for (int i = 0; i < nel; ++i) sig[(int64_t)k * nel + i] = lsig[k * nel + i];
+  for (int i = 0; i < nel; ++i) {
+    eint[i] = leint[i]; rho[i] = lrho[i]; pla[i] = lpla[i]; epsd[i] = lepsd[i]; off[i] = loff[i];
+    qvis[i] = lqvis[i];
+    for (int k = 0; k < 8; ++k) {
+      const int64_t K = iads[8 * (nft + i) + k] - 1;
+      for (int w = 0; w < 8; ++w) fsky[8 * K + w] = lfsky[8 * (8 * i + k) + w];
+    }
+    if (ldtx[i] < *dt2t) *dt2t = ldtx[i];
+    if (lms[i]) *mstop = 1;
   }
   return OR_OK;
 }

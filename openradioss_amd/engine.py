@@ -302,13 +302,14 @@ class GpuBatch:
 
 
 def sforc3_group(params: L.OrHexParams, x, v, ixs, gbuf: dict, fsky, iads, nft: int, nel: int, dt1: float,
-                 dt2t: float, device: int = 0):
+                 dt2t: float, device: int = 0, static: bool = False):
     """SFORC3 on one group (reference layouts; see or_sforc3_group in include/or_gpu.h).
 
     x, v: X(3,NUMNOD) as (numnod,3); ixs: IXS(11,*) as (ncol,11) int32 (1-based ids);
     gbuf: dict of GBUF arrays of the group (sig as (6,nel) = SIG(NEL,6) in Fortran order);
     fsky: FSKY(8,LSKY) as (lsky,8); iads: IADS(8,*) as (ncol,8) 1-based.
-    Updates gbuf and fsky in place; returns (dt2t, mstop).
+    Updates gbuf and fsky in place; returns (dt2t, mstop).  static=True calls the
+    allocation-free variant (or_sforc3_group_static).
     """
     lib = L.load()
     d = C.c_double(dt2t)
@@ -316,7 +317,8 @@ def sforc3_group(params: L.OrHexParams, x, v, ixs, gbuf: dict, fsky, iads, nft: 
     arrs = [gbuf[k] for k in ("sig", "eint", "rho", "vol", "pla", "epsd", "off", "qvis")]
     for a in arrs + [fsky]:
         assert a.dtype == np.float64 and a.flags.c_contiguous
-    L.check(lib.or_sforc3_group(device, C.byref(params), x.shape[0], nel, nft, _p(_f64(x)), _p(_f64(v)),
+    fn = lib.or_sforc3_group_static if static else lib.or_sforc3_group
+    L.check(fn(device, C.byref(params), x.shape[0], nel, nft, _p(_f64(x)), _p(_f64(v)),
                                 _p(np.ascontiguousarray(ixs, np.int32)), *[_p(a) for a in arrs], fsky.shape[0],
                                 _p(fsky), _p(np.ascontiguousarray(iads, np.int32)), float(dt1), C.byref(d),
                                 C.byref(ms)))
