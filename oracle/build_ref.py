@@ -67,8 +67,11 @@ LEAVES = [
     "engine/source/elements/shell/coqini.F",
     "engine/source/materials/mat/mat002/sigeps02c.F",
     "engine/source/materials/mat/mat002/m2cplr.F",
+    "engine/source/elements/solid/solide/sbilan.F",
+    "engine/source/elements/shell/coque/cbilan.F",
+    "engine/source/tools/sensor/sensor_energy_bilan.F",
 ]
-DRIVERS = ["ref_driver.F", "ref_shell.F"]
+DRIVERS = ["ref_driver.F", "ref_shell.F", "ref_bilan.F"]
 
 _MOD = re.compile(r"^\s*module\s+([a-z0-9_]+)\s*(!.*)?$", re.I)
 

@@ -787,11 +787,12 @@ void or_nodes_model(or_model_c *m, or_time_state *ts, double dt2t, int mstop) {
   }
   if (m->ecrit) {
     /* internal energy: SBILAN EI = EINT*VOL (sbilan.F:140), CBILAN EI = EINT(1)+EINT(2)
-     * (cbilan.F:183), summed in element order */
-    double ie = ZERO;
-    for (int64_t e = 0; e < m->ne_hex; ++e) ie = ie + m->heint[e] * m->hvol0[e];
-    for (int64_t e = 0; e < m->ne_q4; ++e) ie = ie + (m->q.eint[e] + m->q.eint[m->ne_q4 + e]);
-    m->ecrit[0] = ie;
+     * (cbilan.F:168), summed in element order into the part's PARTSAV(1) (solids part 1,
+     * shells part 2), the parts added in part order by ECRIT (ecrit.F:344) */
+    double ie_h = ZERO, ie_q = ZERO;
+    for (int64_t e = 0; e < m->ne_hex; ++e) ie_h = ie_h + m->heint[e] * m->hvol0[e];
+    for (int64_t e = 0; e < m->ne_q4; ++e) ie_q = ie_q + (m->q.eint[e] + m->q.eint[m->ne_q4 + e]);
+    m->ecrit[0] = ie_h + ie_q;
     m->ecrit[1] = encin;
     m->ecrit[2] = enrot;
   }
@@ -802,9 +803,13 @@ void or_nodes_model(or_model_c *m, or_time_state *ts, double dt2t, int mstop) {
 /* Energies of a mixed model: out[0] internal (hex EINT*VOL0 + shell EINT(1)+EINT(2)),
  * out[1] translational kinetic 0.5*sum MS|V|^2, out[2] rotational 0.5*sum IN|VR|^2. */
 void or_energy_model(const or_model_c *m, double out[3]) {
-  double ie = 0.0, ke = 0.0, kr = 0.0;
-  for (int64_t e = 0; e < m->ne_hex; ++e) ie += m->heint[e] * m->hvol0[e];
-  for (int64_t e = 0; e < m->ne_q4; ++e) ie += m->q.eint[e] + m->q.eint[m->ne_q4 + e];
+  /* internal energy per part as SBILAN / CBILAN accumulate PARTSAV(1,part) (sbilan.F:140,
+     cbilan.F:168), element after element; ECRIT adds the parts in part order (ecrit.F:344):
+     the solids are part 1, the shells part 2 */
+  double ie_h = 0.0, ie_q = 0.0, ke = 0.0, kr = 0.0;
+  for (int64_t e = 0; e < m->ne_hex; ++e) ie_h += m->heint[e] * m->hvol0[e];
+  for (int64_t e = 0; e < m->ne_q4; ++e) ie_q += m->q.eint[e] + m->q.eint[m->ne_q4 + e];
+  const double ie = ie_h + ie_q;
   for (int64_t n = 0; n < m->nn; ++n) {
     const double *v = m->v + 3 * n;
     ke += m->ms[n] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);

@@ -272,3 +272,39 @@ class RefModel:
                           C.c_double(self.dt2), C.c_double(self.dt12))
             self.tt = self.tt + self.dt2
             self.ncycle += 1
+
+
+def bilan_internal_energy(model, heint=None, hrho=None, hoffg=None, v=None, q_eint=None, q_offg=None, x=None,
+                          vr=None) -> tuple[float, float]:
+    """Internal energy of a state as the reference's SBILAN and CBILAN add it into PARTSAV(1) of
+    the solid part and of the shell part, group of 128 after group (oracle/ref_bilan.F).
+    Returns (solids, shells)."""
+    L = lib()
+    for f in ("ref_sbilan", "ref_cbilan"):
+        getattr(L, f).restype = None
+    npsav = 29
+    eh = eq = 0.0
+    if model.hexp is not None:
+        ps = np.zeros(npsav)
+        conn = model.hexp.conn
+        vol0 = np.ascontiguousarray(model.hexp.vol0, np.float64)
+        for e0 in range(0, model.ne_hex, 128):
+            e1 = min(model.ne_hex, e0 + 128)
+            vel = np.ascontiguousarray(v[conn[e0:e1]], np.float64)     # (nel, 8, 3) = VEL(3,8,NEL)
+            L.ref_sbilan(C.c_int(e1 - e0), _p(vel), _p(np.ascontiguousarray(heint[e0:e1])),
+                         _p(np.ascontiguousarray(hrho[e0:e1])), _p(np.ascontiguousarray(vol0[e0:e1])),
+                         _p(np.ascontiguousarray(vol0[e0:e1])), _p(np.ascontiguousarray(hoffg[e0:e1])), _p(ps))
+        eh = float(ps[0])
+    if model.q4 is not None:
+        ps = np.zeros(npsav)
+        conn1 = np.ascontiguousarray(model.q4.conn + 1, np.int32)
+        rho = float(model.q4.params()["rho"])
+        for e0 in range(0, model.ne_q4, 128):
+            e1 = min(model.ne_q4, e0 + 128)
+            ge = np.ascontiguousarray(q_eint[:, e0:e1])                 # EINT(NEL,2)
+            gvol = np.ones(e1 - e0)
+            L.ref_cbilan(C.c_int(e1 - e0), C.c_int(model.nn), _p(np.ascontiguousarray(x)), _p(np.ascontiguousarray(v)),
+                         _p(np.ascontiguousarray(vr)), _p(np.ascontiguousarray(conn1[e0:e1])), _p(ge), C.c_double(rho),
+                         _p(gvol), _p(np.ascontiguousarray(q_offg[e0:e1])), _p(ps))
+        eq = float(ps[0])
+    return eh, eq
