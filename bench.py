@@ -60,9 +60,10 @@ def gpu_run(model, precision: str, steps: int, warmup: int):
     x, v, vr, d = g.nodes6()
     e = g.energy()
     foot = g.footprint()
+    plan = g.assembly_stats()
     g.close()
     return dict(wall=wall, hex_ms=hex_ms, q4_ms=q4_ms, node_ms=node_ms, ts=ts, d=d, energy=e, footprint=foot,
-                finite=bool(np.isfinite(d).all()), variants=variants)
+                finite=bool(np.isfinite(d).all()), variants=variants, plan=plan)
 
 
 def kernel_name(family: str, real: str, variant: int) -> str:
@@ -336,7 +337,7 @@ def main():
     eu64 = ne * args.steps / r64["wall"]
     log(f"fp64: {eu64:.3e} element-updates/s; hex {r64['hex_ms']:.3f} ms, q4 {r64['q4_ms']:.3f} ms, "
         f"node {r64['node_ms']:.3f} ms per cycle")
-    tb = T.cycle_bytes(model, "fp64")
+    tb = T.cycle_bytes(model, "fp64", r64["plan"])
     # achieved bandwidth from the compulsory bytes (element state, connectivity and node data once);
     # the as-implemented figure (with the assembly's rows and row indices) is reported beside it
     hex_gbs = tb["hex"]["compulsory"] / (r64["hex_ms"] * 1e-3) / 1e9
@@ -383,6 +384,7 @@ def main():
         "final_time": r64["ts"].tt,
         "kernel_variants_fp64": {"hex": r64["variants"][0], "q4": r64["variants"][1]},
         "footprint_gb_fp64": r64["footprint"] / 1e9,
+        "assembly_plan": r64["plan"],
     }
     out["hbm_copy"] = stream_copy()
     out["roofline"]["frac_of_copy"] = hex_gbs / out["hbm_copy"]["gbs"]
@@ -393,7 +395,7 @@ def main():
         eus = ne * args.steps / rs["wall"]
         derr = float(np.abs(rs["d"] - r64["d"]).max() / np.abs(r64["d"]).max())
         eerr = float(abs(rs["energy"].sum() - r64["energy"].sum()) / abs(r64["energy"].sum()))
-        ts32 = T.cycle_bytes(model, "split")
+        ts32 = T.cycle_bytes(model, "split", rs["plan"])
         cyc32 = rs["hex_ms"] + rs["q4_ms"] + rs["node_ms"]
         out["split_form"] = {
             "value": eus, "speedup_vs_fp64": eus / eu64, "ms_per_step": 1e3 * rs["wall"] / args.steps,

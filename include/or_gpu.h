@@ -175,6 +175,11 @@ int or_model_kernel_times(or_model *m, int reset, double *hex_ms, double *q4_ms,
 int or_model_set_timing(or_model *m, int enable);
 /* Bytes of device memory held by the model. */
 int or_model_footprint(const or_model *m, int64_t *bytes);
+/* Diagnostic, no counterpart in the reference: the assembly plan of the uploaded skyline,
+ * out = {hex prefixes, hex rest rows, shell prefixes, shell rest rows}.  A node's leading
+ * rows that share the tile of its first element are summed in that tile's LDS into one
+ * prefix; the other rows ("rest") are stored and summed by the node kernel. */
+int or_model_assembly_stats(const or_model *m, int64_t out[4]);
 /* An upper bound of the bytes a model will hold once created and uploaded, computed
  * without allocating anything (admission by HBM footprint); it counts every skyline
  * row as a stored row, where the element tiles sum most of them in LDS.  strain_rate_filter: the shells' law has

@@ -241,49 +241,116 @@ __device__ __forceinline__ void lds_corner(S *lds, int k, const S (&w)[W]) {
   for (int c = 0; c < W; ++c) lds[(k * W + c) * kTile + threadIdx.x] = w[c];
 }
 
-// The rest rows of this thread's element (corners whose rest row is >= 0).
-template <typename S, int W, int NK>
-__device__ __forceinline__ void tile_rest_rows(const S *lds, gptr<const int32_t> rest, int64_t ne, int64_t i,
-                                               gptr<S> rows) {
-  if (i >= ne) return;
+// The tile's assembly after its element chains: this thread's rest rows (the corners of its
+// element whose rest row is >= 0, written from LDS) and the tile's prefixes (each summed in
+// row order from zero in FP64, ASSPAR4's running sum A = A + FSKY, stored in pre[n][W]),
+// thread t taking prefixes t, t + kTile, ...  The indices are loaded before the tile's
+// barrier (prefetch), so that their latency overlaps the wait for the tile's other waves.
+template <typename S, int W, int NK> struct TileAssembly {
+  int32_t rest[NK];
+  int32_t j, j1, node;
+  v4u src;
+
+  __device__ __forceinline__ void prefetch(gptr<const int32_t> restp, int64_t ne, int64_t i,
+                                           gptr<const int32_t> fr_start, gptr<const int32_t> fr_node,
+                                           gptr<const v4u> fr_src, int64_t tile) {
+    __asm__ __volatile__("" ::: "memory"); // after the element chain, not hoisted into it
 #pragma unroll
-  for (int k = 0; k < NK; ++k) {
-    const int32_t r = rest[(int64_t)k * ne + i];
-    if (r >= 0) {
-      const gptr<S> w = rows + (int64_t)r * W;
-#pragma unroll
-      for (int c = 0; c < W; ++c) w[c] = lds[(k * W + c) * kTile + threadIdx.x];
+    for (int k = 0; k < NK; ++k) rest[k] = (i < ne) ? restp[(int64_t)k * ne + i] : -1;
+    j = fr_start[tile] + (int32_t)threadIdx.x;
+    j1 = fr_start[tile + 1];
+    if (j < j1) {
+      node = fr_node[j];
+      src = fr_src[j];
     }
   }
-}
 
-// After the tile's corners are in LDS (and a barrier): every prefix of the tile, summed
-// in row order from zero in FP64 (ASSPAR4's running sum: A = A + FSKY), stored in pre[n][W].
-template <typename S, int W>
-__device__ __forceinline__ void tile_prefixes(const S *lds, gptr<const int32_t> fr_start, gptr<const int32_t> fr_node,
-                                              gptr<const v4u> fr_src, int64_t tile, gptr<double> pre) {
-  const int32_t j0 = fr_start[tile], j1 = fr_start[tile + 1];
-  for (int32_t j = j0 + (int32_t)threadIdx.x; j < j1; j += kTile) {
-    const int32_t n = fr_node[j];
-    const v4u src = fr_src[j];
-    double a[W];
+  __device__ __forceinline__ void finish(const S *lds, gptr<S> rows, gptr<const int32_t> fr_node,
+                                         gptr<const v4u> fr_src, gptr<double> pre) {
 #pragma unroll
-    for (int c = 0; c < W; ++c) a[c] = 0.0;
+    for (int k = 0; k < NK; ++k) {
+      if (rest[k] >= 0) {
+        const gptr<S> w = rows + (int64_t)rest[k] * W;
 #pragma unroll
-    for (int r = 0; r < orasm::kRunCap; ++r) {
-      const uint32_t u = (src[r >> 1] >> (16 * (r & 1))) & 0xFFFFu;
-      if (u == orasm::kNoSrc) break;
-      const int base = (int)(u >> 8) * W * kTile + (int)(u & 0xFFu);
-#pragma unroll
-      for (int c = 0; c < W; ++c) a[c] = a[c] + double(lds[base + c * kTile]);
+        for (int c = 0; c < W; ++c) w[c] = lds[(k * W + c) * kTile + threadIdx.x];
+      }
     }
-    const gptr<double> p = pre + (int64_t)n * W;
+    for (; j < j1; j += kTile) {
+      double a[W];
 #pragma unroll
-    for (int c = 0; c < W; ++c) p[c] = a[c];
+      for (int c = 0; c < W; ++c) a[c] = 0.0;
+#pragma unroll
+      for (int r = 0; r < orasm::kRunCap; ++r) {
+        const uint32_t u = (src[r >> 1] >> (16 * (r & 1))) & 0xFFFFu;
+        if (u == orasm::kNoSrc) break;
+        const int base = (int)(u >> 8) * W * kTile + (int)(u & 0xFFu);
+#pragma unroll
+        for (int c = 0; c < W; ++c) a[c] = a[c] + double(lds[base + c * kTile]);
+      }
+      const gptr<double> p = pre + (int64_t)node * W;
+#pragma unroll
+      for (int c = 0; c < W; ++c) p[c] = a[c];
+      if (j + kTile < j1) { // the tile holds more prefixes than threads
+        node = fr_node[j + kTile];
+        src = fr_src[j + kTile];
+      }
+    }
+  }
+};
+
+// The element time-step minimum of a tile: each wave's minimum goes to red[] before the
+// tile's barrier, and after it the first thread folds them into the model's word with an
+// integer atomicMin on the FP64 bits (min is exact, so the order does not matter).
+__device__ __forceinline__ void wave_min_to(double *red, double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = (o < v) ? o : v;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+}
+template <int TB> __device__ __forceinline__ void tile_min_commit(const double *red, unsigned long long *word) {
+  if (threadIdx.x == 0) {
+    double v = red[0];
+#pragma unroll
+    for (int w = 1; w < TB / 64; ++w) v = (red[w] < v) ? red[w] : v;
+    atomicMin(word, dbits(v));
   }
 }
 
 // ---------------------------------------------------------------- hex element kernel
+// In the split form, the hex element state read by the material phase (SIG(6), EINT, RHO,
+// VOL0, PLA, EPSD, QVIS) is copied from HBM to LDS by the wave itself at the start of the chain, with
+// global-to-LDS loads that occupy no registers: the copy is in flight during the gather,
+// the geometry and the kinematics, and the material phase reads LDS.  Word a of the
+// tile's element t sits at sst[a * kTile + t]; each wave copies its own 64 elements, so
+// only the wave's own wait for its loads orders the copy before its reads.
+constexpr int kHexStagedWords = 12;
+typedef __attribute__((address_space(3))) void *lds_void;
+
+template <typename S>
+__device__ __forceinline__ void stage_hex_state(S *sst, const HexPtrs<S> &e, int64_t first) {
+  constexpr int D = sizeof(S) / 4; // dwords per word
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t wfirst = first + 64 * w;
+  const gptr<const S> src[kHexStagedWords] = {e.sig,          e.sig + e.ne,     e.sig + 2 * e.ne, e.sig + 3 * e.ne,
+                                              e.sig + 4 * e.ne, e.sig + 5 * e.ne, e.eint,           e.rho,
+                                              e.vol0,         e.pla,            e.epsd,           e.qvis};
+#pragma unroll
+  for (int a = 0; a < kHexStagedWords; ++a) {
+#pragma unroll
+    for (int h = 0; h < D; ++h) {
+      const int d = h * 64 + l; // dword of the wave's slice
+      if (wfirst + d / D < e.ne)
+        __builtin_amdgcn_global_load_lds((gptr<void>)((gptr<const char>)(src[a] + wfirst) + 4 * d),
+                                         (lds_void)((char *)(sst + a * kTile + 64 * w) + 256 * h), 4, 0, 0);
+    }
+  }
+}
+
+// The wave's global-to-LDS loads have landed (they count as vector memory loads).
+__device__ __forceinline__ void wait_staged() { __asm__ __volatile__("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 template <typename R, typename S, int TB, int W>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_hex_forces(
     const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity) {
@@ -301,6 +368,11 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   const int64_t i = (int64_t)tl.first + threadIdx.x;
   double dtx = kEP06;
   int mstop = 0;
+  // the split form stages its element state through LDS; the FP64 chain, bound by its
+  // arithmetic at two waves per SIMD, reads it from HBM after the kinematics
+  constexpr bool kStage = sizeof(S) == 4;
+  __shared__ S sst[kStage ? kHexStagedWords * kTile : 1];
+  if constexpr (kStage) stage_hex_state<S>(sst, e, tl.first);
   if (i < e.ne) {
     const R DT1 = R(time_dt2(ts + parity));
     HexState<S> st;
@@ -319,11 +391,20 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
     // live through the geometry and the kinematics
     st.offg = e.offg[i];
     auto load_rest = [&]() {
-      __asm__ __volatile__("" ::: "memory");
+      if constexpr (kStage) {
+        wait_staged();
+        const S *s = sst + threadIdx.x;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) st.sig[k] = e.sig[(int64_t)k * e.ne + i];
-      st.eint = e.eint[i]; st.rho = e.rho[i]; st.vol0 = e.vol0[i]; st.pla = e.pla[i];
-      st.epsd = e.epsd[i]; st.qvis = e.qvis[i];
+        for (int k = 0; k < 6; ++k) st.sig[k] = s[k * kTile];
+        st.eint = s[6 * kTile]; st.rho = s[7 * kTile]; st.vol0 = s[8 * kTile]; st.pla = s[9 * kTile];
+        st.epsd = s[10 * kTile]; st.qvis = s[11 * kTile];
+      } else {
+        __asm__ __volatile__("" ::: "memory");
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st.sig[k] = e.sig[(int64_t)k * e.ne + i];
+        st.eint = e.eint[i]; st.rho = e.rho[i]; st.vol0 = e.vol0[i]; st.pla = e.pla[i];
+        st.epsd = e.epsd[i]; st.qvis = e.qvis[i];
+      }
     };
     if constexpr (sizeof(R) == 8) {
       // FP64: each node's force triple leaves for LDS as soon as it is final, which keeps
@@ -354,13 +435,15 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
     mstop = o.mstop;
   }
   if (stopped) return; // uniform over the workgroup
-  __syncthreads();
-  tile_rest_rows<S, kHexRowWords, 8>(lds, e.rest, e.ne, i, to_global<S>(M.rowh));
-  tile_prefixes<S, kHexRowWords>(lds, to_global(M.hfr_start), to_global(M.hfr_node),
-                                 to_global<const v4u>(M.hfr_src), tl.first / kTile, to_global(M.preh));
+  __shared__ double red[TB / 64];
+  TileAssembly<S, kHexRowWords, 8> ta;
+  ta.prefetch(e.rest, e.ne, i, to_global(M.hfr_start), to_global(M.hfr_node), to_global<const v4u>(M.hfr_src),
+              tl.first / kTile);
+  wave_min_to(red, dtx);
   if (mstop) atomicOr(M.flags, kMstopJacobian);
-  const double m = block_min<TB>(dtx);
-  if (threadIdx.x == 0) atomicMin(M.dtmin + parity, dbits(m));
+  __syncthreads();
+  tile_min_commit<TB>(red, M.dtmin + parity);
+  ta.finish(lds, to_global<S>(M.rowh), to_global(M.hfr_node), to_global<const v4u>(M.hfr_src), to_global(M.preh));
 }
 
 // ---------------------------------------------------------------- Q4 element kernel
@@ -467,12 +550,14 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
     dt = o.dt;
   }
   if (stopped) return; // uniform over the workgroup
+  __shared__ double red[TB / 64];
+  TileAssembly<S, kQ4RowWords, 4> ta;
+  ta.prefetch(q.rest, q.ne, i, to_global(M.qfr_start), to_global(M.qfr_node), to_global<const v4u>(M.qfr_src),
+              tl.first / kTile);
+  wave_min_to(red, dt);
   __syncthreads();
-  tile_rest_rows<S, kQ4RowWords, 4>(lds, q.rest, q.ne, i, to_global<S>(M.rowq));
-  tile_prefixes<S, kQ4RowWords>(lds, to_global(M.qfr_start), to_global(M.qfr_node), to_global<const v4u>(M.qfr_src),
-                                tl.first / kTile, to_global(M.preq));
-  const double m = block_min<TB>(dt);
-  if (threadIdx.x == 0) atomicMin(M.dtmin + parity, dbits(m));
+  tile_min_commit<TB>(red, M.dtmin + parity);
+  ta.finish(lds, to_global<S>(M.rowq), to_global(M.qfr_node), to_global<const v4u>(M.qfr_src), to_global(M.preq));
 }
 
 // ---------------------------------------------------------------- nodal update pieces
@@ -1024,6 +1109,7 @@ struct or_model {
   // element storage order of each family: slot -> element (or_assembly.h tile_order); empty
   // until the first element upload, fixed after it
   std::vector<int32_t> hperm, qperm;
+  int64_t asm_stats[4] = {0, 0, 0, 0}; // prefixes and rest rows of the hex and shell plans
   // hex (storage type depends on precision)
   int32_t *hconn = nullptr, *hrest = nullptr, *hfr_start = nullptr, *hfr_node = nullptr;
   uint16_t *hfr_src = nullptr;
@@ -1566,6 +1652,8 @@ int or_model_upload_skyline(or_model *m, const int64_t *adsky, const int64_t *ia
     orasm::FamilyPlan plan;
     orasm::plan_family(plan, nk, ne, nn, f == 0 ? m->hperm : m->qperm, rptr, fe, fk, ok);
     const int64_t nfr = (int64_t)plan.fr_node.size();
+    m->asm_stats[2 * f] = nfr;
+    m->asm_stats[2 * f + 1] = plan.nrest;
     void **rows = f == 0 ? &m->rowh : &m->rowq;
     int32_t **frn = f == 0 ? &m->hfr_node : &m->qfr_node;
     uint16_t **frs = f == 0 ? &m->hfr_src : &m->qfr_src;
@@ -1990,6 +2078,13 @@ int or_model_footprint_estimate(int64_t nn, int64_t ne_hex, int64_t ne_q4, int32
     b += ss * nrow * (f == 0 ? kHexRowWords : kQ4RowWords) + (4 + 2 * orasm::kRunCap) * nodes;
   }
   *bytes = b;
+  return OR_OK;
+}
+
+int or_model_assembly_stats(const or_model *m, int64_t out[4]) {
+  if (!m || !out) return fail(OR_ERR_ARG, "null argument");
+  if (!m->have_sky) return fail(OR_ERR_STATE, "upload the skyline first");
+  for (int k = 0; k < 4; ++k) out[k] = m->asm_stats[k];
   return OR_OK;
 }
 

@@ -230,6 +230,14 @@ class GpuModel:
                                                 C.byref(c)))
         return h.value, q.value, n.value, c.value
 
+    def assembly_stats(self) -> dict:
+        """The assembly plan: prefixes (a node's leading rows summed in its first tile's LDS) and
+        rest rows (stored, summed by the node kernel) of each family."""
+        out = np.zeros(4, np.int64)
+        L.check(self._lib.or_model_assembly_stats(self._h, _p(out)))
+        return dict(hex_prefixes=int(out[0]), hex_rest_rows=int(out[1]), q4_prefixes=int(out[2]),
+                    q4_rest_rows=int(out[3]))
+
     def footprint(self) -> int:
         b = C.c_int64()
         L.check(self._lib.or_model_footprint(self._h, C.byref(b)))

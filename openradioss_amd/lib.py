@@ -32,7 +32,7 @@ EXPORTS = (
     "or_model_energy", "or_model_kernel_times", "or_model_set_timing", "or_model_footprint", "or_model_footprint_estimate", "or_sforc3_group",
     "or_cforc3_group", "or_nodal_update", "or_batch_create", "or_batch_destroy", "or_batch_add", "or_batch_remove", "or_batch_run",
     "or_batch_sync", "or_batch_active", "or_model_set_variant", "or_model_autotune", "or_batch_autotune",
-    "or_math_eval", "or_sforc3_group_static",
+    "or_math_eval", "or_sforc3_group_static", "or_model_assembly_stats",
 )
 
 
@@ -88,6 +88,7 @@ _SIGS = {
     "or_model_kernel_times": ([_P, C.c_int, _P, _P, _P, _P], C.c_int),
     "or_model_set_timing": ([_P, C.c_int], C.c_int),
     "or_model_footprint": ([_P, _P], C.c_int),
+    "or_model_assembly_stats": ([_P, _P], C.c_int),
     "or_model_footprint_estimate": ([C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_int, C.c_int, _P, _P, _P],
                                     C.c_int),
     "or_sforc3_group": ([C.c_int, _P, C.c_int64, C.c_int32, C.c_int64] + [_P] * 11 + [C.c_int64, _P, _P,

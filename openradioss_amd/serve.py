@@ -113,7 +113,8 @@ class Server:
             req.ncycle = int(ts.ncycle)
             if self.keep_results:
                 x, v, vr, d = g.nodes6()
-                req.result = dict(d=d, v=v, vr=vr, tt=ts.tt, ncycle=int(ts.ncycle), energy=g.energy())
+                req.result = dict(d=d, v=v, vr=vr, tt=ts.tt, ncycle=int(ts.ncycle), mstop=int(ts.mstop),
+                                  energy=g.energy())
             g.close()
             req.finished = now_fn()
             self.element_updates += req.ne * req.ncycle
