@@ -127,23 +127,52 @@ def cpu_baseline(model, budget_s: float = 12.0):
                        f"{n} cycles after 1 warm-up cycle, {el:.1f} s")
 
 
-def small_leg(cycles: int = 1000) -> dict:
-    """The small reference workloads (bar 20x20x80, 32,000 hexes; tube 60x300, 72,000 shells), FP64,
-    one run(cycles) call after 20 warm-up cycles: at these sizes a cycle is bound by the three
-    dependent kernel launches, not by the kernels' work."""
+def small_leg(cycles: int = 1000) -> tuple[dict, dict]:
+    """The small reference workloads (bar 20x20x80, 32,000 hexes; tube 60x300, 72,000 shells), FP64:
+    `cycles` timed cycles from the initial state (after 20 warm-up cycles on a separate instance);
+    at these sizes a cycle is bound by the three dependent kernel launches, not by the kernels' work.
+    Then each run continues to the length of its reference fixture (bar 1,000 cycles, tube 8,800)
+    and is compared with the state the reference's own compiled routines (glibc libm) reached
+    there: the parity block (displacement and energy deviations)."""
+    import numpy as np
+
     from openradioss_amd import engine as E
     from openradioss_amd import model as M
-    out = {}
-    for name, m in (("bar_20x20x80", M.hex_model_to_model(M.cantilever_bar(20, 20, 80))),
-                    ("tube_60x300", M.square_tube())):
+    gold = os.path.join(ROOT, "tests", "golden")
+    out, parity = {}, {}
+    for name, build, full, fixture in (
+            ("bar_20x20x80", lambda: M.hex_model_to_model(M.cantilever_bar(20, 20, 80)), 1000,
+             "bar_20x20x80_c1000.npz"),
+            ("tube_60x300", M.square_tube, 8800, "tube_60x300_c8800.npz")):
+        m = build()
         ne = m.ne_hex + m.ne_q4
         with E.GpuModel(m, precision="fp64") as g:
             g.run(20)
+        with E.GpuModel(m, precision="fp64") as g:
             t0 = time.perf_counter()
             g.run(cycles)
             t = time.perf_counter() - t0
-        out[name] = dict(elements=ne, cycles=cycles, us_per_cycle=1e6 * t / cycles, element_updates_per_s=ne * cycles / t)
-    return out
+            out[name] = dict(elements=ne, cycles=cycles, us_per_cycle=1e6 * t / cycles,
+                             element_updates_per_s=ne * cycles / t)
+            g.run(full - cycles)
+            ts = g.time_state()
+            x, v, vr, d = g.nodes6()
+            e = g.energy()
+        try:
+            ref = np.load(os.path.join(gold, fixture))
+        except OSError:
+            continue
+        every = int(ref["d_every"][0])
+        dref = ref["d_sub"]
+        r = dict(cycles=int(ts.ncycle), reference=f"tests/golden/{fixture} (the reference's Fortran, glibc libm)",
+                 d_rel_err=float(np.abs(d[::every] - dref).max() / np.abs(dref).max()),
+                 time_rel_err=float(abs(ts.tt - ref["time"][0]) / ref["time"][0]))
+        if "energy" in ref.files:
+            er = ref["energy"]
+            r["energy_rel_err"] = float(abs(e.sum() - er.sum()) / abs(er.sum()))
+            r["internal_energy_rel_err"] = float(abs(e[0] - er[0]) / abs(er[0]))
+        parity[name] = r
+    return out, parity
 
 
 def sweep_leg(precision: str, cycles: int, warmup: int, n: int = 256):
@@ -410,9 +439,12 @@ def main():
             "kernel_variants": {"hex": rs["variants"][0], "q4": rs["variants"][1]},
             "dtype": "f32 element / f64 node"}
         log(f"split: {eus:.3e} element-updates/s (x{eus / eu64:.2f}), disp err {derr:.2e}, energy err {eerr:.2e}")
-    out["small_models"] = small_leg()
+    out["small_models"], out["parity"] = small_leg()
     for k, r in out["small_models"].items():
         log(f"{k}: {r['element_updates_per_s']:.3e} element-updates/s, {r['us_per_cycle']:.1f} us per cycle")
+    for k, r in out["parity"].items():
+        log(f"parity {k} at cycle {r['cycles']}: displacement {r['d_rel_err']:.2e}, "
+            f"energy {r.get('energy_rel_err', float('nan')):.2e} against the reference")
     if not args.no_sweep:
         out["batch_sweep"] = {}
         for prec in ("fp64", "split"):

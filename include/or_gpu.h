@@ -160,6 +160,22 @@ int or_model_sync(or_model *m);
  * CBILAN EINT(1)+EINT(2)), out[1] translational kinetic sum (V+DT05*A)^2*HALF*MS,
  * out[2] rotational kinetic sum (VR+DT05*AR)^2*HALF*IN, DT05 = HALF*DT1.  Synchronous. */
 int or_model_run_output(or_model *m, double out[3]);
+/* What ECRIT prints for one output cycle (ecrit.F:651, FORMAT 1100 at ecrit.F:704).
+ * wfext: OUTPUT%TH%WFEXT, the external work of the imposed velocities since the last node
+ * upload, FIXVEL's DT1*DW per imposed DOF (fixvel.F:376-378, 834), accumulated per node and
+ * summed over the nodes.  itypts / nelts: the element that controls the cycle's time step
+ * (ITYPTS 1 solid, 3 shell, 0 none; NELTS as a 0-based index into that family's element
+ * arrays), chosen as MQVISCB and CDT3 choose it (mqviscb.F:622-630, cdt3.F:205-215): the last
+ * solid with the smallest time step, unless a shell's is smaller, then the first such shell. */
+typedef struct {
+  double enint, encin, enrot, wfext;
+  int32_t itypts, pad;
+  int64_t nelts;
+} or_ecrit;
+/* or_model_run_output with the listing's external work and controlling element.  Synchronous. */
+int or_model_run_ecrit(or_model *m, or_ecrit *out);
+/* The external work of the imposed velocities so far (as in or_ecrit.wfext).  Synchronous. */
+int or_model_wfext(or_model *m, double *wfext);
 /* Any output pointer may be NULL. */
 int or_model_download_nodes(or_model *m, double *x, double *v, double *vr, double *d);
 int or_model_download_hex(or_model *m, double *sig, double *eint, double *rho, double *pla,

@@ -13,7 +13,7 @@
 // The file is compiled twice: once with the compiler's default instruction scheduler
 // (everything), and once with -DOR_ILP_TU and the ILP-first scheduler
 // (-mllvm -amdgpu-sched-strategy=max-ilp), which only provides the element-kernel launches
-// of variants 4..7 (or_ilp_elements).  Scheduling reorders instructions and never changes
+// of variants 2 and 3 (or_ilp_elements).  Scheduling reorders instructions and never changes
 // an operation or its operands, so every variant gives the same bits.
 #include <hip/hip_runtime.h>
 
@@ -152,6 +152,8 @@ struct NodePtrs {
   const int32_t *deg_h, *deg_q;    // [nn] rows of each node per family (null when the family is absent)
   const int32_t *rs_h, *rs_q;      // [nn + 1] first rest row of each node per family
   const double *pre_h, *pre_q;     // [nn][3], [nn][6] FP64 prefixes
+  double *wext;                    // [2][nn] external work of the node's imposed DOFs, and the part
+                                   // each cycle defers to the next (FIXVEL's VEL(4,N)), or null
   int64_t nn;
 };
 
@@ -180,6 +182,7 @@ struct ModelDev {
   unsigned long long *dtmin;  // [2] element time-step minimum (FP64 bits), by parity
   int *flags;                 // bit 0: an element reported MSTOP
   double *ecrit;              // [2][nn] per-node kinetic terms of ECRIT on output cycles, or null
+  double *dtel_h, *dtel_q;    // per-element time-step candidates on output cycles (storage order), or null
   double dtmx;                // DTMX
   double tend;                // the model stops after the cycle whose time reaches tend
   HexConsts hc;
@@ -353,8 +356,11 @@ __device__ __forceinline__ void wait_staged() { __asm__ __volatile__("s_waitcnt 
 
 template <typename R, typename S, int TB, int W>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_hex_forces(
-    const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity) {
+    const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity_output) {
   static_assert(TB == kTile, "a hex workgroup is one assembly tile");
+  // bit 0: cycle parity; bit 1: an output cycle (each element's time-step candidate is kept)
+  const int parity = parity_output & 1;
+  const bool output = (parity_output & 2) != 0;
   __shared__ S lds[8 * kHexRowWords * kTile];
   const Tile tl = tiles[xcd_contiguous_block(blockIdx.x, gridDim.x)];
   const ModelDev &M = md[tl.model];
@@ -430,6 +436,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
       for (int k = 0; k < 6; ++k) e.sig[(int64_t)k * e.ne + i] = st.sig[k];
       e.eint[i] = st.eint; e.rho[i] = st.rho; e.pla[i] = st.pla;
       e.epsd[i] = st.epsd; e.offg[i] = st.offg; e.qvis[i] = st.qvis;
+      if (output && M.dtel_h) to_global(M.dtel_h)[i] = o.dtx;
     }
     dtx = o.dtx;
     mstop = o.mstop;
@@ -498,8 +505,10 @@ template <typename R, typename S> struct Q4IpIO {
 
 template <typename R, typename S, int TB, int W>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_q4_forces(
-    const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity) {
+    const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity_output) {
   static_assert(TB == kTile, "a shell workgroup is one assembly tile");
+  const int parity = parity_output & 1; // bit 1: an output cycle, as for the hex kernel
+  const bool output = (parity_output & 2) != 0;
   __shared__ S lds[4 * kQ4RowWords * kTile];
   const Tile tl = tiles[xcd_contiguous_block(blockIdx.x, gridDim.x)];
   const ModelDev &M = md[tl.model];
@@ -546,6 +555,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
         const S w[6] = {S(-o.f[0][k]), S(-o.f[1][k]), S(-o.f[2][k]), S(-o.m[0][k]), S(-o.m[1][k]), S(-o.m[2][k])};
         lds_corner<S, 6>(lds, k, w);
       }
+      if (output && M.dtel_q) to_global(M.dtel_q)[i] = o.dt;
     }
     dt = o.dt;
   }
@@ -576,16 +586,30 @@ __device__ __forceinline__ void accele3(double &a0, double &a1, double &a2, doub
 template <typename P>
 __device__ __forceinline__ void bcs_fixvel(int32_t code, double &ax, double &ay, double &az, double &vx, double &vy,
                                            double &vz, double &rx, double &ry, double &rz, double &wx, double &wy,
-                                           double &wz, P vimp, double DT12) {
+                                           double &wz, P vimp, double DT12, double DT1, double DT2, double msn,
+                                           double &work, double &pend) {
   if (code & kTX) { ax = 0.0; vx = 0.0; }
   if (code & kTY) { ay = 0.0; vy = 0.0; }
   if (code & kTZ) { az = 0.0; vz = 0.0; }
   if (code & kRX) { rx = 0.0; wx = 0.0; }
   if (code & kRY) { ry = 0.0; wy = 0.0; }
   if (code & kRZ) { rz = 0.0; wz = 0.0; }
-  if (code & kIX) ax = (vimp[0] - vx) / DT12;
-  if (code & kIY) ay = (vimp[1] - vy) / DT12;
-  if (code & kIZ) az = (vimp[2] - vz) / DT12;
+  // the imposed DOF's work (fixvel.F:376-378, 834, 837): DW = FOURTH*MS*AXI*WEIGHT*(A*DT12 +
+  // TWO*V)*(A - AOLD) with AXI = WEIGHT = 1; DT1*DW enters WFEXT now, DT2*DW (kept per node in
+  // pend) at the next cycle, before the DOF's new work (fixvel.F:342-344)
+  const double carried = pend;
+  work = work + carried;
+  pend = 0.0;
+  auto imposed = [&](double &a, double v, double vi) {
+    const double aold = a;
+    a = (vi - v) / DT12;
+    const double dw = ((1.0 / 4.0) * msn) * (a * DT12 + 2.0 * v) * (a - aold);
+    work = work + DT1 * dw;
+    pend = pend + DT2 * dw;
+  };
+  if (code & kIX) imposed(ax, vx, vimp[0]);
+  if (code & kIY) imposed(ay, vy, vimp[1]);
+  if (code & kIZ) imposed(az, vz, vimp[2]);
 }
 
 // The DT2 pipeline (resol.F:2826, 6125-6129, 6352, 6494-6497): DT2 from the element
@@ -697,7 +721,17 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
   const gptr<v2d> p = (gptr<v2d>)to_global(nd.xv + 6 * n);
   const v2d pa = p[0], pb = p[1], pc = p[2];
   double x = pa.x, y = pa.y, z = pb.x, vx = pb.y, vy = pc.x, vz = pc.y;
-  if (code) bcs_fixvel(code, ax, ay, az, vx, vy, vz, rx, ry, rz, wx, wy, wz, to_global(nd.vimp) + 3 * n, DT12);
+  if (code) {
+    // the node's external work (WFEXT) accumulates in wext[n] when its DOFs are imposed
+    const bool imp = (code & (kIX | kIY | kIZ)) && nd.wext;
+    double work = imp ? to_global(nd.wext)[n] : 0.0, pend = imp ? to_global(nd.wext)[nd.nn + n] : 0.0;
+    bcs_fixvel(code, ax, ay, az, vx, vy, vz, rx, ry, rz, wx, wy, wz, to_global(nd.vimp) + 3 * n, DT12, DT1, DT2,
+               msn, work, pend);
+    if (imp) {
+      to_global(nd.wext)[n] = work;
+      to_global(nd.wext)[nd.nn + n] = pend;
+    }
+  }
   if (output && M.ecrit) {
     // ECRIT's kinetic energies at its output point, between FIXVEL and VELOCITY
     // (sortie_main.F:368 from resol.F:8523): V + DT05*A with DT05 = HALF*DT1 (ecrit.F:193,
@@ -756,11 +790,12 @@ template <typename S>
 __global__ __launch_bounds__(kBlock) void k_energy_partial(const S *heint, const S *hvol0, int64_t neh,
                                                            const S *qeint, int64_t neq, const double *xv,
                                                            const double *vr, const double *ms, const double *in,
-                                                           const double *ecrit, int64_t nn, int64_t per_block,
-                                                           double *partial) {
-  double ie = 0.0, ke = 0.0, kr = 0.0;
+                                                           const double *ecrit, const double *wext, int64_t nn,
+                                                           int64_t per_block, double *partial) {
+  double ie = 0.0, ke = 0.0, kr = 0.0, we = 0.0;
   const int64_t b0 = (int64_t)blockIdx.x * per_block;
   for (int64_t j = b0 + threadIdx.x; j < b0 + per_block; j += kBlock) {
+    if (j < nn && wext) we = we + wext[j];
     if (j < neh) ie = ie + double(heint[j]) * double(hvol0[j]);
     if (j < neq) ie = ie + (double(qeint[j]) + double(qeint[neq + j]));
     if (j < nn && ecrit) {
@@ -779,28 +814,65 @@ __global__ __launch_bounds__(kBlock) void k_energy_partial(const S *heint, const
   ie = block_sum(ie);
   ke = block_sum(ke);
   kr = block_sum(kr);
+  we = block_sum(we);
   if (threadIdx.x == 0) {
-    partial[3 * blockIdx.x + 0] = ie;
-    partial[3 * blockIdx.x + 1] = ke;
-    partial[3 * blockIdx.x + 2] = kr;
+    partial[4 * blockIdx.x + 0] = ie;
+    partial[4 * blockIdx.x + 1] = ke;
+    partial[4 * blockIdx.x + 2] = kr;
+    partial[4 * blockIdx.x + 3] = we;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_energy_final(const double *partial, int nb, double kin_factor,
                                                          double *out) {
-  double ie = 0.0, ke = 0.0, kr = 0.0;
+  double ie = 0.0, ke = 0.0, kr = 0.0, we = 0.0;
   for (int j = threadIdx.x; j < nb; j += kBlock) {
-    ie = ie + partial[3 * j];
-    ke = ke + partial[3 * j + 1];
-    kr = kr + partial[3 * j + 2];
+    ie = ie + partial[4 * j];
+    ke = ke + partial[4 * j + 1];
+    kr = kr + partial[4 * j + 2];
+    we = we + partial[4 * j + 3];
   }
   ie = block_sum(ie);
   ke = block_sum(ke);
   kr = block_sum(kr);
+  we = block_sum(we);
   if (threadIdx.x == 0) {
     out[0] = ie;
     out[1] = kin_factor * ke;
     out[2] = kin_factor * kr;
+    out[3] = we;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(double *p, int64_t n, double v) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j < n) p[j] = v;
+}
+
+// The element that controls the time step of an output cycle, as MQVISCB and CDT3 name it
+// (NELTS, ITYPTS): with DT2T = EP06 on entry (resol.F:2826), the solids in element order
+// replace it when DTX <= DT2T (mqviscb.F:622-630), so the last solid with the smallest DTX;
+// then the shells when DT < DT2T (cdt3.F:205-215), so the first shell whose DT is below
+// every solid's.  Elements that cannot control the step carry EP06.  w[0], w[1]: the
+// minimum of each family (FP64 bits, integer atomicMin); w[2]: the largest solid index at
+// the solid minimum (atomicMax); w[3]: the smallest shell index at the shell minimum.
+__global__ __launch_bounds__(kBlock) void k_ctrl_min(const double *dt, int64_t ne, unsigned long long *w) {
+  double v = kEP06;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < ne; j += (int64_t)gridDim.x * kBlock)
+    v = (dt[j] < v) ? dt[j] : v;
+  const double b = block_min<kBlock>(v);
+  if (threadIdx.x == 0 && b < kEP06) atomicMin(w, dbits(b));
+}
+__global__ __launch_bounds__(kBlock) void k_ctrl_pick(const double *dt, const int32_t *logical, int64_t ne,
+                                                      const unsigned long long *wmin, int last,
+                                                      unsigned long long *pick) {
+  const double m = bitsd(*wmin);
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < ne; j += (int64_t)gridDim.x * kBlock) {
+    if (dt[j] == m && m < kEP06) {
+      const unsigned long long e = (unsigned long long)logical[j];
+      if (last) atomicMax(pick, e);
+      else atomicMin(pick, e);
+    }
   }
 }
 
@@ -961,7 +1033,10 @@ __global__ __launch_bounds__(kBlock) void k_nodal_dropin(int64_t nn, const int64
   }
   double vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
   const int32_t cd = code ? code[n] : 0;
-  if (cd) bcs_fixvel(cd, ax, ay, az, vx, vy, vz, rx, ry, rz, wx, wy, wz, vimp + 3 * n, DT12);
+  if (cd) {
+    double work = 0.0, pend = 0.0; // the drop-in reports no external work (the caller's FIXVEL keeps WFEXT)
+    bcs_fixvel(cd, ax, ay, az, vx, vy, vz, rx, ry, rz, wx, wy, wz, vimp + 3 * n, DT12, 0.0, 0.0, ms[n], work, pend);
+  }
   // VELOCITY (velocity.F:58-86)
   vx = vx + DT12 * ax; vy = vy + DT12 * ay; vz = vz + DT12 * az;
   v[3 * n] = vx; v[3 * n + 1] = vy; v[3 * n + 2] = vz;
@@ -1125,8 +1200,15 @@ struct or_model {
   DevTime *ts = nullptr;               // [2]
   unsigned long long *dtmin = nullptr; // [2]
   int *flags = nullptr;
-  double *energy = nullptr;            // [3 + 3*nb]
+  double *energy = nullptr;            // [4 + 4*nb]
   double *ecrit = nullptr;             // [2][nn] per-node ECRIT terms, allocated at the first output cycle
+  double *wext = nullptr;              // [2][nn] external work per node (and its deferred part), when
+                                       // the model has imposed DOFs
+  // the controlling element of output cycles: per-element candidates (storage order), the
+  // element index of each storage slot, and the scan's words; allocated at the first output cycle
+  double *dtel_h = nullptr, *dtel_q = nullptr;
+  int32_t *lidx_h = nullptr, *lidx_q = nullptr;
+  unsigned long long *ctrl = nullptr;  // [4]
   int64_t parity = 0;
   // what the kernels see (host copy, device copy) and the model's own tile lists
   ModelDev hdev{};
@@ -1293,7 +1375,7 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int6
   A(&m->tiles_q, (ne_q4 + kTile - 1) / kTile);
   A(&m->tiles_n, m->nt_n);
   const int64_t nb = 1024;
-  A(&m->energy, 3 + 3 * nb);
+  A(&m->energy, 4 + 4 * nb);
   if (rc == OR_OK && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(OR_ERR_HIP, "hipStreamCreate");
   if (rc == OR_OK) {
@@ -1346,7 +1428,8 @@ int or_model_destroy(or_model *m) {
                   m->qrest, m->qfr_start, m->qfr_node, m->qfr_src, m->qoffg, m->qthk, m->qthke, m->qforc, m->qmom,
                   m->qgstr, m->qhourg, m->qeint, m->qepsd,
                   m->qsig, m->qpla, m->qepsdip, m->rowq, m->ts, m->dtmin, m->flags, m->energy, m->dev,
-                  m->tiles_h, m->tiles_q, m->tiles_n, m->ecrit};
+                  m->tiles_h, m->tiles_q, m->tiles_n, m->ecrit, m->wext, m->dtel_h, m->dtel_q, m->lidx_h,
+                  m->lidx_q, m->ctrl};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
@@ -1415,6 +1498,16 @@ int or_model_upload_nodes(or_model *m, const double *x, const double *v, const d
     OR_HIP(hipMemcpyAsync(m->in, in, sizeof(double) * nn, hipMemcpyHostToDevice, m->stream));
   }
   OR_HIP(hipMemsetAsync(m->d, 0, sizeof(double) * 3 * nn, m->stream));
+  // external work of the imposed DOFs (FIXVEL's WFEXT), from zero at this upload
+  bool imposed = false;
+  for (int64_t n = 0; code && n < nn && !imposed; ++n) imposed = (code[n] & (kIX | kIY | kIZ)) != 0;
+  if (imposed && !m->wext) {
+    OR_HIP(quiesce(m));
+    OR_HIP(hipMalloc((void **)&m->wext, sizeof(double) * 2 * (size_t)nn));
+    m->bytes += (int64_t)(sizeof(double) * 2 * nn);
+    mark_changed(m);
+  }
+  if (m->wext) OR_HIP(hipMemsetAsync(m->wext, 0, sizeof(double) * 2 * (size_t)nn, m->stream));
   OR_HIP(quiesce(m));
   m->have_nodes = true;
   return OR_OK;
@@ -1744,10 +1837,11 @@ static void fill_dev(or_model *m, ModelDev &d, double tend) {
   d.qmom = m->qmom; d.qgstr = m->qgstr; d.qhourg = m->qhourg; d.qeint = m->qeint; d.qepsd = m->qepsd;
   d.qsig = m->qsig; d.qpla = m->qpla; d.qepsdip = m->qepsdip; d.qthke = m->qthke; d.neq = m->neq;
   d.nd = NodePtrs{m->xv, m->vr, m->d, m->ms, m->in, m->code, m->vimp, m->deg_h, m->deg_q, m->rs_h, m->rs_q,
-                  m->preh, m->preq, m->nn};
+                  m->preh, m->preq, m->wext, m->nn};
   d.rowh = m->rowh; d.rowq = m->rowq;
   d.preh = m->preh; d.preq = m->preq;
   d.ts = m->ts; d.dtmin = m->dtmin; d.flags = m->flags; d.ecrit = m->ecrit;
+  d.dtel_h = m->dtel_h; d.dtel_q = m->dtel_q;
   d.dtmx = m->neh > 0 ? m->hc.dtmx : 1.0e20;
   d.tend = tend;
   d.hc = m->hc;
@@ -1817,15 +1911,16 @@ template <typename R, typename S>
 static int launch_cycle(hipStream_t st, const LaunchSet &L, int parity, hipEvent_t after_hex, hipEvent_t after_q4,
                         int output = 0) {
   constexpr int fp64 = sizeof(R) == 8 ? 1 : 0;
+  const int po = parity | (output ? 2 : 0); // element kernels: parity and the output-cycle bit
   if (L.nth > 0) {
-    if (L.vh >= 2) or_ilp_elements(0, fp64, L.vh - 2, st, L.md, L.th, L.nth, parity);
-    else launch_hex<R, S>(L.vh, st, L, parity);
+    if (L.vh >= 2) or_ilp_elements(0, fp64, L.vh - 2, st, L.md, L.th, L.nth, po);
+    else launch_hex<R, S>(L.vh, st, L, po);
     OR_HIP(hipGetLastError());
   }
   if (after_hex) OR_HIP(hipEventRecord(after_hex, st));
   if (L.ntq > 0) {
-    if (L.vq >= 2) or_ilp_elements(1, fp64, L.vq - 2, st, L.md, L.tq, L.ntq, parity);
-    else launch_q4<R, S>(L.vq, st, L, parity);
+    if (L.vq >= 2) or_ilp_elements(1, fp64, L.vq - 2, st, L.md, L.tq, L.ntq, po);
+    else launch_q4<R, S>(L.vq, st, L, po);
     OR_HIP(hipGetLastError());
   }
   if (after_q4) OR_HIP(hipEventRecord(after_q4, st));
@@ -1875,9 +1970,33 @@ int or_model_run(or_model *m, int64_t ncycles) {
   return OR_OK;
 }
 
-static int energy_sums(or_model *m, const double *ecrit, double out[3]);
+static int energy_sums(or_model *m, const double *ecrit, double out[4]);
 
-int or_model_run_output(or_model *m, double out[3]) {
+// Per-element candidates of the controlling element and each storage slot's element index,
+// allocated (and the slot indices uploaded) at the first output cycle.
+static int ensure_ctrl(or_model *m) {
+  if (m->ctrl) return OR_OK;
+  OR_HIP(quiesce(m));
+  auto put = [&](double **dt, int32_t **lidx, const std::vector<int32_t> &perm, int64_t ne) -> int {
+    if (ne <= 0) return OR_OK;
+    std::vector<int32_t> li((size_t)ne);
+    for (int64_t s = 0; s < ne; ++s) li[(size_t)s] = perm.empty() ? (int32_t)s : perm[(size_t)s];
+    OR_HIP(hipMalloc((void **)dt, sizeof(double) * (size_t)ne));
+    OR_HIP(hipMalloc((void **)lidx, sizeof(int32_t) * (size_t)ne));
+    OR_HIP(hipMemcpy(*lidx, li.data(), sizeof(int32_t) * (size_t)ne, hipMemcpyHostToDevice));
+    m->bytes += (int64_t)(12 * ne);
+    return OR_OK;
+  };
+  int rc = put(&m->dtel_h, &m->lidx_h, m->hperm, m->neh);
+  if (rc == OR_OK) rc = put(&m->dtel_q, &m->lidx_q, m->qperm, m->neq);
+  if (rc) return rc;
+  OR_HIP(hipMalloc((void **)&m->ctrl, 4 * sizeof(unsigned long long)));
+  m->bytes += 32;
+  mark_changed(m);
+  return OR_OK;
+}
+
+int or_model_run_ecrit(or_model *m, or_ecrit *out) {
   if (!m || !out) return fail(OR_ERR_ARG, "null argument");
   if (!m->have_nodes || !m->have_sky || (m->neh > 0 && (!m->have_hparams || !m->have_hex)) ||
       (m->neq > 0 && (!m->have_qparams || !m->have_q4)))
@@ -1890,7 +2009,18 @@ int or_model_run_output(or_model *m, double out[3]) {
     m->bytes += (int64_t)(sizeof(double) * 2 * m->nn);
     mark_changed(m);
   }
-  int rc = refresh_dev(m);
+  int rc = ensure_ctrl(m);
+  if (rc) return rc;
+  // candidates of elements that cannot control the step (and of a stopped model) stay EP06
+  auto fill_ep06 = [&](double *p, int64_t n) -> int {
+    if (n > 0)
+      hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, m->stream, p, n,
+                         kEP06);
+    OR_HIP(hipGetLastError());
+    return OR_OK;
+  };
+  if ((rc = fill_ep06(m->dtel_h, m->neh)) || (rc = fill_ep06(m->dtel_q, m->neq))) return rc;
+  rc = refresh_dev(m);
   if (rc) return rc;
   LaunchSet L;
   L.md = m->dev;
@@ -1902,7 +2032,49 @@ int or_model_run_output(or_model *m, double out[3]) {
                                  : launch_cycle<float, float>(m->stream, L, p, nullptr, nullptr, 1);
   if (rc) return rc;
   m->parity ^= 1;
-  return energy_sums(m, m->ecrit, out);
+  double e[4];
+  if ((rc = energy_sums(m, m->ecrit, e))) return rc;
+  out->enint = e[0]; out->encin = e[1]; out->enrot = e[2]; out->wfext = e[3];
+  // the controlling element (k_ctrl_min / k_ctrl_pick)
+  unsigned long long w0[4];
+  const double big = kEP06;
+  std::memcpy(&w0[0], &big, 8);
+  w0[1] = w0[0];
+  w0[2] = 0;
+  w0[3] = ~0ull;
+  OR_HIP(hipMemcpyAsync(m->ctrl, w0, sizeof(w0), hipMemcpyHostToDevice, m->stream));
+  const unsigned nb = 1024;
+  if (m->neh > 0) {
+    hipLaunchKernelGGL(k_ctrl_min, dim3(nb), dim3(kBlock), 0, m->stream, m->dtel_h, m->neh, m->ctrl);
+    hipLaunchKernelGGL(k_ctrl_pick, dim3(nb), dim3(kBlock), 0, m->stream, m->dtel_h, m->lidx_h, m->neh, m->ctrl, 1,
+                       m->ctrl + 2);
+  }
+  if (m->neq > 0) {
+    hipLaunchKernelGGL(k_ctrl_min, dim3(nb), dim3(kBlock), 0, m->stream, m->dtel_q, m->neq, m->ctrl + 1);
+    hipLaunchKernelGGL(k_ctrl_pick, dim3(nb), dim3(kBlock), 0, m->stream, m->dtel_q, m->lidx_q, m->neq, m->ctrl + 1,
+                       0, m->ctrl + 3);
+  }
+  OR_HIP(hipGetLastError());
+  unsigned long long w[4];
+  OR_HIP(hipMemcpyAsync(w, m->ctrl, sizeof(w), hipMemcpyDeviceToHost, m->stream));
+  OR_HIP(quiesce(m));
+  double mh, mq;
+  std::memcpy(&mh, &w[0], 8);
+  std::memcpy(&mq, &w[1], 8);
+  out->itypts = 0;
+  out->pad = 0;
+  out->nelts = -1;
+  if (mh < kEP06) { out->itypts = 1; out->nelts = (int64_t)w[2]; }
+  if (mq < mh) { out->itypts = 3; out->nelts = (int64_t)w[3]; }
+  return OR_OK;
+}
+
+int or_model_run_output(or_model *m, double out[3]) {
+  or_ecrit e;
+  const int rc = or_model_run_ecrit(m, &e);
+  if (rc) return rc;
+  out[0] = e.enint; out[1] = e.encin; out[2] = e.enrot;
+  return OR_OK;
 }
 
 int or_model_sync(or_model *m) {
@@ -2013,36 +2185,50 @@ int or_model_download_q4(or_model *m, double *offg, double *thk, double *forc, d
 
 // Fixed-order reduction of the internal energy and of the kinetic terms: 0.5*MS*|V|^2 from the
 // resident state, or the per-node ECRIT terms of the last output cycle (ecrit != null).
-static int energy_sums(or_model *m, const double *ecrit, double out[3]) {
+static int energy_sums(or_model *m, const double *ecrit, double out[4]) {
   int64_t n = m->nn;
   if (m->neh > n) n = m->neh;
   if (m->neq > n) n = m->neq;
   const int nb = 1024;
   const int64_t per = (n + nb - 1) / nb;
   const int64_t per_block = ((per + kBlock - 1) / kBlock) * kBlock;
-  double *partial = m->energy + 3;
+  double *partial = m->energy + 4;
   OR_HIP(quiesce(m));
   if (m->prec == OR_PREC_FP64)
     hipLaunchKernelGGL((k_energy_partial<double>), dim3(nb), dim3(kBlock), 0, m->stream, (const double *)m->heint,
                        (const double *)m->hvol0, m->neh, (const double *)m->qeint, m->neq, m->xv, m->vr, m->ms, m->in,
-                       ecrit, m->nn, per_block, partial);
+                       ecrit, m->wext, m->nn, per_block, partial);
   else
     hipLaunchKernelGGL((k_energy_partial<float>), dim3(nb), dim3(kBlock), 0, m->stream, (const float *)m->heint,
                        (const float *)m->hvol0, m->neh, (const float *)m->qeint, m->neq, m->xv, m->vr, m->ms, m->in,
-                       ecrit, m->nn, per_block, partial);
+                       ecrit, m->wext, m->nn, per_block, partial);
   OR_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, m->stream, (const double *)partial, nb,
                      ecrit ? 1.0 : 0.5, m->energy);
   OR_HIP(hipGetLastError());
   OR_HIP(quiesce(m));
-  OR_HIP(hipMemcpy(out, m->energy, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  OR_HIP(hipMemcpy(out, m->energy, 4 * sizeof(double), hipMemcpyDeviceToHost));
   return OR_OK;
 }
 
 int or_model_energy(or_model *m, double out[3]) {
   if (!m || !out) return fail(OR_ERR_ARG, "null argument");
   OR_HIP(hipSetDevice(m->device));
-  return energy_sums(m, nullptr, out);
+  double e[4];
+  const int rc = energy_sums(m, nullptr, e);
+  if (rc) return rc;
+  out[0] = e[0]; out[1] = e[1]; out[2] = e[2];
+  return OR_OK;
+}
+
+int or_model_wfext(or_model *m, double *wfext) {
+  if (!m || !wfext) return fail(OR_ERR_ARG, "null argument");
+  OR_HIP(hipSetDevice(m->device));
+  double e[4];
+  const int rc = energy_sums(m, nullptr, e);
+  if (rc) return rc;
+  *wfext = e[3];
+  return OR_OK;
 }
 
 int or_model_footprint_estimate(int64_t nn, int64_t ne_hex, int64_t ne_q4, int32_t npt, int precision,
@@ -2058,6 +2244,7 @@ int or_model_footprint_estimate(int64_t nn, int64_t ne_hex, int64_t ne_q4, int32
   const int64_t th = (ne_hex + kTile - 1) / kTile, tq = (ne_q4 + kTile - 1) / kTile;
   // the allocations of or_model_create, in the same order
   int64_t b = 8 * (6 * nn + 3 * nn + nn + 3 * nn) + 4 * nn;             // xv, d, ms, vimp, code
+  b += 16 * nn;                                                          // wext (models with imposed DOFs)
   if (ne_q4 > 0) b += 8 * (4 * nn + nn);                                 // vr, in
   if (ne_hex > 0)                                                        // deg, rs, prefix, tiles, conn, rest, state
     b += 4 * (nn + nn + 1) + 8 * 3 * nn + 4 * (th + 1) + 4 * 16 * ne_hex + ss * 13 * ne_hex;
@@ -2066,7 +2253,7 @@ int or_model_footprint_estimate(int64_t nn, int64_t ne_hex, int64_t ne_q4, int32
          ss * 6 * (int64_t)npt * ne_q4;
     if (strain_rate_filter) b += ss * (int64_t)npt * ne_q4;              // LBUF EPSD (or_model_set_q4_params)
   }
-  b += (int64_t)sizeof(DevTime) * 2 + 8 * 2 + 4 + 8 * (3 + 3 * 1024) + (int64_t)sizeof(ModelDev);
+  b += (int64_t)sizeof(DevTime) * 2 + 8 * 2 + 4 + 8 * (4 + 4 * 1024) + (int64_t)sizeof(ModelDev);
   b += (int64_t)sizeof(Tile) * (th + tq + nsl);
   // or_model_upload_skyline: rest rows (at most every row of the family) and the prefix lists
   // (at most one prefix per node and family)

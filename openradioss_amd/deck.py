@@ -464,9 +464,14 @@ def to_model(deck: Deck, dtfac_hex: float = 0.9, dtfac_q4: float = 0.9, dtmx: fl
         n = group(gid, "/IMPVEL")
         code[n] |= bit[d]
         vimp[n, d] = val
-    return M.build_model(deck.x, hex_conn=hex_conn, hex_mat=hex_mat, hex_prop=hex_prop, q4_conn=q4_conn,
-                         q4_mat=q4_mat, q4_prop=q4_prop, thke=thke, v0=v0, code=code, vimp=vimp,
-                         dtfac_hex=dtfac_hex, dtfac_q4=dtfac_q4, dtmx=dtmx)
+    m = M.build_model(deck.x, hex_conn=hex_conn, hex_mat=hex_mat, hex_prop=hex_prop, q4_conn=q4_conn,
+                      q4_mat=q4_mat, q4_prop=q4_prop, thke=thke, v0=v0, code=code, vimp=vimp,
+                      dtfac_hex=dtfac_hex, dtfac_q4=dtfac_q4, dtmx=dtmx)
+    if m.hexp is not None:
+        m.hexp.ids = np.concatenate([blk[1] for blk in deck.bricks]).astype(np.int64)
+    if m.q4 is not None:
+        m.q4.ids = np.concatenate([blk[1] for blk in deck.shells]).astype(np.int64)
+    return m
 
 
 def read_run(path: str, engine: str, **kw) -> tuple[M.Model, EngineSettings]:

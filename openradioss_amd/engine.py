@@ -171,6 +171,51 @@ class GpuModel:
         L.check(self._lib.or_model_run_output(self._h, _p(out)))
         return out
 
+    def run_ecrit(self) -> dict:
+        """One output cycle: what ECRIT prints for it (or_model_run_ecrit): the energies, the external
+        work of the imposed velocities and the controlling element (itypts 1 solid, 3 shell, 0 none;
+        nelts its 0-based index in its family, -1 none)."""
+        e = L.OrEcrit()
+        L.check(self._lib.or_model_run_ecrit(self._h, C.byref(e)))
+        return dict(enint=e.enint, encin=e.encin, enrot=e.enrot, wfext=e.wfext + self.wfext_offset,
+                    itypts=e.itypts, nelts=e.nelts)
+
+    # external work carried over from a checkpoint (the device sums it from the node upload on;
+    # the DT2*DW half of the checkpointed cycle's imposed work is not carried)
+    wfext_offset = 0.0
+
+    def wfext(self) -> float:
+        """External work of the imposed velocities (since the node upload, plus wfext_offset)."""
+        w = C.c_double()
+        L.check(self._lib.or_model_wfext(self._h, C.byref(w)))
+        return w.value + self.wfext_offset
+
+    def run_listing(self, ncycles: int, every: int, listing=None):
+        """Run ncycles cycles printing the Engine listing's cycle line (ecrit.F FORMAT 1100) at every
+        cycle whose NCYCLE is a multiple of `every`; returns the openradioss_amd.listing.Listing (a
+        new one unless one is passed to continue)."""
+        from . import listing as LS
+        m = self.model
+        if listing is None:
+            xmass = float(np.cumsum(np.asarray(m.ms, np.float64))[-1])  # node after node, as ECRIT adds it
+            listing = LS.Listing(xmass, None if m.hexp is None else m.hexp.ids, None if m.q4 is None else m.q4.ids)
+        if every <= 0:
+            raise ValueError("every must be > 0")
+        left = ncycles
+        while left > 0:
+            t0 = self.time_state()
+            if t0.ncycle % every == 0:
+                e = self.run_ecrit()
+                t1 = self.time_state()
+                listing.add(t0.ncycle, t0.tt, t1.dt2, e["enint"], e["encin"], e["enrot"], e["wfext"], e["itypts"],
+                            e["nelts"])
+                left -= 1
+            else:
+                k = min(left, every - t0.ncycle % every)
+                self.run(k)
+                left -= k
+        return listing
+
     # -- restart ---------------------------------------------------------
     _TS_FIELDS = ("tt", "dt1", "dt2", "dt12", "dt2old", "ncycle", "mstop")
 
@@ -180,7 +225,8 @@ class GpuModel:
         x, v, vr, d = self.nodes6()
         ts = self.time_state()
         out = {"x": x, "v": v, "vr": vr, "d": d,
-               "time": np.array([float(getattr(ts, f)) for f in self._TS_FIELDS])}
+               "time": np.array([float(getattr(ts, f)) for f in self._TS_FIELDS]),
+               "wfext": np.array([self.wfext()])}
         if self.model.hexp is not None:
             out.update({"hex_" + k: a for k, a in self.hex_state().items()})
         if self.model.q4 is not None:
@@ -198,6 +244,8 @@ class GpuModel:
         for f, val in zip(cls._TS_FIELDS, ck["time"]):
             setattr(ts, f, int(val) if f in ("ncycle", "mstop") else float(val))
         g.set_time_state(ts)
+        if "wfext" in ck:
+            g.wfext_offset = float(ck["wfext"][0])
         return g
 
     def save(self, path: str):

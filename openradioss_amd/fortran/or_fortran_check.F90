@@ -195,6 +195,7 @@ contains
     integer(c_int32_t), allocatable, target :: code(:), connh(:, :), connq(:, :)
     integer(c_int64_t), allocatable, target :: adsky(:), iadsh(:, :), iadcq(:, :)
     type(or_time_state_t) :: ts
+    type(or_ecrit_t) :: ec
     open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', action='read')
     read (u) nn, neh, neq, npt, prec, ncyc, hp, qp
     allocate (xx(3, nn), vv(3, nn), vr(3, nn), d(3, nn), ms(nn), in(nn), code(nn), vimp(3, nn))
@@ -221,7 +222,9 @@ contains
                                                                c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
                                                                c_null_ptr, c_null_ptr)
     if (ierr == OR_OK) ierr = or_model_upload_skyline(h, c_loc(adsky), c_loc(iadsh), c_loc(iadcq))
-    if (ierr == OR_OK) ierr = or_model_run(h, int(ncyc, c_int64_t))
+    ! ncyc cycles, the last one an output cycle (ECRIT's line)
+    if (ierr == OR_OK) ierr = or_model_run(h, int(ncyc - 1, c_int64_t))
+    if (ierr == OR_OK) ierr = or_model_run_ecrit(h, ec)
     if (ierr == OR_OK) ierr = or_model_sync(h)
     if (ierr == OR_OK) ierr = or_model_download_nodes(h, c_loc(xx), c_loc(vv), c_loc(vr), c_loc(d))
     if (ierr == OR_OK) ierr = or_model_get_time(h, ts)
@@ -230,7 +233,7 @@ contains
       if (or_model_destroy(h) /= OR_OK) write (*, '(a)') 'or_model_destroy failed'
     end if
     open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
-    write (u) int(ierr, c_int32_t), xx, vv, vr, d, ts
+    write (u) int(ierr, c_int32_t), xx, vv, vr, d, ts, ec
     close (u)
   end subroutine run_model
 end program or_fortran_check

@@ -17,6 +17,7 @@ module or_gpu_mod
   public :: or_model_upload_nodes, or_model_upload_hex, or_model_upload_q4, or_model_upload_skyline
   public :: or_model_upload_displacement
   public :: or_model_set_time, or_model_get_time, or_model_run, or_model_run_output, or_model_sync
+  public :: or_ecrit_t, or_model_run_ecrit, or_model_wfext
   public :: or_model_download_nodes, or_model_download_hex, or_model_download_q4
   public :: or_model_energy, or_model_kernel_times, or_model_set_timing, or_model_footprint
   public :: or_model_footprint_estimate
@@ -58,6 +59,14 @@ module or_gpu_mod
     real(c_double) :: tt, dt1, dt2, dt12, dt2old
     integer(c_int64_t) :: ncycle
     integer(c_int32_t) :: mstop
+  end type
+
+  ! what ECRIT prints for one output cycle (include/or_gpu.h or_ecrit): ENINT, ENCIN, ENROT,
+  ! WFEXT, ITYPTS and NELTS (0-based index of the controlling element in its family)
+  type, bind(c) :: or_ecrit_t
+    real(c_double) :: enint, encin, enrot, wfext
+    integer(c_int32_t) :: itypts, pad
+    integer(c_int64_t) :: nelts
   end type
 
   interface
@@ -160,6 +169,18 @@ module or_gpu_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: handle
       real(c_double), intent(out) :: e(3)
+    end function
+
+    integer(c_int) function or_model_run_ecrit(handle, e) bind(c, name="or_model_run_ecrit")
+      import :: c_int, c_ptr, or_ecrit_t
+      type(c_ptr), value :: handle
+      type(or_ecrit_t), intent(out) :: e
+    end function
+
+    integer(c_int) function or_model_wfext(handle, w) bind(c, name="or_model_wfext")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: w
     end function
 
     integer(c_int) function or_model_sync(handle) bind(c, name="or_model_sync")

@@ -32,7 +32,7 @@ EXPORTS = (
     "or_model_energy", "or_model_kernel_times", "or_model_set_timing", "or_model_footprint", "or_model_footprint_estimate", "or_sforc3_group",
     "or_cforc3_group", "or_nodal_update", "or_batch_create", "or_batch_destroy", "or_batch_add", "or_batch_remove", "or_batch_run",
     "or_batch_sync", "or_batch_active", "or_model_set_variant", "or_model_autotune", "or_batch_autotune",
-    "or_math_eval", "or_sforc3_group_static", "or_model_assembly_stats",
+    "or_math_eval", "or_sforc3_group_static", "or_model_assembly_stats", "or_model_run_ecrit", "or_model_wfext",
 )
 
 
@@ -53,6 +53,11 @@ class OrQ4Params(C.Structure):
 class OrTimeState(C.Structure):
     _fields_ = [("tt", C.c_double), ("dt1", C.c_double), ("dt2", C.c_double), ("dt12", C.c_double),
                 ("dt2old", C.c_double), ("ncycle", C.c_int64), ("mstop", C.c_int32)]
+
+
+class OrEcrit(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("enint", "encin", "enrot", "wfext")] + [
+        ("itypts", C.c_int32), ("pad", C.c_int32), ("nelts", C.c_int64)]
 
 
 class OrError(RuntimeError):
@@ -100,6 +105,8 @@ _SIGS = {
                                                                                       C.c_double, _P], C.c_int),
     "or_math_eval": ([C.c_int, C.c_int, C.c_int64, _P, _P, _P], C.c_int),
     "or_model_run_output": ([_P, _P], C.c_int),
+    "or_model_run_ecrit": ([_P, _P], C.c_int),
+    "or_model_wfext": ([_P, _P], C.c_int),
     "or_nodal_update": ([C.c_int, C.c_int64, C.c_int64] + [_P] * 12 + [C.c_double, C.c_double, _P], C.c_int),
     "or_device_memory": ([C.c_int, _P, _P], C.c_int),
     "or_batch_create": ([_P, C.c_int], C.c_int),

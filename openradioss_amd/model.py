@@ -306,6 +306,7 @@ class HexPart:
     mat: Law2
     prop: SolidProp
     dtfac: float = 0.9
+    ids: np.ndarray | None = None  # (ne,) element user ids (None: index + 1)
 
     @property
     def ne(self) -> int:
@@ -331,6 +332,7 @@ class Q4Part:
     mat: Law2
     prop: ShellProp
     dtfac: float = 0.9
+    ids: np.ndarray | None = None  # (ne,) element user ids (None: index + 1)
 
     @property
     def ne(self) -> int:

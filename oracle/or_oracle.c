@@ -86,6 +86,16 @@ void or_hex_forces(int64_t e0, int64_t e1, int64_t ne, const int32_t *conn,
                    double *PLA, double *EPSD, double *OFFG, double *QVIS,
                    double *EHOU, const int64_t *IADS, double *FSKY,
                    const or_hex_params *p, double DT1, double *dt2t, int *mstop) {
+  or_hex_forces_dt(e0, e1, ne, conn, X, V, SIG, EINT, RHO, VOL0, PLA, EPSD, OFFG, QVIS, EHOU, IADS, FSKY, p, DT1,
+                   dt2t, mstop, NULL);
+}
+
+void or_hex_forces_dt(int64_t e0, int64_t e1, int64_t ne, const int32_t *conn,
+                      const double *X, const double *V, double *SIG,
+                      double *EINT, double *RHO, const double *VOL0,
+                      double *PLA, double *EPSD, double *OFFG, double *QVIS,
+                      double *EHOU, const int64_t *IADS, double *FSKY,
+                      const or_hex_params *p, double DT1, double *dt2t, int *mstop, double *dtel) {
   const double ONEP333 = c_onep333();
   double DT2T = *dt2t;
   for (int64_t e = e0; e < e1; ++e) {
@@ -345,6 +355,9 @@ void or_hex_forces(int64_t e0, int64_t e1, int64_t ne, const int32_t *conn,
       }
       DTX = p->dtfac * DTX;
       if (VOLN > ZERO && (off != ZERO && offg >= ZERO)) DT2T = dmin(DTX, DT2T);
+      /* the candidate MQVISCB weighs for NELTS (mqviscb.F:622-630): -1 when the element is
+         skipped (OFF <= 0, OFFG <= 0 or VOL <= 0) */
+      if (dtel) dtel[e] = (VOLN > ZERO && off > ZERO && offg > ZERO) ? DTX : -ONE;
     }
     /* pressure + energy (m2law.F, IEOS == 0) */
     {

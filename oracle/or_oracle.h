@@ -98,6 +98,11 @@ typedef struct {
 
 /* SFORC3 chain for elements [e0,e1), writing skyline rows.  Returns the
  * element time-step minimum folded into *dt2t. */
+/* or_hex_forces that also writes each element's time-step candidate for NELTS (dtel, may be NULL) */
+void or_hex_forces_dt(int64_t e0, int64_t e1, int64_t ne, const int32_t *conn, const double *x, const double *v,
+                      double *sig, double *eint, double *rho, const double *vol0, double *pla, double *epsd,
+                      double *offg, double *qvis, double *ehou, const int64_t *iads, double *fsky,
+                      const or_hex_params *p, double dt1, double *dt2t, int *mstop, double *dtel);
 void or_hex_forces(int64_t e0, int64_t e1, int64_t ne,
                    const int32_t *conn,            /* [ne*8] 0-based node ids */
                    const double *x, const double *v, /* [3*nn] AoS X(3,NUMNOD) */
@@ -171,6 +176,10 @@ typedef struct {
   double *epsd_ip;         /* LBUF%EPSD [npt][ne] */
 } or_q4_state;
 
+/* or_q4_forces that also writes each element's time-step candidate for NELTS (dtel, may be NULL) */
+void or_q4_forces_dt(int64_t e0, int64_t e1, const int32_t *conn, const double *x, const double *v,
+                     const double *vr, or_q4_state *s, const int64_t *iadc, double *fsky,
+                     const or_q4_params *p, double dt1, double *dt2t, double *dtel);
 void or_q4_forces(int64_t e0, int64_t e1, const int32_t *conn /* [ne*4] */, const double *x, const double *v,
                   const double *vr, or_q4_state *s, const int64_t *iadc /* [ne*4] */, double *fsky,
                   const or_q4_params *p, double dt1, double *dt2t);
@@ -198,8 +207,13 @@ typedef struct {
   or_q4_state q;
   const or_q4_params *qp;
   /* if non-NULL, or_cycle_model writes the energies of ECRIT for this cycle:
-   * [0] internal, [1] translational kinetic, [2] rotational kinetic */
+   * [0] internal, [1] translational kinetic, [2] rotational kinetic, [3] WFEXT */
   double *ecrit;
+  double wfext;              /* external work of the imposed velocities (FIXVEL's WFEXT) */
+  double *wpend;             /* [3*nn] each imposed DOF's DT2*DW, added at the next cycle (VEL(4,N)) */
+  double *dtel_h, *dtel_q;   /* [ne] per-element time-step candidates of output cycles, or NULL */
+  int32_t itypts;            /* controlling element of the last output cycle: 1 solid, 3 shell, 0 none */
+  int64_t nelts;             /* its 0-based index in its family, -1 none */
 } or_model_c;
 
 void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads);

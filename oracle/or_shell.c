@@ -265,6 +265,12 @@ static void m2cplr_point(const m2c_t *c, double off, double gs, double epsp_in, 
 void or_q4_forces(int64_t e0, int64_t e1, const int32_t *conn, const double *X, const double *V,
                   const double *VR, or_q4_state *s, const int64_t *IADC, double *FSKY,
                   const or_q4_params *p, double DT1, double *dt2t) {
+  or_q4_forces_dt(e0, e1, conn, X, V, VR, s, IADC, FSKY, p, DT1, dt2t, NULL);
+}
+
+void or_q4_forces_dt(int64_t e0, int64_t e1, const int32_t *conn, const double *X, const double *V,
+                     const double *VR, or_q4_state *s, const int64_t *IADC, double *FSKY,
+                     const or_q4_params *p, double DT1, double *dt2t, double *dtel) {
   const int64_t ne = s->ne;
   const int npt = p->npt;
   const double ZEP81 = c_zep81(), ZEP072169 = c_zep072169();
@@ -598,6 +604,8 @@ void or_q4_forces(int64_t e0, int64_t e1, const int32_t *conn, const double *X, 
       double DT = p->dtfac * ALDT / p->ssp;
       if (!(DT > p->dtmin || off < ONE || offg < ZERO)) off = ZERO; /* IDTMIN(3)=2 deletion */
       if (offg > ZERO && off != ZERO && DT < DT2T) DT2T = DT;
+      /* the candidate CDT3 weighs for NELTS (cdt3.F:205-215): -1 when OFFG <= 0 or OFF = 0 */
+      if (dtel) dtel[e] = (offg > ZERO && off != ZERO) ? DT : -ONE;
       double DIVM = dmax(ALDT * ALDT, EM20);
       STI = HALF * VOL0 * p->young / DIVM;
       STI = ZEP81 * STI * off;
@@ -688,16 +696,35 @@ void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads) {
     if (g < ngh) {
       int64_t e0 = g * G, e1 = (e0 + G < m->ne_hex) ? e0 + G : m->ne_hex;
       int ms = 0;
-      or_hex_forces(e0, e1, m->ne_hex, m->hconn, m->x, m->v, m->hsig, m->heint, m->hrho, m->hvol0, m->hpla,
-                    m->hepsd, m->hoffg, m->hqvis, NULL, m->hiads, m->fsky, m->hp, DT1, &dtl, &ms);
+      or_hex_forces_dt(e0, e1, m->ne_hex, m->hconn, m->x, m->v, m->hsig, m->heint, m->hrho, m->hvol0, m->hpla,
+                       m->hepsd, m->hoffg, m->hqvis, NULL, m->hiads, m->fsky, m->hp, DT1, &dtl, &ms,
+                       m->ecrit ? m->dtel_h : NULL);
       mstop |= ms;
     } else {
       int64_t e0 = (g - ngh) * G, e1 = (e0 + G < m->ne_q4) ? e0 + G : m->ne_q4;
-      or_q4_forces(e0, e1, m->qconn, m->x, m->v, m->vr, &m->q, m->qiadc, m->fsky, m->qp, DT1, &dtl);
+      or_q4_forces_dt(e0, e1, m->qconn, m->x, m->v, m->vr, &m->q, m->qiadc, m->fsky, m->qp, DT1, &dtl,
+                      m->ecrit ? m->dtel_q : NULL);
     }
     if (dtl < DT2T) DT2T = dtl;
   }
   (void)nthreads;
+  if (m->ecrit && (m->dtel_h || m->dtel_q)) {
+    /* NELTS / ITYPTS of the cycle, element after element in group order with DT2T = EP06 on
+       entry (resol.F:2826): solids replace it when DTX <= DT2T (mqviscb.F:622-630), shells when
+       DT < DT2T (cdt3.F:205-215) */
+    double t = EP06;
+    m->itypts = 0;
+    m->nelts = -1;
+    for (int64_t e = 0; m->dtel_h && e < m->ne_hex; ++e) {
+      const double d = m->dtel_h[e];
+      if (d < ZERO || d > t) continue;
+      t = d; m->itypts = 1; m->nelts = e;
+    }
+    for (int64_t e = 0; m->dtel_q && e < m->ne_q4; ++e) {
+      const double d = m->dtel_q[e];
+      if (d >= ZERO && d < t) { t = d; m->itypts = 3; m->nelts = e; }
+    }
+  }
   or_nodes_model(m, ts, DT2T, mstop);
 }
 
@@ -756,10 +783,23 @@ void or_nodes_model(or_model_c *m, or_time_state *ts, double dt2t, int mstop) {
       if (code & 32) { ar[0] = ZERO; vr[0] = ZERO; }
       if (code & 16) { ar[1] = ZERO; vr[1] = ZERO; }
       if (code & 8) { ar[2] = ZERO; vr[2] = ZERO; }
-      /* FIXVEL global frame (fixvel.F:366-379): bits 256,128,64 */
-      if (code & 256) a[0] = (m->vimp[3 * n + 0] - v[0]) / DT12;
-      if (code & 128) a[1] = (m->vimp[3 * n + 1] - v[1]) / DT12;
-      if (code & 64) a[2] = (m->vimp[3 * n + 2] - v[2]) / DT12;
+      /* FIXVEL global frame (fixvel.F:366-379): bits 256,128,64; each imposed DOF adds its work
+         DT1*DW to WFEXT now and DT2*DW at the next cycle (fixvel.F:376-378, 834, 837),
+         DW = FOURTH*MS*AXI*WEIGHT*(A*DT12 + TWO*V)*(A - AOLD)
+         with AXI = ONE - AX + AX*X(2,I) = ONE (AX = 0 in 3D, fixvel.F:123-127) and WEIGHT = 1 */
+      static const int bit[3] = {256, 128, 64};
+      for (int j = 0; j < 3; ++j) {
+        if (!(code & bit[j])) continue;
+        /* the DOF's DT2*DW of the previous cycle, kept in VEL(4,N), enters WFEXT first
+           (fixvel.F:342-344 with RW_SMS = ONE without mass scaling, fixvel.F:168-169) */
+        m->wfext = m->wfext + m->wpend[3 * n + j];
+        m->wpend[3 * n + j] = ZERO;
+        const double aold = a[j];
+        a[j] = (m->vimp[3 * n + j] - v[j]) / DT12;
+        const double DW = FOURTH * m->ms[n] * ONE * ONE * (a[j] * DT12 + TWO * v[j]) * (a[j] - aold);
+        m->wfext = m->wfext + ts->dt1 * DW;                       /* fixvel.F:834 */
+        m->wpend[3 * n + j] = m->wpend[3 * n + j] + DT2 * DW;    /* fixvel.F:837 */
+      }
     }
     if (m->ecrit) {
       /* ecrit.F:221-225 (MAS = MS*WEIGHT_MD, WEIGHT_MD = 1) and 319-324 */
@@ -795,6 +835,7 @@ void or_nodes_model(or_model_c *m, or_time_state *ts, double dt2t, int mstop) {
     m->ecrit[0] = ie_h + ie_q;
     m->ecrit[1] = encin;
     m->ecrit[2] = enrot;
+    m->ecrit[3] = m->wfext;
   }
   ts->tt = ts->tt + ts->dt2;
   ts->ncycle += 1;

@@ -187,7 +187,8 @@ class ModelC(C.Structure):
                 ("ne_hex", C.c_int64), ("hconn", _I32), ("hiads", _I64)] + \
                [(n, _D) for n in ("hsig", "heint", "hrho", "hpla", "hepsd", "hoffg", "hqvis", "hvol0")] + \
                [("hp", C.POINTER(HexParams)), ("ne_q4", C.c_int64), ("qconn", _I32), ("qiadc", _I64),
-                ("q", Q4State), ("qp", C.POINTER(Q4Params)), ("ecrit", _D)]
+                ("q", Q4State), ("qp", C.POINTER(Q4Params)), ("ecrit", _D), ("wfext", C.c_double), ("wpend", _D),
+                ("dtel_h", _D), ("dtel_q", _D), ("itypts", C.c_int32), ("nelts", C.c_int64)]
 
 
 def q4_params(part) -> Q4Params:
@@ -268,6 +269,8 @@ class ModelOracle:
             for k in ("offg", "thk", "forc", "mom", "gstr", "hourg", "eint", "epsd", "sig", "pla", "epsd_ip"):
                 setattr(c.q, k, _pd(self.q[k]))
             c.qp = C.pointer(self.qp)
+        self.wpend = np.zeros(3 * nn)
+        c.wpend = _pd(self.wpend)
         self.ts = initial_time_state()
 
     def cycle(self, n: int = 1):
@@ -283,13 +286,30 @@ class ModelOracle:
     def cycle_output(self) -> np.ndarray:
         """One cycle that also returns ECRIT's energies of the cycle (internal, translational and
         rotational kinetic at the output point between ACCELE and VELOCITY)."""
-        out = np.zeros(3)
+        return self.cycle_ecrit()["energies"]
+
+    def cycle_ecrit(self) -> dict:
+        """One output cycle: ECRIT's energies, WFEXT and the controlling element (ITYPTS 1 solid, 3
+        shell, 0 none; NELTS its 0-based index in its family, -1 none)."""
+        out = np.zeros(4)
+        self._dtel_h = np.zeros(max(self.m.ne_hex, 1))
+        self._dtel_q = np.zeros(max(self.m.ne_q4, 1))
         self.c.ecrit = _pd(out)
+        self.c.dtel_h = _pd(self._dtel_h) if self.m.ne_hex else None
+        self.c.dtel_q = _pd(self._dtel_q) if self.m.ne_q4 else None
         try:
             lib(self.math).or_cycle_model(C.byref(self.c), C.byref(self.ts), C.c_int(self.nthreads))
         finally:
             self.c.ecrit = None
-        return out
+            self.c.dtel_h = None
+            self.c.dtel_q = None
+        return dict(energies=out[:3].copy(), enint=out[0], encin=out[1], enrot=out[2], wfext=out[3],
+                    itypts=int(self.c.itypts), nelts=int(self.c.nelts))
+
+    @property
+    def wfext(self) -> float:
+        """External work of the imposed velocities so far (FIXVEL's WFEXT)."""
+        return float(self.c.wfext)
 
 
 def math_site_counts() -> dict:

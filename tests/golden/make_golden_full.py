@@ -9,6 +9,10 @@ own compiled routines (oracle/_ref, built by oracle/build_ref.py from /root/refe
   top at 15 m/s) after 8,800 cycles, before its first fold: digests of D, V, VR and X, D at
   every 4th node, the time scalars.
 
+Both hold `energy` = [internal, translational kinetic, rotational kinetic] of the final state:
+the internal energy as the reference's compiled SBILAN / CBILAN add it (oracle/ref_bilan.F),
+the kinetic sums 0.5*MS*|V|^2 and 0.5*IN*|VR|^2 of the reference's velocities, node after node.
+
 Run in the build container:  python tests/golden/make_golden_full.py [bar] [tube]
 (about 1 minute for the bar, 20 for the tube).
 """
@@ -59,6 +63,18 @@ def main():
             out.update(v_sub=r.v[::c["v_every"]], v_every=np.array([c["v_every"]]))
         if m.q4 is not None:
             out.update(sha_vr=np.array(digest(r.vr)))
+        kw = dict(v=r.v, x=r.x, vr=r.vr)
+        if m.hexp is not None:
+            kw.update(heint=r.heint, hrho=r.hrho, hoffg=r.hoffg)
+        if m.q4 is not None:
+            kw.update(q_eint=r.q["eint"], q_offg=r.q["offg"])
+        eh, eq = R.bilan_internal_energy(m, **kw)
+        ke = 0.0
+        kr = 0.0
+        for n in range(m.nn):
+            ke = ke + 0.5 * float(m.ms[n]) * float(r.v[n] @ r.v[n])
+            kr = kr + 0.5 * float(m.inr[n]) * float(r.vr[n] @ r.vr[n])
+        out.update(energy=np.array([eh + eq, ke, kr]))
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
         print(name, "written, tt", r.tt)
 

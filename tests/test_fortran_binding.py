@@ -320,7 +320,8 @@ def test_fortran_nodal_update_equals_python_call(tmp_path, gpu_lib):
 @needs_exe
 @pytest.mark.gpu
 def test_fortran_resident_model_equals_python_model(tmp_path, gpu_lib):
-    print("case: Fortran resident model (create, upload, run 60 cycles, download), mixed block + plate", flush=True)
+    print("case: Fortran resident model (create, upload, 59 cycles and an output cycle, download), mixed block + "
+          "plate", flush=True)
     from golden_cases import Q4_CASES, build_q4_case
     m = build_q4_case(Q4_CASES["mixed_c120"])
     hp, qp = E.hex_params(m), E.q4_params(m)
@@ -349,10 +350,15 @@ def test_fortran_resident_model_equals_python_model(tmp_path, gpu_lib):
     n3 = 24 * m.nn
     xf, vf, vrf, df = (np.frombuffer(b[4 + k * n3:4 + (k + 1) * n3], np.float64).reshape(m.nn, 3) for k in range(4))
     tsf = time_state_of(b[4 + 4 * n3:])
+    ecf = L.OrEcrit.from_buffer_copy(b[4 + 4 * n3 + 52:4 + 4 * n3 + 52 + C.sizeof(L.OrEcrit)])
     g = E.GpuModel(m)
-    g.run(60)
+    g.run(59)
+    ec = g.run_ecrit()
     x, v, vr, d = g.nodes6()
     ts = g.time_state()
     g.close()
     assert tsf.ncycle == ts.ncycle == 60 and tsf.tt == ts.tt
     assert np.array_equal(xf, x) and np.array_equal(vf, v) and np.array_equal(vrf, vr) and np.array_equal(df, d)
+    # the listing record of the output cycle, as OR_MODEL_RUN_ECRIT returned it to Fortran
+    assert (ecf.enint, ecf.encin, ecf.enrot, ecf.wfext) == (ec["enint"], ec["encin"], ec["enrot"], ec["wfext"])
+    assert (ecf.itypts, ecf.nelts) == (ec["itypts"], ec["nelts"]) and ec["nelts"] >= 0
