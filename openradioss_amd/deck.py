@@ -464,14 +464,14 @@ def to_model(deck: Deck, dtfac_hex: float = 0.9, dtfac_q4: float = 0.9, dtmx: fl
         n = group(gid, "/IMPVEL")
         code[n] |= bit[d]
         vimp[n, d] = val
-    m = M.build_model(deck.x, hex_conn=hex_conn, hex_mat=hex_mat, hex_prop=hex_prop, q4_conn=q4_conn,
-                      q4_mat=q4_mat, q4_prop=q4_prop, thke=thke, v0=v0, code=code, vimp=vimp,
-                      dtfac_hex=dtfac_hex, dtfac_q4=dtfac_q4, dtmx=dtmx)
-    if m.hexp is not None:
-        m.hexp.ids = np.concatenate([blk[1] for blk in deck.bricks]).astype(np.int64)
-    if m.q4 is not None:
-        m.q4.ids = np.concatenate([blk[1] for blk in deck.shells]).astype(np.int64)
-    return m
+    hex_ids = np.concatenate([blk[1] for blk in deck.bricks]).astype(np.int64) if hex_conn is not None else None
+    q4_ids = np.concatenate([blk[1] for blk in deck.shells]).astype(np.int64) if q4_conn is not None else None
+    for ids, kind in ((hex_ids, "BRICK"), (q4_ids, "SHELL")):
+        if ids is not None and np.unique(ids).shape[0] != ids.shape[0]:
+            raise DeckError(f"/{kind}: an element id appears twice")
+    return M.build_model(deck.x, hex_conn=hex_conn, hex_mat=hex_mat, hex_prop=hex_prop, q4_conn=q4_conn,
+                         q4_mat=q4_mat, q4_prop=q4_prop, thke=thke, v0=v0, code=code, vimp=vimp,
+                         dtfac_hex=dtfac_hex, dtfac_q4=dtfac_q4, dtmx=dtmx, hex_ids=hex_ids, q4_ids=q4_ids)
 
 
 def read_run(path: str, engine: str, **kw) -> tuple[M.Model, EngineSettings]:
@@ -605,17 +605,18 @@ def write(m: M.Model, title: str = "model") -> str:
         w(f"/PROP/SOLID/1\nsolids\n{_i10(1)}{_i10(0) * 7}{_r(0.0)}\n"
           f"{_r(pr.qa)}{_r(pr.qb)}{_r(pr.qh)}{_r(0.0)}{_r(0.0)}\n{_r(0.0)}\n")
         w(f"/PART/1\nsolids\n{_i10(1)}{_i10(1)}{_i10(0)}\n/BRICK/1\n")
-        for e, row in enumerate(m.hexp.conn, 1):
-            w(_i10(e) + "".join(_i10(n + 1) for n in row) + "\n")
+        hids = np.arange(1, m.ne_hex + 1) if m.hexp.ids is None else m.hexp.ids
+        for e, row in zip(hids, m.hexp.conn):
+            w(_i10(int(e)) + "".join(_i10(n + 1) for n in row) + "\n")
     if m.q4 is not None:
         pr = m.q4.prop
         w(f"/PROP/SHELL/2\nshells\n{_i10(1)}{_i10(2)}{_i10(0) * 3}{' ' * 10}{_r(0.0)}\n"
           f"{_r(pr.hm)}{_r(pr.hf)}{_r(pr.hr)}{_r(0.0)}{_r(0.0)}\n"
           f"{_i10(pr.npt)}{' ' * 10}{_r(pr.thick)}{_r(0.0)}{' ' * 10}{_i10(pr.ithk)}{_i10(pr.ipla)}{_i10(0)}\n")
         w(f"/PART/2\nshells\n{_i10(2)}{_i10(2)}{_i10(0)}\n/SHELL/2\n")
-        base = m.ne_hex
-        for e, (row, t) in enumerate(zip(m.q4.conn, m.q4.thke), base + 1):
-            w(_i10(e) + "".join(_i10(n + 1) for n in row) + _r(0.0) + _r(t) + "\n")
+        qids = np.arange(m.ne_hex + 1, m.ne_hex + m.ne_q4 + 1) if m.q4.ids is None else m.q4.ids
+        for e, row, t in zip(qids, m.q4.conn, m.q4.thke):
+            w(_i10(int(e)) + "".join(_i10(n + 1) for n in row) + _r(0.0) + _r(t) + "\n")
     gid = [0]
 
     def grnod(nodes, name):

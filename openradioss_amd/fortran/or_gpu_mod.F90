@@ -23,7 +23,7 @@ module or_gpu_mod
   public :: or_model_footprint_estimate
   public :: or_batch_create, or_batch_destroy, or_batch_add, or_batch_remove, or_batch_run, or_batch_sync
   public :: or_batch_active, or_model_set_variant, or_model_autotune, or_batch_autotune
-  public :: or_sforc3_group, or_cforc3_group, or_nodal_update, or_math_eval
+  public :: or_sforc3_group, or_sforc3_group_static, or_cforc3_group, or_nodal_update, or_math_eval
   public :: or_gpu_eligible_solid, or_gpu_eligible_shell, or_pack_hex_params, or_pack_q4_params
 
   integer(c_int), parameter, public :: OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_HIP = 2, OR_ERR_NODEV = 3, &
@@ -307,6 +307,24 @@ module or_gpu_mod
     ! EPSD, OFF, QVIS of the group; FSKY rows written like SCUMU3P; DT2T min-folded.
     integer(c_int) function or_sforc3_group(device, p, numnod, nel, nft, x, v, ixs, sig, eint, rho, vol, pla, &
         epsd, off, qvis, lsky, fsky, iads, dt1, dt2t, mstop) bind(c, name="or_sforc3_group")
+      import :: c_int, c_int32_t, c_int64_t, c_double, or_hex_params_t
+      integer(c_int), value :: device
+      type(or_hex_params_t), intent(in) :: p
+      integer(c_int64_t), value :: numnod, nft, lsky
+      integer(c_int32_t), value :: nel
+      real(c_double), intent(in) :: x(3, *), v(3, *), vol(*)
+      integer(c_int32_t), intent(in) :: ixs(11, *), iads(8, *)
+      real(c_double), intent(inout) :: sig(*), eint(*), rho(*), pla(*), epsd(*), off(*), qvis(*)
+      real(c_double), intent(inout) :: fsky(8, *)
+      real(c_double), value :: dt1
+      real(c_double), intent(inout) :: dt2t
+      integer(c_int32_t), intent(inout) :: mstop
+    end function
+
+    ! the same without any device allocation (one file-scope device array per device; calls
+    ! that overlap on one device race), for callers that cannot allocate
+    integer(c_int) function or_sforc3_group_static(device, p, numnod, nel, nft, x, v, ixs, sig, eint, rho, vol, &
+        pla, epsd, off, qvis, lsky, fsky, iads, dt1, dt2t, mstop) bind(c, name="or_sforc3_group_static")
       import :: c_int, c_int32_t, c_int64_t, c_double, or_hex_params_t
       integer(c_int), value :: device
       type(or_hex_params_t), intent(in) :: p

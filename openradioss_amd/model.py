@@ -391,17 +391,31 @@ class Model:
         return self.q4 is not None
 
 
-def mixed_skyline(nn: int, hex_conn=None, q4_conn=None):
-    """FILLCNE-ordered skyline over hex then Q4 contributions; returns (iads, iadc, adsky)."""
+def id_rank(ids, ne: int) -> np.ndarray:
+    """Position of each element in ascending user-id order (the order MY_ORDERS gives FILLCNE
+    and SPMD_MSIN, domdec2.F:2138-2166 and spmd_msin.F:95-99); element index order without ids."""
+    if ids is None:
+        return np.arange(ne, dtype=np.int64)
+    ids = np.asarray(ids, dtype=np.int64)
+    if ids.shape != (ne,):
+        raise ValueError("one user id per element")
+    rank = np.empty(ne, dtype=np.int64)
+    rank[np.argsort(ids, kind="stable")] = np.arange(ne, dtype=np.int64)
+    return rank
+
+
+def mixed_skyline(nn: int, hex_conn=None, q4_conn=None, hex_ids=None, q4_ids=None):
+    """FILLCNE-ordered skyline over hex then Q4 contributions (solids before shells, each family
+    by ascending user id, domdec2.F:2166-2240); returns (iads, iadc, adsky)."""
     nodes, ranks, sizes = [], [], []
     base = 0
-    for conn in (hex_conn, q4_conn):
+    for conn, ids in ((hex_conn, hex_ids), (q4_conn, q4_ids)):
         if conn is None or conn.shape[0] == 0:
             sizes.append(None)
             continue
         ne, nk = conn.shape
         nodes.append(conn.reshape(-1).astype(np.int64))
-        ranks.append(np.repeat(np.arange(base, base + ne, dtype=np.int64), nk))
+        ranks.append(np.repeat(base + id_rank(ids, ne), nk))
         sizes.append((ne, nk))
         base += ne
     node = np.concatenate(nodes)
@@ -433,14 +447,14 @@ def q4_area(xyz: np.ndarray) -> np.ndarray:
 def build_model(x, hex_conn=None, hex_mat: Law2 | None = None, hex_prop: SolidProp | None = None,
                 q4_conn=None, q4_mat: Law2 | None = None, q4_prop: ShellProp | None = None, thke=None,
                 v0=None, vr0=None, code=None, vimp=None, dtfac_hex: float = 0.9, dtfac_q4: float = 0.9,
-                dtmx: float = EP20) -> Model:
+                dtmx: float = EP20, hex_ids=None, q4_ids=None) -> Model:
     """Starter-equivalent builder: volumes, SMASS3 / CINMAS lumped masses and
     inertias, FILLCNE skyline, BC codes."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     nn = x.shape[0]
     hex_conn = None if hex_conn is None else np.ascontiguousarray(hex_conn, dtype=np.int32)
     q4_conn = None if q4_conn is None else np.ascontiguousarray(q4_conn, dtype=np.int32)
-    iads, iadc, adsky = mixed_skyline(nn, hex_conn, q4_conn)
+    iads, iadc, adsky = mixed_skyline(nn, hex_conn, q4_conn, hex_ids, q4_ids)
     ms = np.zeros(nn)
     inr = np.zeros(nn)
     hexp = q4p = None
@@ -453,7 +467,7 @@ def build_model(x, hex_conn=None, hex_mat: Law2 | None = None, hex_prop: SolidPr
         mass = (1.0 * rho) * vol0 * (1.0 / 8.0)          # smass3.F: RHO*VOLU*ONE_OVER_8
         rows_m[iads.reshape(-1)] = np.repeat(mass, 8)
         hexp = HexPart(conn=hex_conn, vol0=vol0, iads=iads, mat=hex_mat, prop=hex_prop or SolidProp(),
-                       dtfac=dtfac_hex)
+                       dtfac=dtfac_hex, ids=None if hex_ids is None else np.asarray(hex_ids, np.int64))
     if q4_conn is not None and q4_conn.shape[0]:
         ne = q4_conn.shape[0]
         thke = np.full(ne, q4_prop.thick) if thke is None else np.asarray(thke, dtype=np.float64)
@@ -462,7 +476,8 @@ def build_model(x, hex_conn=None, hex_mat: Law2 | None = None, hex_prop: SolidPr
         xi = ems * (area / 9.0 + thke * thke * (1.0 / 12.0))  # cinmas.F:1383 (FAC=9, no offset)
         rows_m[iadc.reshape(-1)] = np.repeat(ems, 4)
         rows_i[iadc.reshape(-1)] = np.repeat(xi, 4)
-        q4p = Q4Part(conn=q4_conn, thke=thke, iadc=iadc, mat=q4_mat, prop=q4_prop, dtfac=dtfac_q4)
+        q4p = Q4Part(conn=q4_conn, thke=thke, iadc=iadc, mat=q4_mat, prop=q4_prop, dtfac=dtfac_q4,
+                     ids=None if q4_ids is None else np.asarray(q4_ids, np.int64))
     counts = np.diff(adsky)
     for j in range(int(counts.max()) if nn else 0):
         has = counts > j

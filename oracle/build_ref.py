@@ -27,8 +27,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "_ref")
 OBJ = os.path.join(OUT, "obj")
 FC = "/opt/rocm/llvm/bin/amdflang"
-INC = [f"-I{REF}/engine/share/includes", f"-I{REF}/engine/share/spe_inc", f"-I{REF}/engine/share/r8",
-       f"-I{REF}/common_source/includes", f"-I{REF}/engine/share/message_module", f"-I{OBJ}"]
 FLAGS = ["-cpp", "-DMYREAL8", "-DCPP_mach=CPP_p4linux964", "-DCPP_rel=00", "-ffixed-line-length=132",
          "-O2", "-ffp-contract=off", "-fPIC", f"-J{OBJ}"]
 
@@ -73,13 +71,29 @@ LEAVES = [
 ]
 DRIVERS = ["ref_driver.F", "ref_shell.F", "ref_bilan.F"]
 
+# The Starter's own element initialisation routines that fix the Engine's inputs on this path:
+# the solid volume (SDERI3), the solid corner masses (SMASS3), the shell area (CEVECI), the shell
+# mass and inertia (CINMAS) and the nodal sums in element-id order (SPMD_MSIN, with the C sort
+# MY_ORDERS it calls).  They build a second library with the Starter's include tree and modules.
+STARTER_LEAVES = [
+    "starter/source/elements/solid/solide/sderi3.F",
+    "starter/source/elements/solid/solide/smass3.F",
+    "starter/source/elements/shell/coque/ceveci.F",
+    "starter/source/elements/shell/coque/cinmas.F",
+    "starter/source/elements/initia/spmd_msin.F",
+]
+STARTER_C = ["common_source/tools/sort/my_orders.c", "common_source/tools/memory/cpp_report_alloc.cpp"]
+STARTER_DRIVERS = ["ref_starter.F"]
+STARTER_INC_DIRS = ["starter/share/includes", "starter/share/spe_inc", "starter/share/r8", "common_source/includes",
+                    "starter/share/message_module"]
+
 _MOD = re.compile(r"^\s*module\s+([a-z0-9_]+)\s*(!.*)?$", re.I)
 
 
-def module_index():
+def module_index(roots):
     idx = {}
-    for root in (f"{REF}/common_source", f"{REF}/engine/source", f"{REF}/engine/share"):
-        for d, _, fs in os.walk(root):
+    for root in roots:
+        for d, _, fs in os.walk(os.path.join(REF, root)):
             for f in fs:
                 if f.endswith((".F", ".F90")):
                     p = os.path.join(d, f)
@@ -91,22 +105,23 @@ def module_index():
     return idx
 
 
-def main():
-    if not os.path.isdir(REF):
-        print("reference tree absent: oracle/_ref not built")
-        return 0
-    if os.path.isdir(OBJ):
-        shutil.rmtree(OBJ)
-    os.makedirs(OBJ, exist_ok=True)
-    idx = module_index()
+def build_library(name, leaves, drivers, roots, inc_dirs, objdir, c_sources=()):
+    """Compile `leaves` (and the modules they use, found under `roots`) and our `drivers` into
+    oracle/_ref/<name>; returns the number of reference files compiled, or -1."""
+    if os.path.isdir(objdir):
+        shutil.rmtree(objdir)
+    os.makedirs(objdir, exist_ok=True)
+    inc = [f"-I{REF}/{d}" for d in inc_dirs] + [f"-I{objdir}"]
+    flags = [f if not f.startswith("-J") else f"-J{objdir}" for f in FLAGS]
+    idx = module_index(roots)
     done: dict[str, str] = {}
 
     def compile_one(path: str, depth: int = 0) -> bool:
         if path in done:
             return True
-        obj = os.path.join(OBJ, os.path.relpath(path, REF).replace("/", "__") + ".o")
+        obj = os.path.join(objdir, os.path.relpath(path, REF).replace("/", "__") + ".o")
         for _ in range(60):
-            r = subprocess.run([FC, *FLAGS, *INC, "-c", path, "-o", obj], capture_output=True, text=True)
+            r = subprocess.run([FC, *flags, *inc, "-c", path, "-o", obj], capture_output=True, text=True)
             if r.returncode == 0:
                 done[path] = obj
                 return True
@@ -121,34 +136,65 @@ def main():
             return False
         return False
 
-    ok = all(compile_one(os.path.join(REF, leaf)) for leaf in LEAVES)
-    if not ok:
-        return 1
+    if not all(compile_one(os.path.join(REF, leaf)) for leaf in leaves):
+        return -1
+    objs = list(done.values())
+    for c in c_sources:
+        obj = os.path.join(objdir, c.replace("/", "__") + ".o")
+        cc = "g++" if c.endswith(".cpp") else "gcc"
+        r = subprocess.run([cc, "-O2", "-fPIC", "-c", os.path.join(REF, c), "-o", obj], capture_output=True,
+                           text=True)
+        if r.returncode != 0:
+            print(r.stderr[-2000:])
+            return -1
+        objs.append(obj)
+        done[os.path.join(REF, c)] = obj
     drv_objs = []
-    for name in DRIVERS:
-        drv = os.path.join(HERE, name)
-        drv_obj = os.path.join(OBJ, name.replace(".F", ".o"))
+    for dname in drivers:
+        drv = os.path.join(HERE, dname)
+        drv_obj = os.path.join(objdir, dname.replace(".F", ".o"))
         for _ in range(40):
-            r = subprocess.run([FC, *FLAGS, *INC, "-c", drv, "-o", drv_obj], capture_output=True, text=True)
+            r = subprocess.run([FC, *flags, *inc, "-c", drv, "-o", drv_obj], capture_output=True, text=True)
             if r.returncode == 0:
                 break
             m = re.search(r"module '([a-z0-9_]+)'", r.stderr, re.I)
             dep = idx.get(m.group(1).lower()) if m else None
             if not (dep and compile_one(dep)):
                 print(r.stderr[-3000:])
-                return 1
+                return -1
         drv_objs.append(drv_obj)
-    lib = os.path.join(OUT, "libref.so")
-    r = subprocess.run([FC, "-shared", "-o", lib, *drv_objs, *done.values()], capture_output=True, text=True)
+    objs = list(dict.fromkeys(list(done.values()) + objs))
+    lib = os.path.join(OUT, name)
+    extra = ["-lstdc++"] if any(c.endswith(".cpp") for c in c_sources) else []
+    r = subprocess.run([FC, "-shared", "-o", lib, *drv_objs, *objs, *extra], capture_output=True, text=True)
     if r.returncode != 0:
         print(r.stderr[-3000:])
-        return 1
-    with open(os.path.join(OUT, "MANIFEST.txt"), "w") as fh:
+        return -1
+    with open(os.path.join(OUT, name.replace(".so", "_MANIFEST.txt")), "w") as fh:
         ver = subprocess.run([FC, "--version"], capture_output=True, text=True).stdout.splitlines()[0]
-        fh.write(f"compiler: {ver}\nflags: {' '.join(FLAGS)}\n")
+        fh.write(f"compiler: {ver}\nflags: {' '.join(flags)}\n")
         for p in sorted(done):
             fh.write(os.path.relpath(p, REF) + "\n")
-    print(f"oracle/_ref/libref.so built from {len(done)} reference files")
+    return len(done)
+
+
+def main():
+    if not os.path.isdir(REF):
+        print("reference tree absent: oracle/_ref not built")
+        return 0
+    os.makedirs(OUT, exist_ok=True)
+    n = build_library("libref.so", LEAVES, DRIVERS, ("common_source", "engine/source", "engine/share"),
+                      ("engine/share/includes", "engine/share/spe_inc", "engine/share/r8", "common_source/includes",
+                       "engine/share/message_module"), OBJ)
+    if n < 0:
+        return 1
+    print(f"oracle/_ref/libref.so built from {n} reference files")
+    n = build_library("libref_starter.so", STARTER_LEAVES, STARTER_DRIVERS,
+                      ("common_source", "starter/source", "starter/share"), STARTER_INC_DIRS, OBJ + "_starter",
+                      STARTER_C)
+    if n < 0:
+        return 1
+    print(f"oracle/_ref/libref_starter.so built from {n} reference files")
     return 0
 
 

@@ -308,3 +308,58 @@ def bilan_internal_energy(model, heint=None, hrho=None, hoffg=None, v=None, q_ei
                          _p(gvol), _p(np.ascontiguousarray(q_offg[e0:e1])), _p(ps))
         eq = float(ps[0])
     return eh, eq
+
+
+# ---------------------------------------------------------------------------------------------
+# The reference Starter's element initialisation (oracle/_ref/libref_starter.so, oracle/ref_starter.F)
+LIB_STARTER = os.path.join(HERE, "_ref", "libref_starter.so")
+_lib_st = None
+
+
+def starter_available() -> bool:
+    return os.path.exists(LIB_STARTER)
+
+
+def lib_starter():
+    global _lib_st
+    if _lib_st is None:
+        # RTLD_LOCAL: the Starter's modules and commons share names with the Engine's
+        _lib_st = C.CDLL(LIB_STARTER, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        for f in ("ref_st_mass_", "ref_st_volume_"):
+            getattr(_lib_st, f).restype = None
+    return _lib_st
+
+
+def starter_masses(x, hex_conn=None, hex_ids=None, rho_hex=0.0, q4_conn=None, q4_ids=None, thke=None,
+                   rho_q4=0.0, ihbe=1) -> dict:
+    """MS and IN of a model as the reference Starter computes them: solid volumes (SDERI3) and corner
+    masses (SMASS3), shell areas (CEVECI), masses and inertias (CINMAS), then the nodal sums over
+    the elements in user-id order (SPMD_MSIN).  conn: 0-based node ids; ids: element user ids
+    (default index + 1)."""
+    L = lib_starter()
+    x = np.ascontiguousarray(x, np.float64)
+    nn = x.shape[0]
+    nels = 0 if hex_conn is None else int(hex_conn.shape[0])
+    nelc = 0 if q4_conn is None else int(q4_conn.shape[0])
+    ixs = np.zeros((max(nels, 1), 11), np.int32)
+    if nels:
+        ixs[:, 0] = 1
+        ixs[:, 1:9] = np.asarray(hex_conn) + 1
+        ixs[:, 9] = 1
+        ixs[:, 10] = np.arange(1, nels + 1) if hex_ids is None else np.asarray(hex_ids)
+    ixc = np.zeros((max(nelc, 1), 7), np.int32)
+    th = np.zeros(max(nelc, 1))
+    if nelc:
+        ixc[:, 0] = 1
+        ixc[:, 1:5] = np.asarray(q4_conn) + 1
+        ixc[:, 5] = 1
+        ixc[:, 6] = np.arange(1, nelc + 1) if q4_ids is None else np.asarray(q4_ids)
+        th[:nelc] = thke
+    ms, inr = np.zeros(nn), np.zeros(nn)
+    vols, mss = np.zeros(max(nels, 1)), np.zeros((max(nels, 1), 8))
+    area, msc, inc = np.zeros(max(nelc, 1)), np.zeros(max(nelc, 1)), np.zeros(max(nelc, 1))
+    i32 = lambda v: C.byref(C.c_int32(v))  # noqa: E731
+    f64 = lambda v: C.byref(C.c_double(v))  # noqa: E731
+    L.ref_st_mass_(i32(nn), _p(x), i32(nels), _p(ixs), f64(rho_hex), i32(nelc), _p(ixc), _p(th), f64(rho_q4),
+                   i32(ihbe), _p(ms), _p(inr), _p(vols), _p(mss), _p(area), _p(msc), _p(inc))
+    return dict(ms=ms, inr=inr, vol=vols[:nels], mss=mss[:nels], area=area[:nelc], msc=msc[:nelc], inc=inc[:nelc])

@@ -349,7 +349,7 @@ def test_fortran_resident_model_equals_python_model(tmp_path, gpu_lib):
     assert np.frombuffer(b[:4], np.int32)[0] == L.OR_OK, r.stdout
     n3 = 24 * m.nn
     xf, vf, vrf, df = (np.frombuffer(b[4 + k * n3:4 + (k + 1) * n3], np.float64).reshape(m.nn, 3) for k in range(4))
-    tsf = time_state_of(b[4 + 4 * n3:])
+    tsf = time_state_of(b[4 + 4 * n3:4 + 4 * n3 + 52])
     ecf = L.OrEcrit.from_buffer_copy(b[4 + 4 * n3 + 52:4 + 4 * n3 + 52 + C.sizeof(L.OrEcrit)])
     g = E.GpuModel(m)
     g.run(59)
