@@ -1,0 +1,317 @@
+"""The Engine's time-history file (``<run>T01``): writer and reader of its binary format.
+
+Layout (the ITTYP = 3 branch of hist1.F / hist2.F, written through write_routines.c):
+
+* every record is framed by its byte length, a big-endian int32, before and after it (EOR_C);
+* integers are big-endian int32 (integer_to_IEEE_ASCII, ieee.cpp:40-46), characters one byte each
+  (WRITE_C_C), reals big-endian IEEE single precision after the Fortran conversion REAL*4 = value
+  (round to nearest) and the encoder's own rules (real_to_IEEE_ASCII, ieee.cpp:66-123): a value in
+  (-1.17e-38, 0) is written as -0, one in [0, 1.17e-38) (and -0) as +0, magnitudes above 3.4e38 as
+  infinities, everything else exactly;
+* header (HIST1, hist1.F:194-590): ICODE and the run title; the date line; for TH_VERS >= 50 the
+  record count, the title length and the unit factors; NPART, NUMMAT, NUMGEO, NSUBS, NTHGRP and
+  NGLOBTH with the global variable codes; one record per part (id, title, subset, material and
+  property numbers, its variable count) with its variable codes; one per material and property
+  (id, title); per subset its description, children, parts and variable codes; per TH group its
+  description (id, type, 0, entity count, variable count, title), one record per entity (user id,
+  name) and its variable codes;
+* one block per output time (HIST2, hist2.F:299-330, 470, 601, 959-999): TIME, the NGLOBTH global
+  variables, the part variables and subset variables (one record each, when there are any), then
+  one record per TH group with its entities' variables, entity after entity.
+
+The variable codes of /TH/NODE, /TH/BRIC and /TH/SHEL are the positions in the Starter's tables
+(hm_read_thgrou.F:402-409 nodes, 419-422 solids, 825-830 shells) and the generic names expand as
+hm_read_thgrou.F:1303-1313 (nodes), 1889-1894 (solids) and 2140-2152 (shells) say; a list is the
+sorted set of the codes its names select (hm_read_thvarc.F).  `read` parses any such file.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time as _time
+
+import numpy as np
+
+__all__ = ["NODE_VARS", "BRIC_VARS", "SHEL_VARS", "GLOBAL_VARS", "ThGroup", "ThHeader", "ThError", "encode_reals",
+           "var_codes", "header_bytes", "time_bytes", "read", "next_time"]
+
+
+class ThError(ValueError):
+    """A time-history request this path cannot honour (an unknown or unsupported variable)."""
+
+
+# -- variable tables ----------------------------------------------------------------------------
+NODE_VARS = ("DX", "DY", "DZ", "VX", "VY", "VZ", "AX", "AY", "AZ", "VRX", "VRY", "VRZ", "ARX", "ARY", "ARZ",
+             "X", "Y", "Z", "TEMP")
+NODE_GENERIC = {"DEF": (1, 2, 3, 4, 5, 6), "D": (1, 2, 3), "V": (4, 5, 6), "A": (7, 8, 9), "VR": (10, 11, 12),
+                "AR": (13, 14, 15), "XYZ": (16, 17, 18)}
+BRIC_VARS = ("OFF", "SX", "SY", "SZ", "SXY", "SYZ", "SXZ", "IE", "DENS", "BULK", "VOL", "PLAS", "TEMP", "PLSR")
+BRIC_GENERIC = {"DEF": (1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 13), "STRESS": (2, 3, 4, 5, 6, 7)}
+SHEL_VARS = ("F1", "F2", "F12", "Q1", "Q2", "M1", "M2", "M12", "IEM", "IEB", "OFF", "THIC", "EMIN", "EMAX")
+SHEL_GENERIC = {"DEF": (1, 2, 3, 6, 7, 8, 9, 10, 11, 13, 14), "STRESS": (1, 2, 3, 4, 5, 6, 7, 8), "PLAS": (13, 14)}
+# the variables of this path (nodes: TEMP needs the thermal solver, so it is not one of them)
+_SUPPORTED = {0: range(1, 19), 1: range(1, 15), 3: range(1, 15)}
+_TABLES = {0: (NODE_VARS, NODE_GENERIC), 1: (BRIC_VARS, BRIC_GENERIC), 3: (SHEL_VARS, SHEL_GENERIC)}
+ITY = {"NODE": 0, "BRIC": 1, "SHEL": 3}   # TH group types (ITHGRP(2))
+
+# the 22 global variables of HIST2 (hist2.F:306-327), in order
+GLOBAL_VARS = ("ENINT", "ENCIN", "XMOMT", "YMOMT", "ZMOMT", "XMASS", "DT2", "ENROT", "WFEXT", "REINT",
+               "ECONT_TOTAL", "EHOUR", "ECONT_ELASTIC", "ECONTV", "ECONTD", "WPLAST", "ADDED_MASS",
+               "ADDED_MASS_PCT", "DM_IN", "DM_OUT", "DE_IN", "DE_OUT")
+
+
+def var_codes(kind: str, names) -> tuple[int, ...]:
+    """The sorted variable codes a /TH/<kind> list of names selects (QVIS is BULK)."""
+    ity = ITY[kind]
+    table, generic = _TABLES[ity]
+    tag = set()
+    for raw in names:
+        v = raw.strip().upper()
+        if not v:
+            continue
+        if v == "QVIS":
+            v = "BULK"
+        if v in generic:
+            tag.update(generic[v])
+        elif v in table:
+            tag.add(table.index(v) + 1)
+        else:
+            raise ThError(f"/TH/{kind}: unknown variable {raw!r}")
+    codes = tuple(sorted(tag))
+    bad = [table[c - 1] for c in codes if c not in _SUPPORTED[ity]]
+    if bad:
+        raise ThError(f"/TH/{kind}: variable(s) {', '.join(bad)} are not computed on this path")
+    return codes
+
+
+# -- encoding -----------------------------------------------------------------------------------
+def encode_reals(values) -> bytes:
+    """Reals as WRTDES writes them: REAL*4 = value, then real_to_IEEE_ASCII (big-endian)."""
+    f = np.asarray(values, dtype=np.float64).astype(np.float32).ravel()
+    if np.isnan(f).any():
+        raise ThError("NaN in a time-history record")
+    d = f.astype(np.float64)  # the encoder compares the float against double constants
+    m, e = np.frexp(np.abs(d))
+    mant = ((m - 0.5) * 1.6777216e7).astype(np.int64)
+    expo = e.astype(np.int64) + 126
+    neg = d < 0.0
+    b0 = ((expo & 0xFE) >> 1) | np.where(neg, 128, 0)
+    b1 = ((expo & 0x01) << 7) | ((mant >> 16) & 0x7F)
+    word = (b0 << 24) | (b1 << 16) | (((mant >> 8) & 0xFF) << 8) | (mant & 0xFF)
+    word = np.where(neg & (d > -1.17e-38), 0x80000000, word)
+    word = np.where(neg & (d < -3.4e38), 0xFF800000, word)
+    word = np.where(~neg & (d < 1.17e-38), 0, word)
+    word = np.where(~neg & (d > 3.4e38), 0x7F800000, word)
+    return word.astype(">u4").tobytes()
+
+
+def _ints(values) -> bytes:
+    return np.asarray(values, dtype=np.int64).astype(">i4").tobytes()
+
+
+def _chars(s: str, n: int) -> bytes:
+    return s.encode("latin-1")[:n].ljust(n, b" ")
+
+
+def _rec(payload: bytes) -> bytes:
+    n = len(payload).to_bytes(4, "big", signed=True)
+    return n + payload + n
+
+
+# -- header -------------------------------------------------------------------------------------
+@dataclasses.dataclass
+class ThGroup:
+    """One /TH/NODE, /TH/BRIC or /TH/SHEL group: entities as model indices, sorted, with the user
+    ids and names the header lists."""
+    kind: str                 # 'NODE', 'BRIC' or 'SHEL'
+    gid: int
+    title: str
+    codes: tuple[int, ...]
+    index: np.ndarray         # model indices of the entities (node, hex or Q4 numbering), ascending
+    uid: np.ndarray           # their user ids
+    names: list[str]
+
+    @classmethod
+    def make(cls, kind: str, gid: int, title: str, names, index, uid, entity_names=None) -> "ThGroup":
+        index = np.asarray(index, np.int64)
+        order = np.argsort(index, kind="stable")   # HORD3: internal number order
+        if len(np.unique(index)) != len(index):
+            raise ThError(f"/TH/{kind}/{gid}: an entity is listed twice")
+        en = list(entity_names) if entity_names is not None else [""] * len(index)
+        return cls(kind, gid, title, var_codes(kind, names), index[order], np.asarray(uid, np.int64)[order],
+                   [en[i] for i in order])
+
+
+@dataclasses.dataclass
+class ThHeader:
+    """What HIST1 writes: the run title, parts (id, title, material and property numbers),
+    materials and properties (id, title), the TH groups."""
+    title: str
+    parts: list[tuple[int, str, int, int]]        # (part id, title, material number, property number), 1-based
+    materials: list[tuple[int, str]]              # deck materials; the Starter's void material is added
+    properties: list[tuple[int, str]]
+    groups: list[ThGroup]
+    vers: int = 40                                # /TH/VERS (40: ICODE 3040, 40-character titles)
+    nglob: int = 22
+    date: str | None = None                       # 24 characters, as ctime writes them
+    program: str = "openradioss_amd gfx950"
+    units: tuple[float, float, float] = (1.0, 1.0, 1.0)
+
+    def __post_init__(self):
+        # the Starter reads /TH/BRIC, then /TH/NODE, then /TH/SHEL groups (hm_read_thgrou.F:2549-2641)
+        rank = {"BRIC": 0, "NODE": 1, "SHEL": 2}
+        self.groups = sorted(self.groups, key=lambda g: rank[g.kind])
+
+
+def _icode(vers: int) -> tuple[int, int]:
+    if vers >= 2021:
+        return 4021, 100
+    if vers >= 50:
+        return 3050, 100
+    if vers >= 47:
+        return 3041, 80
+    return 3040, 40
+
+
+def header_bytes(h: ThHeader) -> bytes:
+    """The records HIST1 writes before the first output time (hist1.F:194-590)."""
+    icode, lt = _icode(h.vers)
+    out = [_rec(_ints([icode]) + _chars(h.title, 80))]
+    date = h.date if h.date is not None else _time.strftime("%a %b %d %H:%M:%S %Y")
+    out.append(_rec(_chars(date, 24) + _chars(" RADIOSS ", 9) + _chars(h.program, 26) + _chars("", 21)))
+    if h.vers >= 50:
+        out += [_rec(_ints([2])), _rec(_ints([lt])), _rec(encode_reals(h.units))]
+    mats = list(h.materials) + [(0, "no_title")]  # the Starter's void material comes last
+    out.append(_rec(_ints([len(h.parts), len(mats), len(h.properties), 1, len(h.groups), h.nglob])))
+    if h.nglob:
+        out.append(_rec(_ints(range(1, h.nglob + 1))))
+    for pid, title, mat, prop in h.parts:           # no /TH/PART: every part has 0 variables
+        out.append(_rec(_ints([pid]) + _chars(title, lt) + _ints([0, mat, prop, 0])))
+    for mid, title in mats:
+        out.append(_rec(_ints([mid]) + _chars(title, lt)))
+    for gid, title in h.properties:
+        out.append(_rec(_ints([gid]) + _chars(title, lt)))
+    # the Starter's root subset holds every part (no /SUBSET on this path)
+    out.append(_rec(_ints([0, 0, 0, len(h.parts), 0]) + _chars("GLOBAL MODEL", lt)))
+    if h.parts:
+        out.append(_rec(_ints(range(1, len(h.parts) + 1))))
+    for g in h.groups:
+        out.append(_rec(_ints([g.gid, ITY[g.kind], 0, len(g.index), len(g.codes)]) + _chars(g.title, lt)))
+        for u, name in zip(g.uid, g.names):
+            out.append(_rec(_ints([int(u)]) + _chars(name, lt)))
+        if g.codes:
+            out.append(_rec(_ints(g.codes)))
+    return b"".join(out)
+
+
+def time_bytes(tt: float, glob, group_values) -> bytes:
+    """One output time (hist2.F:299-330 and the group records): TIME, the global variables, then
+    each group's [entities][variables] values."""
+    out = [_rec(encode_reals([tt]))]
+    g = np.asarray(glob, np.float64)
+    if g.size:
+        out.append(_rec(encode_reals(g)))
+    for v in group_values:
+        v = np.asarray(v, np.float64)
+        if v.size:
+            out.append(_rec(encode_reals(v)))
+    return b"".join(out)
+
+
+def next_time(this: float, dthis: float, tt: float, tstop: float) -> float:
+    """The next output time after writing at TT (hist2.F:264-276, explicit branch)."""
+    return min(tstop, max(tt, this + dthis))
+
+
+# -- reader -------------------------------------------------------------------------------------
+def _records(buf: bytes):
+    p, n = 0, len(buf)
+    while p < n:
+        if p + 4 > n:
+            raise ThError(f"truncated record length at byte {p}")
+        ln = int.from_bytes(buf[p:p + 4], "big", signed=True)
+        if ln < 0 or p + 8 + ln > n or buf[p + 4 + ln:p + 8 + ln] != buf[p:p + 4]:
+            raise ThError(f"bad record framing at byte {p}")
+        yield buf[p + 4:p + 4 + ln]
+        p += 8 + ln
+
+
+def _i(r: bytes):
+    return np.frombuffer(r, dtype=">i4").astype(np.int64)
+
+
+def _s(r: bytes) -> str:
+    return r.decode("latin-1").rstrip()
+
+
+def read(path_or_bytes) -> dict:
+    """Parse a time-history file: the header fields and, per output time, the time, the global
+    variables, the part and subset variables and each group's [entities][variables] array
+    (float32, as stored)."""
+    buf = path_or_bytes if isinstance(path_or_bytes, (bytes, bytearray)) else open(path_or_bytes, "rb").read()
+    recs = list(_records(bytes(buf)))
+    k = 0
+
+    def nxt():
+        nonlocal k
+        if k >= len(recs):
+            raise ThError("file ends inside the header")
+        k += 1
+        return recs[k - 1]
+
+    r = nxt()
+    icode = int(_i(r[:4])[0])
+    lt = {4021: 100, 3050: 100, 3041: 80, 3040: 40}.get(icode)
+    if lt is None:
+        raise ThError(f"unknown time-history code {icode}")
+    out = {"icode": icode, "title": _s(r[4:]), "date": _s(nxt())}
+    if icode >= 3050:
+        out["nrecord"] = int(_i(nxt())[0])
+        out["ltitl"] = int(_i(nxt())[0])
+        out["units"] = np.frombuffer(nxt(), ">f4").astype(np.float64)
+    npart, nummat, numgeo, nsubs, nthgrp, nglob = (int(v) for v in _i(nxt()))
+    out["nglob"] = nglob
+    out["glob_codes"] = _i(nxt()) if nglob else np.zeros(0, np.int64)
+    parts, mats, props, subsets, groups = [], [], [], [], []
+    for _ in range(npart):
+        r = nxt()
+        pid = int(_i(r[:4])[0])
+        sid, mat, prop, nvar = (int(v) for v in _i(r[4 + lt:]))
+        codes = _i(nxt()) if nvar else np.zeros(0, np.int64)
+        parts.append({"id": pid, "title": _s(r[4:4 + lt]), "subset": sid, "mat": mat, "prop": prop, "codes": codes})
+    for lst, n in ((mats, nummat), (props, numgeo)):
+        for _ in range(n):
+            r = nxt()
+            lst.append((int(_i(r[:4])[0]), _s(r[4:])))
+    for _ in range(nsubs):
+        r = nxt()
+        sid, parent, nchild, nps, nvar = (int(v) for v in _i(r[:20]))
+        sub = {"id": sid, "parent": parent, "title": _s(r[20:]),
+               "children": _i(nxt()) if nchild else np.zeros(0, np.int64),
+               "parts": _i(nxt()) if nps else np.zeros(0, np.int64)}
+        sub["codes"] = _i(nxt()) if nvar else np.zeros(0, np.int64)
+        subsets.append(sub)
+    for _ in range(nthgrp):
+        r = nxt()
+        gid, ity, _zero, nn, nvar = (int(v) for v in _i(r[:20]))
+        ents = [nxt() for _ in range(nn)]
+        groups.append({"id": gid, "ity": ity, "title": _s(r[20:]), "uid": np.array([int(_i(e[:4])[0]) for e in ents]),
+                       "names": [_s(e[4:]) for e in ents], "codes": _i(nxt()) if nvar else np.zeros(0, np.int64)})
+    out.update(parts=parts, materials=mats, properties=props, subsets=subsets, groups=groups)
+    out["header_records"] = k
+    npv = sum(len(p["codes"]) for p in parts)
+    nsv = sum(len(s["codes"]) for s in subsets)
+    times = []
+    while k < len(recs):
+        rec = {"time": float(np.frombuffer(nxt(), ">f4")[0])}
+        rec["glob"] = np.frombuffer(nxt(), ">f4") if nglob else np.zeros(0, np.float32)
+        rec["parts"] = np.frombuffer(nxt(), ">f4") if npv else np.zeros(0, np.float32)
+        rec["subsets"] = np.frombuffer(nxt(), ">f4") if nsv else np.zeros(0, np.float32)
+        vals = []
+        for g in groups:
+            n = len(g["uid"]) * len(g["codes"])
+            vals.append(np.frombuffer(nxt(), ">f4").reshape(len(g["uid"]), len(g["codes"])) if n else
+                        np.zeros((len(g["uid"]), len(g["codes"])), np.float32))
+        rec["groups"] = vals
+        times.append(rec)
+    out["times"] = times
+    return out

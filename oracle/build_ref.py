@@ -178,11 +178,31 @@ def build_library(name, leaves, drivers, roots, inc_dirs, objdir, c_sources=()):
     return len(done)
 
 
+def build_ieee() -> int:
+    """The time-history file's number encoders (engine/source/output/tools/ieee.cpp, one C++ file)
+    as oracle/_ref/libref_ieee.so."""
+    src = "engine/source/output/tools/ieee.cpp"
+    lib = os.path.join(OUT, "libref_ieee.so")
+    # the machine and precision macros of the reference's own gfortran build (cmake_linux64_gf.txt:16-17, 134)
+    defs = ["-DCPP_mach=CPP_p4linux964", "-DCPP_rel=80", "-DMYREAL8", "-std=c++14"]
+    r = subprocess.run(["g++", "-O2", "-fPIC", "-shared", *defs, f"-I{REF}/engine/share/spe_inc",
+                        os.path.join(REF, src), "-o", lib], capture_output=True, text=True)
+    if r.returncode != 0:
+        print(r.stderr[-2000:])
+        return -1
+    with open(os.path.join(OUT, "libref_ieee_MANIFEST.txt"), "w") as fh:
+        fh.write(f"compiler: g++ -O2 {' '.join(defs)}\n{src}\n")
+    return 1
+
+
 def main():
     if not os.path.isdir(REF):
         print("reference tree absent: oracle/_ref not built")
         return 0
     os.makedirs(OUT, exist_ok=True)
+    if build_ieee() < 0:
+        return 1
+    print("oracle/_ref/libref_ieee.so built from engine/source/output/tools/ieee.cpp")
     n = build_library("libref.so", LEAVES, DRIVERS, ("common_source", "engine/source", "engine/share"),
                       ("engine/share/includes", "engine/share/spe_inc", "engine/share/r8", "common_source/includes",
                        "engine/share/message_module"), OBJ)
