@@ -176,6 +176,24 @@ typedef struct {
 int or_model_run_ecrit(or_model *m, or_ecrit *out);
 /* The external work of the imposed velocities so far (as in or_ecrit.wfext).  Synchronous. */
 int or_model_wfext(or_model *m, double *wfext);
+/* Time history (HIST2, hist2.F:299-330 and 959-999, called from sortie_main.F:1049 at ECRIT's
+ * point of the cycle).  or_model_th_setup lists the entities of the /TH/NODE, /TH/BRIC and
+ * /TH/SHEL groups (0-based node, hex and shell indices, each node once) and starts the EHOUR
+ * accumulation (shvis3.F:464, chvis3.F:397: every cycle adds its elements' hourglass energy);
+ * a later call replaces the lists and keeps EHOUR. */
+int or_model_th_setup(or_model *m, int64_t n_node, const int32_t *node, int64_t n_hex, const int32_t *hex,
+                      int64_t n_q4, const int32_t *q4);
+/* One output cycle (as or_model_run_ecrit, whose record it fills) with the time-history values:
+ * glob = XMOMT, YMOMT, ZMOMT (ecrit.F:226-228, (V+DT05*A)*MS summed over the nodes) and EHOUR;
+ * node_out[n_node][18] = D, V, A, VR, AR, X as THNOD reads them (thnod.F:59-97: D and X before
+ * DEPLA, V after BCS10, A after FIXVEL; VR, AR zero without rotational DOFs);
+ * hex_out[n_hex][13] = OFF, SIG(6), EINT, RHO, QVIS, VOL0, PLA, EPSD of THSOL (thsol.F:189-200,
+ * 511-548); q4_out[n_q4][12+NPT] = FOR(5), MOM(3), EINT(2), OFF, THK and each point's PLA of
+ * THCOQ (thcoq.F:296-340, 452-520).  Synchronous. */
+int or_model_run_th(or_model *m, or_ecrit *ec, double glob[4], double *node_out, double *hex_out, double *q4_out);
+/* EHOUR so far; and its value set (a restart), after or_model_th_setup.  Synchronous. */
+int or_model_ehour(or_model *m, double *ehour);
+int or_model_set_ehour(or_model *m, double ehour);
 /* Any output pointer may be NULL. */
 int or_model_download_nodes(or_model *m, double *x, double *v, double *vr, double *d);
 int or_model_download_hex(or_model *m, double *sig, double *eint, double *rho, double *pla,

@@ -181,7 +181,10 @@ struct ModelDev {
   DevTime *ts;                // [2] cycle scalars, by cycle parity
   unsigned long long *dtmin;  // [2] element time-step minimum (FP64 bits), by parity
   int *flags;                 // bit 0: an element reported MSTOP
-  double *ecrit;              // [2][nn] per-node kinetic terms of ECRIT on output cycles, or null
+  double *ecrit;              // [5][nn] per-node kinetic and momentum terms of ECRIT on output cycles, or null
+  const int32_t *th_slot;     // [nn] the node's row in th_node, or -1 (time history), or null
+  double *th_node;            // [rows][18] D, V, A, VR, AR, X of the time-history nodes on output cycles
+  double *ehou_h, *ehou_q;    // per tile: its elements' hourglass energy summed over the cycles, or null
   double *dtel_h, *dtel_q;    // per-element time-step candidates on output cycles (storage order), or null
   double dtmx;                // DTMX
   double tend;                // the model stops after the cycle whose time reaches tend
@@ -321,6 +324,26 @@ template <int TB> __device__ __forceinline__ void tile_min_commit(const double *
   }
 }
 
+// The hourglass energy (EHOU) of a tile's elements: each element stores its EHOU in the tile's
+// LDS as soon as it is known (no register holds it through the rest of the chain); after the
+// tile's barrier the first wave sums them in a fixed order (per lane its elements in order,
+// then a shuffle tree) and adds the sum to the tile's accumulator, which only this tile
+// writes, so the sum is the same on every run.
+template <int TB, typename R> __device__ __forceinline__ void tile_ehou_commit(const R *eh, int64_t first, int64_t ne,
+                                                                               double *acc) {
+  if (threadIdx.x < 64) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < TB / 64; ++k) {
+      const int t = threadIdx.x + 64 * k;
+      if (first + t < ne) v = v + double(eh[t]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    if (threadIdx.x == 0) *to_global(acc) = *to_global(acc) + v;
+  }
+}
+
 // ---------------------------------------------------------------- hex element kernel
 // In the split form, the hex element state read by the material phase (SIG(6), EINT, RHO,
 // VOL0, PLA, EPSD, QVIS) is copied from HBM to LDS by the wave itself at the start of the chain, with
@@ -354,7 +377,7 @@ __device__ __forceinline__ void stage_hex_state(S *sst, const HexPtrs<S> &e, int
 // The wave's global-to-LDS loads have landed (they count as vector memory loads).
 __device__ __forceinline__ void wait_staged() { __asm__ __volatile__("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-template <typename R, typename S, int TB, int W>
+template <typename R, typename S, int TB, int W, bool EH>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_hex_forces(
     const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity_output) {
   static_assert(TB == kTile, "a hex workgroup is one assembly tile");
@@ -374,6 +397,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   const int64_t i = (int64_t)tl.first + threadIdx.x;
   double dtx = kEP06;
   int mstop = 0;
+  __shared__ R seh[EH ? TB : 1];
   // the split form stages its element state through LDS; the FP64 chain, bound by its
   // arithmetic at two waves per SIMD, reads it from HBM after the kinematics
   constexpr bool kStage = sizeof(S) == 4;
@@ -383,6 +407,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
     const R DT1 = R(time_dt2(ts + parity));
     HexState<S> st;
     HexOut<R> o;
+    // EH: the model keeps EHOUR (its own instantiation, so that the other keeps its registers)
+    auto ehou_sink = [&](R v) {
+      if constexpr (EH) seh[threadIdx.x] = v;
+    };
     double xd[8], yd[8], zd[8];
     R vx[8], vy[8], vz[8];
 #pragma unroll
@@ -419,12 +447,12 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
         const S w[3] = {S(fx), S(fy), S(fz)};
         lds_corner<S, 3>(lds, k, w);
       };
-      hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, write_row, load_rest);
+      hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, write_row, load_rest, ehou_sink);
     } else {
       // FP32: the forces are gathered first and stored together (registers are not the limit)
       R f1[8], f2[8], f3[8];
       auto keep = [&](int k, R fx, R fy, R fz) { f1[k] = fx; f2[k] = fy; f3[k] = fz; };
-      hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, keep, load_rest);
+      hex_element<R, S>(xd, yd, zd, vx, vy, vz, st, c, DT1, o, keep, load_rest, ehou_sink);
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const S w[3] = {S(f1[k]), S(f2[k]), S(f3[k])};
@@ -450,6 +478,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   if (mstop) atomicOr(M.flags, kMstopJacobian);
   __syncthreads();
   tile_min_commit<TB>(red, M.dtmin + parity);
+  if constexpr (EH) {
+    if (M.ehou_h) tile_ehou_commit<TB>(seh, tl.first, e.ne, M.ehou_h + tl.first / kTile);
+  }
   ta.finish(lds, to_global<S>(M.rowh), to_global(M.hfr_node), to_global<const v4u>(M.hfr_src), to_global(M.preh));
 }
 
@@ -503,7 +534,7 @@ template <typename R, typename S> struct Q4IpIO {
   }
 };
 
-template <typename R, typename S, int TB, int W>
+template <typename R, typename S, int TB, int W, bool EH>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_q4_forces(
     const ModelDev *__restrict__ md, const Tile *__restrict__ tiles, int parity_output) {
   static_assert(TB == kTile, "a shell workgroup is one assembly tile");
@@ -520,6 +551,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   const Q4Consts &c = M.qc;
   const int64_t i = (int64_t)tl.first + threadIdx.x;
   double dt = kEP06;
+  __shared__ R seh[EH ? TB : 1];
   if (i < q.ne) {
     const R DT1 = R(time_dt2(ts + parity));
     double xg[4], yg[4], zg[4];
@@ -540,6 +572,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
     st.offg = R(q.offg[i]); st.thk = R(q.thk[i]); st.thke = R(q.thke[i]); st.epsd = R(q.epsd[i]);
     Q4IpIO<R, S> io{q, i, !stopped};
     Q4Out<R> o;
+    // EH: the model keeps EHOUR (its own instantiation, so that the other keeps its registers)
+    auto ehou_sink = [&](R v) {
+      if constexpr (EH) seh[threadIdx.x] = v;
+    };
     auto nload = [&](int k, double(&x)[3], R(&v)[3]) {
       __asm__ __volatile__("" ::: "memory"); // a fresh load, not the values held since the gather
       double a, b, cc, u, w, z;
@@ -547,7 +583,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
       x[0] = a; x[1] = b; x[2] = cc;
       v[0] = R(u); v[1] = R(w); v[2] = R(z);
     };
-    q4_element<R>(xg, yg, zg, vl, vrl, nc[2] == nc[3], st, c, DT1, io, nload, o);
+    q4_element<R>(xg, yg, zg, vl, vrl, nc[2] == nc[3], st, c, DT1, io, nload, o, ehou_sink);
     if (!stopped) {
       q.offg[i] = S(st.offg); q.thk[i] = S(st.thk); q.epsd[i] = S(st.epsd);
 #pragma unroll
@@ -567,6 +603,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(W))) void k_
   wave_min_to(red, dt);
   __syncthreads();
   tile_min_commit<TB>(red, M.dtmin + parity);
+  if constexpr (EH) {
+    if (M.ehou_q) tile_ehou_commit<TB>(seh, tl.first, q.ne, M.ehou_q + tl.first / kTile);
+  }
   ta.finish(lds, to_global<S>(M.rowq), to_global(M.qfr_node), to_global<const v4u>(M.qfr_src), to_global(M.preq));
 }
 
@@ -753,6 +792,32 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
       kr = (sx * sx + sy * sy + sz * sz) * (1.0 / 2.0) * inn;
     }
     ke[nd.nn + n] = kr;
+    // XMOMT, YMOMT, ZMOMT (ecrit.F:226-228)
+    ke[2 * nd.nn + n] = ux * msn;
+    ke[3 * nd.nn + n] = uy * msn;
+    ke[4 * nd.nn + n] = uz * msn;
+    // the time history's nodal values at the same point (thnod.F:59-97): D and X before
+    // DEPLA, V after BCS10, A after FIXVEL, VR and AR (zero without rotational DOFs)
+    const int32_t slot = M.th_slot ? to_global(M.th_slot)[n] : -1;
+    if (slot >= 0) {
+      const gptr<double> t = to_global(M.th_node) + 18 * (int64_t)slot;
+      const gptr<const double> dn = to_global(nd.d) + 3 * n;
+      double qx = 0.0, qy = 0.0, qz = 0.0;
+      if (nd.vr) {
+        qx = wx; qy = wy; qz = wz;
+        if (!pr) {
+          const gptr<const v2d> q = (gptr<const v2d>)to_global(nd.vr + 4 * n);
+          const v2d q0 = q[0], q1 = q[1];
+          qx = q0.x; qy = q0.y; qz = q1.x;
+        }
+      }
+      t[0] = dn[0]; t[1] = dn[1]; t[2] = dn[2];
+      t[3] = vx; t[4] = vy; t[5] = vz;
+      t[6] = ax; t[7] = ay; t[8] = az;
+      t[9] = qx; t[10] = qy; t[11] = qz;
+      t[12] = rx; t[13] = ry; t[14] = rz;
+      t[15] = x; t[16] = y; t[17] = z;
+    }
   }
   // VELOCITY (velocity.F:58-86)
   vx = vx + DT12 * ax; vy = vy + DT12 * ay; vz = vz + DT12 * az;
@@ -773,6 +838,8 @@ __global__ __launch_bounds__(kBlock) void k_node_update(const ModelDev *__restri
 }
 
 // ---------------------------------------------------------------- energies (fixed-order)
+// ENINT, ENCIN, ENROT, WFEXT, XMOMT, YMOMT, ZMOMT
+constexpr int kEnergySums = 7;
 __device__ __forceinline__ double block_sum(double v) {
   __shared__ double red[kBlock];
   red[threadIdx.x] = v;
@@ -792,7 +859,7 @@ __global__ __launch_bounds__(kBlock) void k_energy_partial(const S *heint, const
                                                            const double *vr, const double *ms, const double *in,
                                                            const double *ecrit, const double *wext, int64_t nn,
                                                            int64_t per_block, double *partial) {
-  double ie = 0.0, ke = 0.0, kr = 0.0, we = 0.0;
+  double ie = 0.0, ke = 0.0, kr = 0.0, we = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
   const int64_t b0 = (int64_t)blockIdx.x * per_block;
   for (int64_t j = b0 + threadIdx.x; j < b0 + per_block; j += kBlock) {
     if (j < nn && wext) we = we + wext[j];
@@ -802,6 +869,9 @@ __global__ __launch_bounds__(kBlock) void k_energy_partial(const S *heint, const
       // per-node terms written by the node kernel on an output cycle
       ke = ke + ecrit[j];
       kr = kr + ecrit[nn + j];
+      mx = mx + ecrit[2 * nn + j];
+      my = my + ecrit[3 * nn + j];
+      mz = mz + ecrit[4 * nn + j];
     } else if (j < nn) {
       const double *p = xv + 6 * j;
       ke = ke + ms[j] * (p[3] * p[3] + p[4] * p[4] + p[5] * p[5]);
@@ -811,36 +881,30 @@ __global__ __launch_bounds__(kBlock) void k_energy_partial(const S *heint, const
       }
     }
   }
-  ie = block_sum(ie);
-  ke = block_sum(ke);
-  kr = block_sum(kr);
-  we = block_sum(we);
+  const double v[kEnergySums] = {block_sum(ie), block_sum(ke), block_sum(kr), block_sum(we),
+                                 block_sum(mx), block_sum(my), block_sum(mz)};
   if (threadIdx.x == 0) {
-    partial[4 * blockIdx.x + 0] = ie;
-    partial[4 * blockIdx.x + 1] = ke;
-    partial[4 * blockIdx.x + 2] = kr;
-    partial[4 * blockIdx.x + 3] = we;
+#pragma unroll
+    for (int k = 0; k < kEnergySums; ++k) partial[kEnergySums * blockIdx.x + k] = v[k];
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_energy_final(const double *partial, int nb, double kin_factor,
                                                          double *out) {
-  double ie = 0.0, ke = 0.0, kr = 0.0, we = 0.0;
+  double a[kEnergySums];
+#pragma unroll
+  for (int k = 0; k < kEnergySums; ++k) a[k] = 0.0;
   for (int j = threadIdx.x; j < nb; j += kBlock) {
-    ie = ie + partial[4 * j];
-    ke = ke + partial[4 * j + 1];
-    kr = kr + partial[4 * j + 2];
-    we = we + partial[4 * j + 3];
+#pragma unroll
+    for (int k = 0; k < kEnergySums; ++k) a[k] = a[k] + partial[kEnergySums * j + k];
   }
-  ie = block_sum(ie);
-  ke = block_sum(ke);
-  kr = block_sum(kr);
-  we = block_sum(we);
+#pragma unroll
+  for (int k = 0; k < kEnergySums; ++k) a[k] = block_sum(a[k]);
   if (threadIdx.x == 0) {
-    out[0] = ie;
-    out[1] = kin_factor * ke;
-    out[2] = kin_factor * kr;
-    out[3] = we;
+    out[0] = a[0];
+    out[1] = kin_factor * a[1];
+    out[2] = kin_factor * a[2];
+    for (int k = 3; k < kEnergySums; ++k) out[k] = a[k];
   }
 }
 
@@ -874,6 +938,49 @@ __global__ __launch_bounds__(kBlock) void k_ctrl_pick(const double *dt, const in
       else atomicMin(pick, e);
     }
   }
+}
+
+// ---------------------------------------------------------------- time history
+// The element values of THSOL (thsol.F:189-200, 511-548) and THCOQ (thcoq.F:296-340, 452-520)
+// that the time-history file reads from the element buffers, for the listed elements (storage
+// slots): hexes OFF, SIG(6), EINT, RHO, QVIS, VOL0, PLA, EPSD; shells FOR(5), MOM(3), EINT(2),
+// OFF, THK and the PLA of each through-thickness point.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_th_elements(const ModelDev *__restrict__ md, const int32_t *hex,
+                                                        int64_t nh, const int32_t *q4, int64_t nq, int npt,
+                                                        double *hout, double *qout) {
+  const ModelDev &M = md[0];
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j < nh) {
+    const HexPtrs<S> e = hex_ptrs<S>(M);
+    const int64_t i = hex[j];
+    const gptr<double> o = to_global(hout) + 13 * j;
+    o[0] = double(e.offg[i]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o[1 + k] = double(e.sig[(int64_t)k * e.ne + i]);
+    o[7] = double(e.eint[i]); o[8] = double(e.rho[i]); o[9] = double(e.qvis[i]); o[10] = double(e.vol0[i]);
+    o[11] = double(e.pla[i]); o[12] = double(e.epsd[i]);
+  } else if (j < nh + nq) {
+    const Q4Ptrs<S> q = q4_ptrs<S>(M);
+    const int64_t i = q4[j - nh];
+    const gptr<double> o = to_global(qout) + (12 + npt) * (j - nh);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) o[k] = double(q.forc[(int64_t)k * q.ne + i]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[5 + k] = double(q.mom[(int64_t)k * q.ne + i]);
+    o[8] = double(q.eint[i]); o[9] = double(q.eint[q.ne + i]);
+    o[10] = double(q.offg[i]); o[11] = double(q.thk[i]);
+    for (int k = 0; k < npt; ++k) o[12 + k] = double(q.pla[(int64_t)k * q.ne + i]);
+  }
+}
+
+// EHOUR: the tiles' accumulators, solid tiles then shell tiles, summed in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_th_ehour(const double *h, int64_t nh, const double *q, int64_t nq,
+                                                     double *out) {
+  double v = 0.0;
+  for (int64_t j = threadIdx.x; j < nh + nq; j += kBlock) v = v + (j < nh ? h[j] : q[j - nh]);
+  v = block_sum(v);
+  if (threadIdx.x == 0) *out = v;
 }
 
 // ---------------------------------------------------------------- group drop-in kernels
@@ -1200,8 +1307,14 @@ struct or_model {
   DevTime *ts = nullptr;               // [2]
   unsigned long long *dtmin = nullptr; // [2]
   int *flags = nullptr;
-  double *energy = nullptr;            // [4 + 4*nb]
-  double *ecrit = nullptr;             // [2][nn] per-node ECRIT terms, allocated at the first output cycle
+  double *energy = nullptr;            // [kEnergySums * (1 + nb)]
+  double *ecrit = nullptr;             // [5][nn] per-node ECRIT terms, allocated at the first output cycle
+  double momentum[3] = {0.0, 0.0, 0.0}; // XMOMT, YMOMT, ZMOMT of the last output cycle
+  // time history: the rows of its nodes (th_slot), their output-cycle values and the sampled
+  // elements (th_buf: nodes [n][18], hexes [n][13], shells [n][12 + npt]), EHOUR per tile
+  int32_t *th_slot = nullptr, *th_hex = nullptr, *th_q4 = nullptr;
+  double *th_buf = nullptr, *ehou_h = nullptr, *ehou_q = nullptr;
+  int64_t th_nn = 0, th_nh = 0, th_nq = 0;
   double *wext = nullptr;              // [2][nn] external work per node (and its deferred part), when
                                        // the model has imposed DOFs
   // the controlling element of output cycles: per-element candidates (storage order), the
@@ -1375,7 +1488,7 @@ int or_model_create(or_model **out, int device, int64_t nn, int64_t ne_hex, int6
   A(&m->tiles_q, (ne_q4 + kTile - 1) / kTile);
   A(&m->tiles_n, m->nt_n);
   const int64_t nb = 1024;
-  A(&m->energy, 4 + 4 * nb);
+  A(&m->energy, kEnergySums * (1 + nb));
   if (rc == OR_OK && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(OR_ERR_HIP, "hipStreamCreate");
   if (rc == OR_OK) {
@@ -1429,7 +1542,7 @@ int or_model_destroy(or_model *m) {
                   m->qgstr, m->qhourg, m->qeint, m->qepsd,
                   m->qsig, m->qpla, m->qepsdip, m->rowq, m->ts, m->dtmin, m->flags, m->energy, m->dev,
                   m->tiles_h, m->tiles_q, m->tiles_n, m->ecrit, m->wext, m->dtel_h, m->dtel_q, m->lidx_h,
-                  m->lidx_q, m->ctrl};
+                  m->lidx_q, m->ctrl, m->th_slot, m->th_hex, m->th_q4, m->th_buf, m->ehou_h, m->ehou_q};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
@@ -1842,6 +1955,7 @@ static void fill_dev(or_model *m, ModelDev &d, double tend) {
   d.preh = m->preh; d.preq = m->preq;
   d.ts = m->ts; d.dtmin = m->dtmin; d.flags = m->flags; d.ecrit = m->ecrit;
   d.dtel_h = m->dtel_h; d.dtel_q = m->dtel_q;
+  d.th_slot = m->th_slot; d.th_node = m->th_buf; d.ehou_h = m->ehou_h; d.ehou_q = m->ehou_q;
   d.dtmx = m->neh > 0 ? m->hc.dtmx : 1.0e20;
   d.tend = tend;
   d.hc = m->hc;
@@ -1866,34 +1980,46 @@ struct LaunchSet {
   const Tile *th = nullptr, *tq = nullptr, *tn = nullptr;
   int32_t nth = 0, ntq = 0, ntn = 0;
   int vh = 0, vq = 0; // element kernel variants (their block sizes match th / tq)
+  bool eh = false, eq = false; // the set keeps EHOUR (hexes, shells): one model with a time history
 };
 
 // v & 1 selects the register budget (2 or 3 waves per SIMD); the caller routes v >= 2 to
 // the unit built with the ILP-first scheduler.
-template <typename R, typename S>
-static void launch_hex(int v, hipStream_t st, const LaunchSet &L, int parity) {
+template <typename R, typename S, bool EH>
+static void launch_hex_eh(int v, hipStream_t st, const LaunchSet &L, int parity) {
   constexpr int W0 = variant_waves<R>(0), W1 = variant_waves<R>(1);
   const dim3 g((unsigned)L.nth);
-  if ((v & 1) == 0) hipLaunchKernelGGL((k_hex_forces<R, S, kTile, W0>), g, dim3(kTile), 0, st, L.md, L.th, parity);
-  else hipLaunchKernelGGL((k_hex_forces<R, S, kTile, W1>), g, dim3(kTile), 0, st, L.md, L.th, parity);
+  if ((v & 1) == 0) hipLaunchKernelGGL((k_hex_forces<R, S, kTile, W0, EH>), g, dim3(kTile), 0, st, L.md, L.th, parity);
+  else hipLaunchKernelGGL((k_hex_forces<R, S, kTile, W1, EH>), g, dim3(kTile), 0, st, L.md, L.th, parity);
+}
+template <typename R, typename S>
+static void launch_hex(int v, hipStream_t st, const LaunchSet &L, int parity) {
+  if (L.eh) launch_hex_eh<R, S, true>(v, st, L, parity);
+  else launch_hex_eh<R, S, false>(v, st, L, parity);
 }
 
-template <typename R, typename S>
-static void launch_q4(int v, hipStream_t st, const LaunchSet &L, int parity) {
+template <typename R, typename S, bool EH>
+static void launch_q4_eh(int v, hipStream_t st, const LaunchSet &L, int parity) {
   constexpr int W0 = variant_waves<R>(0), W1 = variant_waves<R>(1);
   const dim3 g((unsigned)L.ntq);
-  if ((v & 1) == 0) hipLaunchKernelGGL((k_q4_forces<R, S, kTile, W0>), g, dim3(kTile), 0, st, L.md, L.tq, parity);
-  else hipLaunchKernelGGL((k_q4_forces<R, S, kTile, W1>), g, dim3(kTile), 0, st, L.md, L.tq, parity);
+  if ((v & 1) == 0) hipLaunchKernelGGL((k_q4_forces<R, S, kTile, W0, EH>), g, dim3(kTile), 0, st, L.md, L.tq, parity);
+  else hipLaunchKernelGGL((k_q4_forces<R, S, kTile, W1, EH>), g, dim3(kTile), 0, st, L.md, L.tq, parity);
+}
+template <typename R, typename S>
+static void launch_q4(int v, hipStream_t st, const LaunchSet &L, int parity) {
+  if (L.eq) launch_q4_eh<R, S, true>(v, st, L, parity);
+  else launch_q4_eh<R, S, false>(v, st, L, parity);
 }
 
 // Element launches of variants 2..3 (variant - 2 in the unit built with the ILP-first
 // scheduler): family 0 hexes, 1 shells; fp64 selects <double,double> or <float,float>.
 void or_ilp_elements(int family, int fp64, int variant, hipStream_t st, const void *md, const void *tiles,
-                     int32_t ntiles, int parity)
+                     int32_t ntiles, int parity, bool ehour)
 #ifdef OR_ILP_TU
 {
   LaunchSet L;
   L.md = (const ModelDev *)md;
+  L.eh = L.eq = ehour;
   if (family == 0) {
     L.th = (const Tile *)tiles; L.nth = ntiles;
     if (fp64) launch_hex<double, double>(variant, st, L, parity);
@@ -1913,13 +2039,13 @@ static int launch_cycle(hipStream_t st, const LaunchSet &L, int parity, hipEvent
   constexpr int fp64 = sizeof(R) == 8 ? 1 : 0;
   const int po = parity | (output ? 2 : 0); // element kernels: parity and the output-cycle bit
   if (L.nth > 0) {
-    if (L.vh >= 2) or_ilp_elements(0, fp64, L.vh - 2, st, L.md, L.th, L.nth, po);
+    if (L.vh >= 2) or_ilp_elements(0, fp64, L.vh - 2, st, L.md, L.th, L.nth, po, L.eh);
     else launch_hex<R, S>(L.vh, st, L, po);
     OR_HIP(hipGetLastError());
   }
   if (after_hex) OR_HIP(hipEventRecord(after_hex, st));
   if (L.ntq > 0) {
-    if (L.vq >= 2) or_ilp_elements(1, fp64, L.vq - 2, st, L.md, L.tq, L.ntq, po);
+    if (L.vq >= 2) or_ilp_elements(1, fp64, L.vq - 2, st, L.md, L.tq, L.ntq, po, L.eq);
     else launch_q4<R, S>(L.vq, st, L, po);
     OR_HIP(hipGetLastError());
   }
@@ -1945,6 +2071,7 @@ int or_model_run(or_model *m, int64_t ncycles) {
   LaunchSet L;
   L.md = m->dev;
   L.th = m->tiles_h; L.nth = m->nt_h; L.vh = m->var_h;
+  L.eh = m->ehou_h != nullptr; L.eq = m->ehou_q != nullptr;
   L.tq = m->tiles_q; L.ntq = m->nt_q; L.vq = m->var_q;
   L.tn = m->tiles_n; L.ntn = m->nt_n;
   for (int64_t c = 0; c < ncycles; ++c) {
@@ -1970,7 +2097,7 @@ int or_model_run(or_model *m, int64_t ncycles) {
   return OR_OK;
 }
 
-static int energy_sums(or_model *m, const double *ecrit, double out[4]);
+static int energy_sums(or_model *m, const double *ecrit, double out[kEnergySums]);
 
 // Per-element candidates of the controlling element and each storage slot's element index,
 // allocated (and the slot indices uploaded) at the first output cycle.
@@ -2005,8 +2132,8 @@ int or_model_run_ecrit(or_model *m, or_ecrit *out) {
   OR_HIP(hipSetDevice(m->device));
   if (!m->ecrit) {
     OR_HIP(quiesce(m));
-    OR_HIP(hipMalloc((void **)&m->ecrit, sizeof(double) * 2 * (size_t)m->nn));
-    m->bytes += (int64_t)(sizeof(double) * 2 * m->nn);
+    OR_HIP(hipMalloc((void **)&m->ecrit, sizeof(double) * 5 * (size_t)m->nn));
+    m->bytes += (int64_t)(sizeof(double) * 5 * m->nn);
     mark_changed(m);
   }
   int rc = ensure_ctrl(m);
@@ -2025,6 +2152,7 @@ int or_model_run_ecrit(or_model *m, or_ecrit *out) {
   LaunchSet L;
   L.md = m->dev;
   L.th = m->tiles_h; L.nth = m->nt_h; L.vh = m->var_h;
+  L.eh = m->ehou_h != nullptr; L.eq = m->ehou_q != nullptr;
   L.tq = m->tiles_q; L.ntq = m->nt_q; L.vq = m->var_q;
   L.tn = m->tiles_n; L.ntn = m->nt_n;
   const int p = (int)(m->parity & 1);
@@ -2032,9 +2160,10 @@ int or_model_run_ecrit(or_model *m, or_ecrit *out) {
                                  : launch_cycle<float, float>(m->stream, L, p, nullptr, nullptr, 1);
   if (rc) return rc;
   m->parity ^= 1;
-  double e[4];
+  double e[kEnergySums];
   if ((rc = energy_sums(m, m->ecrit, e))) return rc;
   out->enint = e[0]; out->encin = e[1]; out->enrot = e[2]; out->wfext = e[3];
+  m->momentum[0] = e[4]; m->momentum[1] = e[5]; m->momentum[2] = e[6];
   // the controlling element (k_ctrl_min / k_ctrl_pick)
   unsigned long long w0[4];
   const double big = kEP06;
@@ -2074,6 +2203,128 @@ int or_model_run_output(or_model *m, double out[3]) {
   const int rc = or_model_run_ecrit(m, &e);
   if (rc) return rc;
   out[0] = e.enint; out[1] = e.encin; out[2] = e.enrot;
+  return OR_OK;
+}
+
+// ---------------------------------------------------------------- time history (HIST2)
+int or_model_th_setup(or_model *m, int64_t n_node, const int32_t *node, int64_t n_hex, const int32_t *hex,
+                      int64_t n_q4, const int32_t *q4) {
+  if (!m) return fail(OR_ERR_ARG, "null model");
+  if (n_node < 0 || n_hex < 0 || n_q4 < 0 || (n_node && !node) || (n_hex && !hex) || (n_q4 && !q4))
+    return fail(OR_ERR_ARG, "entity counts must be >= 0 with their lists given");
+  if (m->batch) return fail(OR_ERR_STATE, "the model belongs to a batch");
+  if (!m->have_nodes || !m->have_sky || (m->neh > 0 && !m->have_hex) || (m->neq > 0 && !m->have_q4))
+    return fail(OR_ERR_STATE, "nodes, element state and skyline must be uploaded before the time history");
+  std::vector<int32_t> slot((size_t)m->nn, -1);
+  for (int64_t j = 0; j < n_node; ++j) {
+    if (node[j] < 0 || node[j] >= m->nn) return fail(OR_ERR_ARG, "time-history node out of range");
+    if (slot[(size_t)node[j]] >= 0) return fail(OR_ERR_ARG, "time-history node listed twice");
+    slot[(size_t)node[j]] = (int32_t)j;
+  }
+  auto slots = [&](const int32_t *ids, int64_t n, int64_t ne, const std::vector<int32_t> &perm,
+                   std::vector<int32_t> &out) -> int {
+    std::vector<int32_t> inv((size_t)ne);
+    for (int64_t s = 0; s < ne; ++s) inv[(size_t)(perm.empty() ? s : perm[(size_t)s])] = (int32_t)s;
+    out.resize((size_t)n);
+    for (int64_t j = 0; j < n; ++j) {
+      if (ids[j] < 0 || ids[j] >= ne) return fail(OR_ERR_ARG, "time-history element out of range");
+      out[(size_t)j] = inv[(size_t)ids[j]];
+    }
+    return OR_OK;
+  };
+  std::vector<int32_t> hs, qs;
+  int rc = slots(hex, n_hex, m->neh, m->hperm, hs);
+  if (rc == OR_OK) rc = slots(q4, n_q4, m->neq, m->qperm, qs);
+  if (rc) return rc;
+  OR_HIP(hipSetDevice(m->device));
+  OR_HIP(quiesce(m));
+  for (void **p : {(void **)&m->th_slot, (void **)&m->th_hex, (void **)&m->th_q4, (void **)&m->th_buf})
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+  const int64_t words = 18 * n_node + 13 * n_hex + (12 + m->npt) * n_q4 + 1;
+  OR_HIP(hipMalloc((void **)&m->th_slot, sizeof(int32_t) * (size_t)m->nn));
+  OR_HIP(hipMemcpy(m->th_slot, slot.data(), sizeof(int32_t) * (size_t)m->nn, hipMemcpyHostToDevice));
+  OR_HIP(hipMalloc((void **)&m->th_buf, sizeof(double) * (size_t)words));
+  OR_HIP(hipMemset(m->th_buf, 0, sizeof(double) * (size_t)words));
+  if (n_hex) {
+    OR_HIP(hipMalloc((void **)&m->th_hex, sizeof(int32_t) * (size_t)n_hex));
+    OR_HIP(hipMemcpy(m->th_hex, hs.data(), sizeof(int32_t) * (size_t)n_hex, hipMemcpyHostToDevice));
+  }
+  if (n_q4) {
+    OR_HIP(hipMalloc((void **)&m->th_q4, sizeof(int32_t) * (size_t)n_q4));
+    OR_HIP(hipMemcpy(m->th_q4, qs.data(), sizeof(int32_t) * (size_t)n_q4, hipMemcpyHostToDevice));
+  }
+  // EHOUR accumulates from the first setup on (a later setup keeps it)
+  if (m->nt_h > 0 && !m->ehou_h) {
+    OR_HIP(hipMalloc((void **)&m->ehou_h, sizeof(double) * (size_t)m->nt_h));
+    OR_HIP(hipMemset(m->ehou_h, 0, sizeof(double) * (size_t)m->nt_h));
+    m->bytes += 8 * (int64_t)m->nt_h;
+  }
+  if (m->nt_q > 0 && !m->ehou_q) {
+    OR_HIP(hipMalloc((void **)&m->ehou_q, sizeof(double) * (size_t)m->nt_q));
+    OR_HIP(hipMemset(m->ehou_q, 0, sizeof(double) * (size_t)m->nt_q));
+    m->bytes += 8 * (int64_t)m->nt_q;
+  }
+  m->bytes += 4 * m->nn + 8 * words + 4 * (n_hex + n_q4);
+  m->th_nn = n_node; m->th_nh = n_hex; m->th_nq = n_q4;
+  mark_changed(m);
+  return OR_OK;
+}
+
+static int th_ehour(or_model *m, double *ehour) {
+  double *w = m->th_buf + 18 * m->th_nn + 13 * m->th_nh + (12 + m->npt) * m->th_nq;
+  hipLaunchKernelGGL(k_th_ehour, dim3(1), dim3(kBlock), 0, m->stream, m->ehou_h, (int64_t)(m->ehou_h ? m->nt_h : 0),
+                     m->ehou_q, (int64_t)(m->ehou_q ? m->nt_q : 0), w);
+  OR_HIP(hipGetLastError());
+  OR_HIP(hipMemcpyAsync(ehour, w, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  OR_HIP(quiesce(m));
+  return OR_OK;
+}
+
+int or_model_run_th(or_model *m, or_ecrit *ec, double glob[4], double *node_out, double *hex_out, double *q4_out) {
+  if (!m || !ec || !glob) return fail(OR_ERR_ARG, "null argument");
+  if (!m->th_buf) return fail(OR_ERR_STATE, "or_model_th_setup first");
+  if ((m->th_nn && !node_out) || (m->th_nh && !hex_out) || (m->th_nq && !q4_out))
+    return fail(OR_ERR_ARG, "an output array of the listed entities is null");
+  int rc = or_model_run_ecrit(m, ec);
+  if (rc) return rc;
+  OR_HIP(hipSetDevice(m->device));
+  double *hb = m->th_buf + 18 * m->th_nn, *qb = hb + 13 * m->th_nh;
+  const int64_t ne = m->th_nh + m->th_nq;
+  if (ne > 0) {
+    const dim3 g((unsigned)((ne + kBlock - 1) / kBlock));
+    if (m->prec == OR_PREC_FP64)
+      hipLaunchKernelGGL(k_th_elements<double>, g, dim3(kBlock), 0, m->stream, m->dev, m->th_hex, m->th_nh, m->th_q4,
+                         m->th_nq, m->npt, hb, qb);
+    else
+      hipLaunchKernelGGL(k_th_elements<float>, g, dim3(kBlock), 0, m->stream, m->dev, m->th_hex, m->th_nh, m->th_q4,
+                         m->th_nq, m->npt, hb, qb);
+    OR_HIP(hipGetLastError());
+  }
+  if (m->th_nn) OR_HIP(hipMemcpyAsync(node_out, m->th_buf, sizeof(double) * 18 * m->th_nn, hipMemcpyDeviceToHost,
+                                      m->stream));
+  if (m->th_nh) OR_HIP(hipMemcpyAsync(hex_out, hb, sizeof(double) * 13 * m->th_nh, hipMemcpyDeviceToHost, m->stream));
+  if (m->th_nq)
+    OR_HIP(hipMemcpyAsync(q4_out, qb, sizeof(double) * (12 + m->npt) * m->th_nq, hipMemcpyDeviceToHost, m->stream));
+  glob[0] = m->momentum[0]; glob[1] = m->momentum[1]; glob[2] = m->momentum[2];
+  return th_ehour(m, &glob[3]);
+}
+
+int or_model_ehour(or_model *m, double *ehour) {
+  if (!m || !ehour) return fail(OR_ERR_ARG, "null argument");
+  if (!m->th_buf) return fail(OR_ERR_STATE, "or_model_th_setup first");
+  OR_HIP(hipSetDevice(m->device));
+  return th_ehour(m, ehour);
+}
+
+int or_model_set_ehour(or_model *m, double ehour) {
+  if (!m) return fail(OR_ERR_ARG, "null model");
+  if (!m->th_buf) return fail(OR_ERR_STATE, "or_model_th_setup first");
+  OR_HIP(hipSetDevice(m->device));
+  OR_HIP(quiesce(m));
+  if (m->ehou_h) OR_HIP(hipMemset(m->ehou_h, 0, sizeof(double) * (size_t)m->nt_h));
+  if (m->ehou_q) OR_HIP(hipMemset(m->ehou_q, 0, sizeof(double) * (size_t)m->nt_q));
+  double *first = m->ehou_h ? m->ehou_h : m->ehou_q;
+  if (first) OR_HIP(hipMemcpy(first, &ehour, sizeof(double), hipMemcpyHostToDevice));
   return OR_OK;
 }
 
@@ -2184,15 +2435,16 @@ int or_model_download_q4(or_model *m, double *offg, double *thk, double *forc, d
 }
 
 // Fixed-order reduction of the internal energy and of the kinetic terms: 0.5*MS*|V|^2 from the
-// resident state, or the per-node ECRIT terms of the last output cycle (ecrit != null).
-static int energy_sums(or_model *m, const double *ecrit, double out[4]) {
+// resident state, or the per-node ECRIT terms of the last output cycle (ecrit != null, which
+// also gives the momentum sums out[4..6]).
+static int energy_sums(or_model *m, const double *ecrit, double out[kEnergySums]) {
   int64_t n = m->nn;
   if (m->neh > n) n = m->neh;
   if (m->neq > n) n = m->neq;
   const int nb = 1024;
   const int64_t per = (n + nb - 1) / nb;
   const int64_t per_block = ((per + kBlock - 1) / kBlock) * kBlock;
-  double *partial = m->energy + 4;
+  double *partial = m->energy + kEnergySums;
   OR_HIP(quiesce(m));
   if (m->prec == OR_PREC_FP64)
     hipLaunchKernelGGL((k_energy_partial<double>), dim3(nb), dim3(kBlock), 0, m->stream, (const double *)m->heint,
@@ -2207,14 +2459,14 @@ static int energy_sums(or_model *m, const double *ecrit, double out[4]) {
                      ecrit ? 1.0 : 0.5, m->energy);
   OR_HIP(hipGetLastError());
   OR_HIP(quiesce(m));
-  OR_HIP(hipMemcpy(out, m->energy, 4 * sizeof(double), hipMemcpyDeviceToHost));
+  OR_HIP(hipMemcpy(out, m->energy, kEnergySums * sizeof(double), hipMemcpyDeviceToHost));
   return OR_OK;
 }
 
 int or_model_energy(or_model *m, double out[3]) {
   if (!m || !out) return fail(OR_ERR_ARG, "null argument");
   OR_HIP(hipSetDevice(m->device));
-  double e[4];
+  double e[kEnergySums];
   const int rc = energy_sums(m, nullptr, e);
   if (rc) return rc;
   out[0] = e[0]; out[1] = e[1]; out[2] = e[2];
@@ -2224,7 +2476,7 @@ int or_model_energy(or_model *m, double out[3]) {
 int or_model_wfext(or_model *m, double *wfext) {
   if (!m || !wfext) return fail(OR_ERR_ARG, "null argument");
   OR_HIP(hipSetDevice(m->device));
-  double e[4];
+  double e[kEnergySums];
   const int rc = energy_sums(m, nullptr, e);
   if (rc) return rc;
   *wfext = e[3];
@@ -2253,7 +2505,7 @@ int or_model_footprint_estimate(int64_t nn, int64_t ne_hex, int64_t ne_q4, int32
          ss * 6 * (int64_t)npt * ne_q4;
     if (strain_rate_filter) b += ss * (int64_t)npt * ne_q4;              // LBUF EPSD (or_model_set_q4_params)
   }
-  b += (int64_t)sizeof(DevTime) * 2 + 8 * 2 + 4 + 8 * (4 + 4 * 1024) + (int64_t)sizeof(ModelDev);
+  b += (int64_t)sizeof(DevTime) * 2 + 8 * 2 + 4 + 8 * (kEnergySums * (1 + 1024)) + (int64_t)sizeof(ModelDev);
   b += (int64_t)sizeof(Tile) * (th + tq + nsl);
   // or_model_upload_skyline: rest rows (at most every row of the family) and the prefix lists
   // (at most one prefix per node and family)
@@ -2800,6 +3052,7 @@ int or_model_autotune(or_model *m, int32_t cycles_per_variant, int32_t chosen[2]
       LaunchSet L;
       L.md = m->dev;
       L.th = m->tiles_h; L.nth = m->nt_h; L.vh = m->var_h;
+      L.eh = m->ehou_h != nullptr; L.eq = m->ehou_q != nullptr;
       L.tq = m->tiles_q; L.ntq = m->nt_q; L.vq = m->var_q;
       L.tn = m->tiles_n; L.ntn = m->nt_n;
       float ms = 0.f;

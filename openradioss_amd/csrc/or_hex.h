@@ -89,6 +89,11 @@ template <typename R> struct HexOut {
   R ehou;
 };
 
+// Receives an element's EHOU as soon as it is known (the default ignores it).
+struct NoEhou {
+  template <typename T> __device__ __forceinline__ void operator()(T) const {}
+};
+
 // SHVIS3 hourglass force of node j (0..7) in the three directions, H(d, 1..4) per direction:
 // F = -G(j,1)*H1 - G(j,2)*H2 - G(j,3)*H3 -/+ H4 with G(j,m) = +-1 -/+ PXH(m, P(j)) as SHVIS3
 // forms them (P = 1,2,3,4,3,4,1,2; minus for nodes 1-4, plus for nodes 5-8).
@@ -303,9 +308,10 @@ __device__ __forceinline__ void hex_kinematics(R (&vx)[8], R (&vy)[8], R (&vz)[8
 
 // Material and forces: SRHO3, SROTA3, MMAIN / M2LAW, MSTRAIN_RATE, MQVISCB, SMALLB3,
 // SHVIS3 forces, SFINT3, SCUMU3P.  Updates the element state st and fills o.
-template <typename R, typename S, typename Write>
+template <typename R, typename S, typename Write, typename Ehou = NoEhou>
 __device__ __forceinline__ void hex_material_forces(const HexGeom<R> &g, const HexKin<R> &kn, HexState<S> &st,
-                                                    const HexConsts &c, const R DT1, HexOut<R> &o, Write &&write) {
+                                                    const HexConsts &c, const R DT1, HexOut<R> &o, Write &&write,
+                                                    Ehou ehou_sink = Ehou{}) {
   const R ZERO = R(0), ONE = R(1), TWO = R(2), THREE = R(3);
   const R HALF = R(1.0 / 2.0), THIRD = R(1.0 / 3.0), FOURTH = R(1.0 / 4.0);
   const R TWO_THIRD = R(2.0 / 3.0), HUNDRED = R(100.0);
@@ -514,6 +520,7 @@ __device__ __forceinline__ void hex_material_forces(const HexGeom<R> &g, const H
     o.ehou = DT1 * (H[2][0] * HG[2][0] + H[2][1] * HG[2][1] + H[2][2] * HG[2][2] + H[2][3] * HG[2][3] +
                     H[0][0] * HG[0][0] + H[0][1] * HG[0][1] + H[0][2] * HG[0][2] + H[0][3] * HG[0][3] +
                     H[1][0] * HG[1][0] + H[1][1] * HG[1][1] + H[1][2] * HG[1][2] + H[1][3] * HG[1][3]);
+    ehou_sink(o.ehou);
     R s1 = (sg0 - QNEW) * VOLN, s2 = (sg1 - QNEW) * VOLN, s3 = (sg2 - QNEW) * VOLN;
     R s4 = sg3 * VOLN, s5 = sg4 * VOLN, s6 = sg5 * VOLN;
     const R PXk[4] = {PX1, PX2, PX3, PX4};
@@ -576,17 +583,17 @@ __device__ __forceinline__ void hex_material_forces(const HexGeom<R> &g, const H
 
 // The whole chain for one element.  st.offg and the gathered nodes are read beforehand;
 // load_rest() reads the rest of st between the kinematics and the material phase.
-template <typename R, typename S, typename Write, typename LoadRest>
+template <typename R, typename S, typename Write, typename LoadRest, typename Ehou = NoEhou>
 __device__ __forceinline__ void hex_element(const double (&xd)[8], const double (&yd)[8], const double (&zd)[8],
                                             R (&vx)[8], R (&vy)[8], R (&vz)[8], HexState<S> &st,
                                             const HexConsts &c, const R DT1, HexOut<R> &o, Write &&write,
-                                            LoadRest &&load_rest) {
+                                            LoadRest &&load_rest, Ehou ehou_sink = Ehou{}) {
   HexGeom<R> g;
   hex_geometry<R>(xd, yd, zd, double(st.offg), g);
   HexKin<R> kn;
   hex_kinematics<R>(vx, vy, vz, g, DT1, kn);
   load_rest();
-  hex_material_forces<R, S>(g, kn, st, c, DT1, o, write);
+  hex_material_forces<R, S>(g, kn, st, c, DT1, o, write, ehou_sink);
 }
 
 } // namespace orgpu

@@ -47,6 +47,7 @@ template <typename R> struct Q4Out {
   R f[3][4], m[3][4];
   R sti;
   double dt;    // element time step, or +EP06 when excluded from the minimum
+  R ehou;       // CHVIS3's hourglass energy of the cycle (EHOU)
 };
 
 __device__ __forceinline__ double expR(double a) { return ormath::or_exp(a); }
@@ -216,11 +217,11 @@ __device__ __forceinline__ void q4_frame(const double (&xg)[4], const double (&y
 // velocity again after the through-thickness loop (the values of xg/yg/zg and vl), from
 // which the frame and the local velocities that CHVIS3 and CFINT3 use are formed again
 // rather than kept live across the loop.
-template <typename R, typename IPIO, typename NLOAD>
+template <typename R, typename IPIO, typename NLOAD, typename Ehou = NoEhou>
 __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (&yg)[4], const double (&zg)[4],
                                            R (&vl)[4][3], R (&vrl)[4][3], bool tri, Q4Elem<R> &st,
                                            const Q4Consts &c, const R DT1, IPIO &ip_io, NLOAD &&nload,
-                                           Q4Out<R> &o) {
+                                           Q4Out<R> &o, Ehou ehou_sink = Ehou{}) {
   const R ZERO = R(0), ONE = R(1), TWO = R(2), THREE = R(3), HALF = R(0.5), FOURTH = R(0.25);
   const R EM20 = R(1.0 / 1.0e20), EM01 = R(1.0 / 10.0);
   const R FOUR_OVER_5 = R(4.0 / 5.0);
@@ -501,14 +502,19 @@ __device__ __forceinline__ void q4_element(const double (&xg)[4], const double (
     H11 = HOUR1A * GAMA1; H12 = HOUR1A * GAMA2; H13 = HOUR1A * GAMA3;
     R HOUR2A = hourg[1] + HG2 * (H1L + H1Q * fabsR(HG2));
     H21 = HOUR2A * GAMA1; H22 = HOUR2A * GAMA2; H23 = HOUR2A * GAMA3;
+    R EH = HOUR1A * HG1 + HOUR2A * HG2; // EHOU (chvis3.F:299, 322, 336-338)
     HG1 = VZ[0] * GAMA1 + VZ[1] * GAMA2 + VZ[2] * GAMA3 + VZ[3] * GAMA4;
     hourg[2] = hourg[2] + HG1 * HH2;
     R HOUR3A = hourg[2] + HG1 * (H2L + H2Q * fabsR(HG1));
     H31 = HOUR3A * GAMA1; H32 = HOUR3A * GAMA2; H33 = HOUR3A * GAMA3;
+    EH = EH + HOUR3A * HG1;
     HG1 = HGRX;
     HG2 = HGRY;
     hourg[3] = HG1 * (H3L + H3Q * fabsR(HG1));
     hourg[4] = HG2 * (H3L + H3Q * fabsR(HG2));
+    EH = EH + hourg[3] * HG1 + hourg[4] * HG2;
+    o.ehou = DT1 * EH * off; // DT1C = DT1
+    ehou_sink(o.ehou);
     ip_io.hourg_store(lt.hourg);
     B1k = hourg[3] * off;
     B2k = hourg[4] * off;

@@ -152,6 +152,9 @@ class Deck:
     bcs: list                                 # [(code bits, group id)]
     inivel: list                              # [(vx, vy, vz, group id)]
     impvel: list                              # [(direction 0..2, value, group id)]
+    titles: dict = dataclasses.field(default_factory=dict)  # ('PART'|'MAT'|'PROP', id) -> title; 'RUN' -> run title
+    th: list = dataclasses.field(default_factory=list)      # [(kind, id, title, variables, entity ids, names)]
+    th_other: list = dataclasses.field(default_factory=list)  # other /TH/ keywords (not written on this path)
 
 
 def _law2(b: _Block) -> M.Law2:
@@ -249,7 +252,7 @@ def _ids(lines) -> list[int]:
 
 # Keywords that do not change the model on this path: titles, output requests, run controls, and
 # defaults that the properties read here must state explicitly anyway.
-_IGNORED = {"TITLE", "ANALY", "IOFLAG", "SPMD", "SUBSET", "SHFRA", "TH"}
+_IGNORED = {"TITLE", "ANALY", "IOFLAG", "SPMD", "SUBSET", "SHFRA"}
 # Global property defaults: a property field left 0 takes their value (hm_read_prop14.F:277-291,
 # hm_read_prop01.F:194-273), so they are read only when they leave every default as it is.
 _DEFAULTS = {"DEF_SHELL", "DEF_SOLID"}
@@ -267,12 +270,33 @@ def _all_default(lines) -> bool:
     return True
 
 
+def _th_group(b: _Block, kind: str, where: str):
+    """/TH/NODE, /TH/BRIC, /TH/SHEL (th_node.cfg, th_bric.cfg, th_shel.cfg): title; variable names
+    in 10-character fields, on lines up to the first entity line; then per entity its id (I10), a
+    skew frame id (I10, nodes and shells) and a name (from column 21)."""
+    b.need(1)
+    names, ents, labels = [], [], []
+    for line in _data(b.lines[1:]):
+        first = line[:10].strip()
+        if not ents and first and not first.lstrip("-").isdigit():
+            names += [line[i:i + 10].strip() for i in range(0, len(line), 10) if line[i:i + 10].strip()]
+            continue
+        if not first:
+            continue
+        if kind in ("NODE", "SHEL") and _int(line[10:20]) != 0:
+            raise DeckError(f"{where}: time-history output in a skew frame is not on this path")
+        ents.append(_int(first))
+        labels.append(line[20:].strip()[:100])
+    return (kind, b.ident(), b.lines[0].rstrip(), names, np.array(ents, np.int64), labels)
+
+
 def parse(text: str) -> Deck:
     """Read the keywords of a Starter deck given as text."""
     node_ids, xs = [], []
     bricks, shells = [], []
     parts, mats, sprops, cprops, grnod, funct = {}, {}, {}, {}, {}, {}
     bcs, inivel, impvel_raw = [], [], []
+    titles, th, th_other = {}, [], []
     for b in _blocks(text):
         p = [s.upper() for s in b.parts]
         head = p[0]
@@ -286,6 +310,15 @@ def parse(text: str) -> Deck:
         if head == "BEGIN":
             if len(b.lines) >= 4 and b.lines[2].split() != b.lines[3].split():
                 raise DeckError(f"{where}: input and work units differ; convert the deck first")
+            if b.lines:
+                titles["RUN"] = b.lines[0].rstrip()
+            continue
+        if head == "TH":
+            kind = p[1] if len(p) > 2 else "?"
+            if kind not in ("NODE", "BRIC", "SHEL"):
+                th_other.append(b.keyword)
+                continue
+            th.append(_th_group(b, kind, where))
             continue
         if head in ("UNIT", "RANDOM"):
             raise DeckError(f"{where}: keyword changes the model ({head.lower()}) and is not applied here")
@@ -311,13 +344,18 @@ def parse(text: str) -> Deck:
                            np.array(thk, np.float64)))
         elif head == "PART":
             b.need(2)
-            prop, mat, _ = (_int(s) for s in _cut(b.lines[1], (10, 10, 10)))
+            prop, mat, sub = (_int(s) for s in _cut(b.lines[1], (10, 10, 10)))
             parts[b.ident()] = (prop, mat)
+            titles[("PART", b.ident())] = b.lines[0].rstrip()
+            if sub:
+                titles[("PART_SUBSET", b.ident())] = sub
         elif head == "MAT":
             law = p[1] if len(p) > 2 else "?"
             mats[b.ident()] = _law2(b) if law in ("PLAS_JOHNS", "LAW2") else law
+            titles[("MAT", b.ident())] = b.lines[0].rstrip() if b.lines else ""
         elif head == "PROP":
             kind = p[1] if len(p) > 2 else "?"
+            titles[("PROP", b.ident())] = b.lines[0].rstrip() if b.lines else ""
             if kind in ("SOLID", "TYPE14"):
                 sprops[b.ident()] = _solid_prop(b)
             elif kind in ("SHELL", "TYPE1"):
@@ -386,7 +424,8 @@ def parse(text: str) -> Deck:
         raise DeckError("the deck has no /NODE block")
     return Deck(node_ids=np.array(node_ids, np.int64), x=np.array(xs, np.float64).reshape(-1, 3),
                 bricks=bricks, shells=shells, parts=parts, mats=mats, solid_props=sprops,
-                shell_props=cprops, grnod=grnod, bcs=bcs, inivel=inivel, impvel=impvel)
+                shell_props=cprops, grnod=grnod, bcs=bcs, inivel=inivel, impvel=impvel, titles=titles, th=th,
+                th_other=th_other)
 
 
 def _family(deck: Deck, blocks, props: dict, kind: str):
@@ -502,10 +541,12 @@ class EngineSettings:
     tstop: float | None = None            # /RUN end time
     dtfac_hex: float = 0.9                # DTFAC1(1), /DT/BRICK
     dtfac_q4: float = 0.9                 # DTFAC1(3), /DT/SHELL
+    dthis: float | None = None            # /TFILE time-history interval
+    th_vers: int = 40                     # /TH/VERS (freform.F:1543-1549)
 
 
 # Engine keywords that request output or name the run; they leave the cycle unchanged.
-_ENGINE_IGNORED = {"TITLE", "VERS", "ANIM", "TFILE", "PRINT", "RFILE", "MON", "H3D", "OUTP"}
+_ENGINE_IGNORED = {"TITLE", "VERS", "ANIM", "PRINT", "RFILE", "MON", "H3D", "OUTP"}
 
 
 def _free(line: str) -> list[str]:
@@ -526,6 +567,11 @@ def parse_engine(text: str) -> EngineSettings:
         if p[0] == "RUN":
             b.need(1)
             out.tstop = _real(_free(b.lines[0])[0])
+        elif p[0] == "TFILE":
+            b.need(1)
+            out.dthis = _real(_free(b.lines[0])[0])
+        elif p[0] == "TH" and len(p) > 1 and p[1] == "VERS":
+            out.th_vers = int(p[2]) if len(p) > 2 and p[2].isdigit() else 40
         elif p[0] == "STOP":
             # /STOP: Emax, Mmax, Nmax (energy / mass error limits that end the run) then output
             # counts (freform.F:770-786); a limit of 0 means none

@@ -190,6 +190,86 @@ class GpuModel:
         L.check(self._lib.or_model_wfext(self._h, C.byref(w)))
         return w.value + self.wfext_offset
 
+    # -- time history ------------------------------------------------------
+    def th_setup(self, nodes=(), hexes=(), q4s=()):
+        """Entities of the time history (0-based node, hex and shell indices); starts the EHOUR
+        accumulation (or_model_th_setup)."""
+        self._th = [np.ascontiguousarray(np.asarray(a, np.int64).astype(np.int32)) for a in (nodes, hexes, q4s)]
+        n, h, q = self._th
+        L.check(self._lib.or_model_th_setup(self._h, C.c_int64(len(n)), n.ctypes.data_as(C.c_void_p),
+                                            C.c_int64(len(h)), h.ctypes.data_as(C.c_void_p),
+                                            C.c_int64(len(q)), q.ctypes.data_as(C.c_void_p)))
+        if getattr(self, "_ehour_restart", None) is not None:  # EHOUR of a checkpoint
+            self.set_ehour(self._ehour_restart)
+            self._ehour_restart = None
+
+    def run_th(self) -> dict:
+        """One output cycle with the time-history values (or_model_run_th): the run_ecrit record plus
+        xmomt, ymomt, zmomt, ehour, the time and the new DT2 of the record, "nodes" [n][18] (D, V, A,
+        VR, AR, X), "hex" [n][13] and "q4" [n][12 + NPT] element values."""
+        if getattr(self, "_th", None) is None:
+            raise RuntimeError("th_setup first")
+        n, h, q = self._th
+        npt = self.model.q4.prop.npt if self.model.q4 is not None else 0
+        no, ho, qo = np.zeros((len(n), 18)), np.zeros((len(h), 13)), np.zeros((len(q), 12 + npt))
+        t0 = self.time_state()
+        e = L.OrEcrit()
+        g = np.zeros(4)
+        L.check(self._lib.or_model_run_th(self._h, C.byref(e), g.ctypes.data_as(C.c_void_p),
+                                          no.ctypes.data_as(C.c_void_p), ho.ctypes.data_as(C.c_void_p),
+                                          qo.ctypes.data_as(C.c_void_p)))
+        t1 = self.time_state()
+        return dict(enint=e.enint, encin=e.encin, enrot=e.enrot, wfext=e.wfext + self.wfext_offset,
+                    itypts=e.itypts, nelts=e.nelts, xmomt=g[0], ymomt=g[1], zmomt=g[2], ehour=g[3],
+                    tt=t0.tt, ncycle=t0.ncycle, dt2=t1.dt2, nodes=no, hex=ho, q4=qo)
+
+    def ehour(self) -> float:
+        """EHOUR so far (after th_setup)."""
+        v = C.c_double()
+        L.check(self._lib.or_model_ehour(self._h, C.byref(v)))
+        return v.value
+
+    def set_ehour(self, value: float):
+        L.check(self._lib.or_model_set_ehour(self._h, C.c_double(value)))
+
+    def run_time_history(self, header, dthis: float, tstop: float, path=None, this: float = 0.0,
+                         max_cycles: int | None = None):
+        """Run until TT reaches tstop (or max_cycles cycles), writing a time-history record at every
+        cycle whose TT has reached the next output time THIS (hist2.F:261-276: then THIS =
+        min(TSTOP, max(TT, THIS + DTHIS))).  header: an openradioss_amd.th.ThHeader whose groups
+        list this model's entities; returns the file's bytes (written to path when given) and the
+        next output time."""
+        from . import th as TH
+        self.th_setup(*TH.group_indices(header))
+        out = [TH.header_bytes(header)]
+        xmass = float(np.cumsum(np.asarray(self.model.ms, np.float64))[-1])  # node after node, as ECRIT adds it
+        done = 0
+        while True:
+            t = self.time_state()
+            if t.tt >= tstop or t.mstop or (max_cycles is not None and done >= max_cycles):
+                break
+            if t.tt >= this:
+                r = self.run_th()
+                out.append(TH.time_bytes(r["tt"], TH.global_values(r, xmass), TH.group_values(header, self.model, r)))
+                this = TH.next_time(this, dthis, r["tt"], tstop)
+                done += 1
+                continue
+            # cycles that cannot reach THIS: DT2 grows by at most 10 % a cycle (resol.F:6125-6129)
+            k, adv, dt = 0, 0.0, t.dt2
+            while True:
+                dt *= 1.1
+                if t.tt + adv + dt >= this or (max_cycles is not None and done + k + 1 >= max_cycles):
+                    break
+                adv += dt
+                k += 1
+            self.run(max(k, 1))
+            done += max(k, 1)
+        data = b"".join(out)
+        if path is not None:
+            with open(path, "wb") as fh:
+                fh.write(data)
+        return data, this
+
     def run_listing(self, ncycles: int, every: int, listing=None):
         """Run ncycles cycles printing the Engine listing's cycle line (ecrit.F FORMAT 1100) at every
         cycle whose NCYCLE is a multiple of `every`; returns the openradioss_amd.listing.Listing (a
@@ -227,6 +307,8 @@ class GpuModel:
         out = {"x": x, "v": v, "vr": vr, "d": d,
                "time": np.array([float(getattr(ts, f)) for f in self._TS_FIELDS]),
                "wfext": np.array([self.wfext()])}
+        if getattr(self, "_th", None) is not None:
+            out["ehour"] = np.array([self.ehour()])
         if self.model.hexp is not None:
             out.update({"hex_" + k: a for k, a in self.hex_state().items()})
         if self.model.q4 is not None:
@@ -246,6 +328,8 @@ class GpuModel:
         g.set_time_state(ts)
         if "wfext" in ck:
             g.wfext_offset = float(ck["wfext"][0])
+        if "ehour" in ck:
+            g._ehour_restart = float(ck["ehour"][0])
         return g
 
     def save(self, path: str):

@@ -18,6 +18,7 @@ module or_gpu_mod
   public :: or_model_upload_displacement
   public :: or_model_set_time, or_model_get_time, or_model_run, or_model_run_output, or_model_sync
   public :: or_ecrit_t, or_model_run_ecrit, or_model_wfext
+  public :: or_model_th_setup, or_model_run_th, or_model_ehour, or_model_set_ehour
   public :: or_model_download_nodes, or_model_download_hex, or_model_download_q4
   public :: or_model_energy, or_model_kernel_times, or_model_set_timing, or_model_footprint
   public :: or_model_footprint_estimate
@@ -181,6 +182,37 @@ module or_gpu_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: handle
       real(c_double), intent(out) :: w
+    end function
+
+    ! HIST2's values (hist2.F:299-330, 959-999): the listed nodes, hexes and shells (0-based)
+    integer(c_int) function or_model_th_setup(handle, n_node, node, n_hex, hex, n_q4, q4) &
+        bind(c, name="or_model_th_setup")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int64_t), value :: n_node, n_hex, n_q4
+      integer(c_int32_t), intent(in) :: node(*), hex(*), q4(*)
+    end function
+
+    ! one output cycle: ECRIT's record, XMOMT/YMOMT/ZMOMT/EHOUR, node (18, n), hex (13, n) and
+    ! shell (12 + NPT, n) values
+    integer(c_int) function or_model_run_th(handle, e, glob, node_out, hex_out, q4_out) &
+        bind(c, name="or_model_run_th")
+      import :: c_int, c_ptr, c_double, or_ecrit_t
+      type(c_ptr), value :: handle
+      type(or_ecrit_t), intent(out) :: e
+      real(c_double), intent(out) :: glob(4), node_out(*), hex_out(*), q4_out(*)
+    end function
+
+    integer(c_int) function or_model_ehour(handle, ehour) bind(c, name="or_model_ehour")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: ehour
+    end function
+
+    integer(c_int) function or_model_set_ehour(handle, ehour) bind(c, name="or_model_set_ehour")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      real(c_double), value :: ehour
     end function
 
     integer(c_int) function or_model_sync(handle) bind(c, name="or_model_sync")

@@ -33,6 +33,7 @@ EXPORTS = (
     "or_cforc3_group", "or_nodal_update", "or_batch_create", "or_batch_destroy", "or_batch_add", "or_batch_remove", "or_batch_run",
     "or_batch_sync", "or_batch_active", "or_model_set_variant", "or_model_autotune", "or_batch_autotune",
     "or_math_eval", "or_sforc3_group_static", "or_model_assembly_stats", "or_model_run_ecrit", "or_model_wfext",
+    "or_model_th_setup", "or_model_run_th", "or_model_ehour", "or_model_set_ehour",
 )
 
 
@@ -107,6 +108,10 @@ _SIGS = {
     "or_model_run_output": ([_P, _P], C.c_int),
     "or_model_run_ecrit": ([_P, _P], C.c_int),
     "or_model_wfext": ([_P, _P], C.c_int),
+    "or_model_th_setup": ([_P, C.c_int64, _P, C.c_int64, _P, C.c_int64, _P], C.c_int),
+    "or_model_run_th": ([_P] * 6, C.c_int),
+    "or_model_ehour": ([_P, _P], C.c_int),
+    "or_model_set_ehour": ([_P, C.c_double], C.c_int),
     "or_nodal_update": ([C.c_int, C.c_int64, C.c_int64] + [_P] * 12 + [C.c_double, C.c_double, _P], C.c_int),
     "or_device_memory": ([C.c_int, _P, _P], C.c_int),
     "or_batch_create": ([_P, C.c_int], C.c_int),

@@ -32,7 +32,8 @@ import time as _time
 import numpy as np
 
 __all__ = ["NODE_VARS", "BRIC_VARS", "SHEL_VARS", "GLOBAL_VARS", "ThGroup", "ThHeader", "ThError", "encode_reals",
-           "var_codes", "header_bytes", "time_bytes", "read", "next_time"]
+           "var_codes", "header_bytes", "time_bytes", "read", "next_time", "group_indices", "global_values",
+           "group_values", "header_from_deck"]
 
 
 class ThError(ValueError):
@@ -220,6 +221,108 @@ def time_bytes(tt: float, glob, group_values) -> bytes:
 def next_time(this: float, dthis: float, tt: float, tstop: float) -> float:
     """The next output time after writing at TT (hist2.F:264-276, explicit branch)."""
     return min(tstop, max(tt, this + dthis))
+
+
+def header_from_deck(deck, model, vers: int = 40, date: str | None = None) -> ThHeader:
+    """The time-history header of a model read from a deck (openradioss_amd.deck): its parts,
+    materials and properties in deck order (the Starter's internal numbers), and its /TH/NODE,
+    /TH/BRIC and /TH/SHEL groups with their entities as model indices."""
+    if deck.th_other:
+        raise ThError(f"{', '.join(deck.th_other)}: only /TH/NODE, /TH/BRIC and /TH/SHEL are written on this path")
+    mats = list(deck.mats)
+    props = list(dict.fromkeys(list(deck.solid_props) + list(deck.shell_props)))
+    parts = []
+    for pid, (prop, mat) in deck.parts.items():
+        if ("PART_SUBSET", pid) in deck.titles:
+            raise ThError(f"/PART/{pid}: parts in a /SUBSET are not written on this path")
+        if mat not in mats or prop not in props:
+            raise ThError(f"/PART/{pid}: material {mat} or property {prop} is not defined")
+        parts.append((pid, deck.titles.get(("PART", pid), ""), mats.index(mat) + 1, props.index(prop) + 1))
+
+    def lookup(kind, ids, user):
+        order = np.argsort(ids, kind="stable")
+        pos = np.minimum(np.searchsorted(ids[order], user), len(ids) - 1)
+        if len(ids) == 0 or np.any(ids[order][pos] != user):
+            raise ThError(f"/TH/{kind}: an entity id is not in the model")
+        return order[pos]
+
+    hex_ids = model.hexp.ids if model.hexp is not None and model.hexp.ids is not None else \
+        np.arange(1, model.ne_hex + 1)
+    q4_ids = model.q4.ids if model.q4 is not None and model.q4.ids is not None else np.arange(1, model.ne_q4 + 1)
+    space = {"NODE": np.asarray(deck.node_ids), "BRIC": np.asarray(hex_ids), "SHEL": np.asarray(q4_ids)}
+    groups = [ThGroup.make(kind, gid, title, names, lookup(kind, space[kind], ents), ents, labels)
+              for kind, gid, title, names, ents, labels in deck.th]
+    return ThHeader(deck.titles.get("RUN", ""), parts, [(m, deck.titles.get(("MAT", m), "")) for m in mats],
+                    [(p, deck.titles.get(("PROP", p), "")) for p in props], groups, vers=vers, date=date)
+
+
+# -- values of one output time ------------------------------------------------------------------
+def group_indices(h: ThHeader):
+    """The nodes, hexes and shells the groups list (each once, ascending): the rows of a sample."""
+    out = []
+    for kind in ("NODE", "BRIC", "SHEL"):
+        idx = [g.index for g in h.groups if g.kind == kind]
+        out.append(np.unique(np.concatenate(idx)) if idx else np.zeros(0, np.int64))
+    return out
+
+
+def global_values(r: dict, xmass: float) -> np.ndarray:
+    """The 22 global variables of one output time (hist2.F:306-327) from an output-cycle record r
+    (enint, encin, xmomt, ymomt, zmomt, dt2, enrot, wfext, ehour).  XMASS is the nodal mass summed
+    node after node; without mass scaling or added mass XMASS - MASS0 is 0, and there is no
+    contact, spring, plastic-work (G_WPLA = 0) or inlet term on this path."""
+    g = np.zeros(22)
+    g[0], g[1], g[2], g[3], g[4] = r["enint"], r["encin"], r["xmomt"], r["ymomt"], r["zmomt"]
+    g[5], g[6], g[7], g[8] = xmass, r["dt2"], r["enrot"], r["wfext"]
+    g[11] = r["ehour"]
+    return g
+
+
+def _bric_table(hv: np.ndarray, rho0: float) -> np.ndarray:
+    """THSOL's WWA(1..14) of law-2 solids from OFF, SIG(6), EINT, RHO, QVIS, VOL0, PLA, EPSD
+    (thsol.F:189-200, 511-548): VOL = VOL0*RHO0/RHO for a Lagrangian element with RHO > 0,
+    TEMP = 0 without the thermal solver."""
+    w = np.zeros((hv.shape[0], 14))
+    w[:, 0:7] = hv[:, 0:7]
+    w[:, 7], w[:, 8], w[:, 9] = hv[:, 7], hv[:, 8], hv[:, 9]
+    rho, vol = hv[:, 8], hv[:, 10]
+    w[:, 10] = np.where(rho > 0.0, vol * rho0 / np.where(rho > 0.0, rho, 1.0), vol)
+    w[:, 11] = hv[:, 11]
+    w[:, 13] = hv[:, 12]
+    return w
+
+
+def _shel_table(qv: np.ndarray, npt: int, ithk: int, thke: np.ndarray) -> np.ndarray:
+    """THCOQ's WWA(1..14) of law-2 Q4 shells (thcoq.F:296-340, 452-520): FOR, MOM, EINT, OFF,
+    THK (THKE without the thickness change), the smallest and largest |PLA| of the points."""
+    w = np.zeros((qv.shape[0], 14))
+    w[:, 0:11] = qv[:, 0:11]
+    w[:, 11] = qv[:, 11] if ithk > 0 else thke
+    pla = np.abs(qv[:, 12:12 + npt])
+    w[:, 12] = np.minimum(1.0e30, pla.min(axis=1)) if npt else 1.0e30
+    w[:, 13] = np.maximum(0.0, pla.max(axis=1)) if npt else 0.0
+    return w
+
+
+def group_values(h: ThHeader, model, r: dict) -> list:
+    """Each group's [entities][variables] values of one output time from an output-cycle record r
+    ("nodes" [n][18]: D, V, A, VR, AR, X; "hex" [n][13]; "q4" [n][12 + NPT], rows as group_indices)."""
+    nodes, hexes, shells = group_indices(h)
+    tabs = {}
+    if len(nodes):
+        tabs["NODE"] = (nodes, np.asarray(r["nodes"]))
+    if len(hexes):
+        tabs["BRIC"] = (hexes, _bric_table(np.asarray(r["hex"]), model.hexp.params(model.dtmx)["rho0"]))
+    if len(shells):
+        q = model.q4
+        tabs["SHEL"] = (shells, _shel_table(np.asarray(r["q4"]), q.prop.npt, int(q.params()["ithk"]),
+                                            np.asarray(q.thke, np.float64)[shells]))
+    out = []
+    for g in h.groups:
+        rows, table = tabs[g.kind]
+        sel = table[np.searchsorted(rows, g.index)]
+        out.append(sel[:, [c - 1 for c in g.codes]])
+    return out
 
 
 # -- reader -------------------------------------------------------------------------------------

@@ -180,6 +180,10 @@ typedef struct {
 void or_q4_forces_dt(int64_t e0, int64_t e1, const int32_t *conn, const double *x, const double *v,
                      const double *vr, or_q4_state *s, const int64_t *iadc, double *fsky,
                      const or_q4_params *p, double dt1, double *dt2t, double *dtel);
+/* ... and each element's hourglass energy of the cycle, CHVIS3's EHOU (ehou, may be NULL) */
+void or_q4_forces_dte(int64_t e0, int64_t e1, const int32_t *conn, const double *x, const double *v,
+                      const double *vr, or_q4_state *s, const int64_t *iadc, double *fsky,
+                      const or_q4_params *p, double dt1, double *dt2t, double *dtel, double *ehou);
 void or_q4_forces(int64_t e0, int64_t e1, const int32_t *conn /* [ne*4] */, const double *x, const double *v,
                   const double *vr, or_q4_state *s, const int64_t *iadc /* [ne*4] */, double *fsky,
                   const or_q4_params *p, double dt1, double *dt2t);
@@ -214,6 +218,15 @@ typedef struct {
   double *dtel_h, *dtel_q;   /* [ne] per-element time-step candidates of output cycles, or NULL */
   int32_t itypts;            /* controlling element of the last output cycle: 1 solid, 3 shell, 0 none */
   int64_t nelts;             /* its 0-based index in its family, -1 none */
+  /* time history (HIST2 at the same output point as ECRIT): with ecrit set, ecrit[4..6] receive
+   * ECRIT's XMOMT, YMOMT, ZMOMT (ecrit.F:221-228) and, if th is non-NULL, th[18*n ..] node n's
+   * D, V, A, VR, AR, X as THNOD reads them (thnod.F:59-97) */
+  double *th;
+  /* EHOUR (shvis3.F:425-464, chvis3.F:299-351): if ehou_h / ehou_q are non-NULL (scratch of
+   * ne_hex / ne_q4 doubles), every cycle adds to ehour the sum of each element group's EHOU, the
+   * groups in order, solids then shells */
+  double *ehou_h, *ehou_q;
+  double ehour;
 } or_model_c;
 
 void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads);

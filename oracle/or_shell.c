@@ -271,6 +271,12 @@ void or_q4_forces(int64_t e0, int64_t e1, const int32_t *conn, const double *X, 
 void or_q4_forces_dt(int64_t e0, int64_t e1, const int32_t *conn, const double *X, const double *V,
                      const double *VR, or_q4_state *s, const int64_t *IADC, double *FSKY,
                      const or_q4_params *p, double DT1, double *dt2t, double *dtel) {
+  or_q4_forces_dte(e0, e1, conn, X, V, VR, s, IADC, FSKY, p, DT1, dt2t, dtel, NULL);
+}
+
+void or_q4_forces_dte(int64_t e0, int64_t e1, const int32_t *conn, const double *X, const double *V,
+                      const double *VR, or_q4_state *s, const int64_t *IADC, double *FSKY,
+                      const or_q4_params *p, double DT1, double *dt2t, double *dtel, double *ehou) {
   const int64_t ne = s->ne;
   const int npt = p->npt;
   const double ZEP81 = c_zep81(), ZEP072169 = c_zep072169();
@@ -583,17 +589,21 @@ void or_q4_forces_dt(int64_t e0, int64_t e1, const int32_t *conn, const double *
       H11 = HOUR1A * GAMA1; H12 = HOUR1A * GAMA2; H13 = HOUR1A * GAMA3;
       double HOUR2A = hour2 + HG2 * (H1L + H1Q * fabs(HG2));
       H21 = HOUR2A * GAMA1; H22 = HOUR2A * GAMA2; H23 = HOUR2A * GAMA3;
+      double EH = HOUR1A * HG1 + HOUR2A * HG2; /* chvis3.F:299 */
       HG1 = VZ[0] * GAMA1 + VZ[1] * GAMA2 + VZ[2] * GAMA3 + VZ[3] * GAMA4;
       double hour3 = hour[2 * ne + e] + HG1 * HH2;
       hour[2 * ne + e] = hour3;
       double HOUR3A = hour3 + HG1 * (H2L + H2Q * fabs(HG1));
       H31 = HOUR3A * GAMA1; H32 = HOUR3A * GAMA2; H33 = HOUR3A * GAMA3;
+      EH = EH + HOUR3A * HG1; /* chvis3.F:322 */
       HG1 = RX[0] - RX[1] + RX[2] - RX[3];
       HG2 = RY[0] - RY[1] + RY[2] - RY[3];
       double hour4 = HG1 * (H3L + H3Q * fabs(HG1));
       double hour5 = HG2 * (H3L + H3Q * fabs(HG2));
       hour[3 * ne + e] = hour4;
       hour[4 * ne + e] = hour5;
+      EH = EH + hour4 * HG1 + hour5 * HG2; /* chvis3.F:336-338, DT1C = DT1 */
+      if (ehou) ehou[e] = DT1 * EH * off;
       B1k = hour4 * off;
       B2k = hour5 * off;
     }
@@ -687,6 +697,8 @@ void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads) {
   int mstop = 0;
   const int64_t G = 128;
   const int64_t ngh = (m->ne_hex + G - 1) / G, ngq = (m->ne_q4 + G - 1) / G;
+  const int track = m->ehou_h != NULL || m->ehou_q != NULL;
+  double *gsum = track ? (double *)calloc((size_t)(ngh + ngq > 0 ? ngh + ngq : 1), sizeof(double)) : NULL;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) reduction(min : DT2T) \
     reduction(| : mstop)
@@ -696,18 +708,34 @@ void or_cycle_model(or_model_c *m, or_time_state *ts, int nthreads) {
     if (g < ngh) {
       int64_t e0 = g * G, e1 = (e0 + G < m->ne_hex) ? e0 + G : m->ne_hex;
       int ms = 0;
+      if (m->ehou_h)
+        for (int64_t e = e0; e < e1; ++e) m->ehou_h[e] = ZERO;
       or_hex_forces_dt(e0, e1, m->ne_hex, m->hconn, m->x, m->v, m->hsig, m->heint, m->hrho, m->hvol0, m->hpla,
-                       m->hepsd, m->hoffg, m->hqvis, NULL, m->hiads, m->fsky, m->hp, DT1, &dtl, &ms,
+                       m->hepsd, m->hoffg, m->hqvis, m->ehou_h, m->hiads, m->fsky, m->hp, DT1, &dtl, &ms,
                        m->ecrit ? m->dtel_h : NULL);
       mstop |= ms;
+      if (m->ehou_h) { /* EHOURT = sum of the group's EHOU (shvis3.F:425-428) */
+        double t = ZERO;
+        for (int64_t e = e0; e < e1; ++e) t = t + m->ehou_h[e];
+        gsum[g] = t;
+      }
     } else {
       int64_t e0 = (g - ngh) * G, e1 = (e0 + G < m->ne_q4) ? e0 + G : m->ne_q4;
-      or_q4_forces_dt(e0, e1, m->qconn, m->x, m->v, m->vr, &m->q, m->qiadc, m->fsky, m->qp, DT1, &dtl,
-                      m->ecrit ? m->dtel_q : NULL);
+      or_q4_forces_dte(e0, e1, m->qconn, m->x, m->v, m->vr, &m->q, m->qiadc, m->fsky, m->qp, DT1, &dtl,
+                       m->ecrit ? m->dtel_q : NULL, m->ehou_q);
+      if (m->ehou_q) { /* chvis3.F:349-351 */
+        double t = ZERO;
+        for (int64_t e = e0; e < e1; ++e) t = t + m->ehou_q[e];
+        gsum[g] = t;
+      }
     }
     if (dtl < DT2T) DT2T = dtl;
   }
   (void)nthreads;
+  if (gsum) { /* EHOUR = EHOUR + EHOURT, group after group (shvis3.F:464, chvis3.F:397) */
+    for (int64_t g = 0; g < ngh + ngq; ++g) m->ehour = m->ehour + gsum[g];
+    free(gsum);
+  }
   if (m->ecrit && (m->dtel_h || m->dtel_q)) {
     /* NELTS / ITYPTS of the cycle, element after element in group order with DT2T = EP06 on
        entry (resol.F:2826): solids replace it when DTX <= DT2T (mqviscb.F:622-630), shells when
@@ -755,7 +783,7 @@ void or_nodes_model(or_model_c *m, or_time_state *ts, double dt2t, int mstop) {
   /* ECRIT output point (sortie_main.F:368 from resol.F:8523, after ACCELE/BCS10/FIXVEL and
    * before VELOCITY): DT05 = HALF*DT1 (ecrit.F:193) */
   const double DT05 = HALF * ts->dt1;
-  double encin = ZERO, enrot = ZERO;
+  double encin = ZERO, enrot = ZERO, xmom = ZERO, ymom = ZERO, zmom = ZERO;
   for (int64_t n = 0; n < nn; ++n) {
     double *a = m->a + 3 * n, *v = m->v + 3 * n;
     /* ACCELE (accele.F:65-80, 97-109) */
@@ -805,9 +833,20 @@ void or_nodes_model(or_model_c *m, or_time_state *ts, double dt2t, int mstop) {
       /* ecrit.F:221-225 (MAS = MS*WEIGHT_MD, WEIGHT_MD = 1) and 319-324 */
       const double vx = v[0] + DT05 * a[0], vy = v[1] + DT05 * a[1], vz = v[2] + DT05 * a[2];
       encin = encin + (vx * vx + vy * vy + vz * vz) * HALF * m->ms[n];
+      xmom = xmom + vx * m->ms[n];
+      ymom = ymom + vy * m->ms[n];
+      zmom = zmom + vz * m->ms[n];
       if (m->in) {
         const double wx = vr[0] + DT05 * ar[0], wy = vr[1] + DT05 * ar[1], wz = vr[2] + DT05 * ar[2];
         enrot = enrot + (wx * wx + wy * wy + wz * wz) * HALF * m->in[n];
+      }
+      if (m->th) { /* THNOD's D, V, A, VR, AR, X at HIST2's point (thnod.F:59-97) */
+        double *t = m->th + 18 * n;
+        const double *xx = m->x + 3 * n, *dd = m->d + 3 * n;
+        for (int c = 0; c < 3; ++c) {
+          t[c] = dd[c]; t[3 + c] = v[c]; t[6 + c] = a[c];
+          t[9 + c] = m->in ? vr[c] : ZERO; t[12 + c] = m->in ? ar[c] : ZERO; t[15 + c] = xx[c];
+        }
       }
     }
     /* VELOCITY (velocity.F:58-86) */
@@ -836,6 +875,9 @@ void or_nodes_model(or_model_c *m, or_time_state *ts, double dt2t, int mstop) {
     m->ecrit[1] = encin;
     m->ecrit[2] = enrot;
     m->ecrit[3] = m->wfext;
+    m->ecrit[4] = xmom;
+    m->ecrit[5] = ymom;
+    m->ecrit[6] = zmom;
   }
   ts->tt = ts->tt + ts->dt2;
   ts->ncycle += 1;

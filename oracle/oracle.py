@@ -188,7 +188,8 @@ class ModelC(C.Structure):
                [(n, _D) for n in ("hsig", "heint", "hrho", "hpla", "hepsd", "hoffg", "hqvis", "hvol0")] + \
                [("hp", C.POINTER(HexParams)), ("ne_q4", C.c_int64), ("qconn", _I32), ("qiadc", _I64),
                 ("q", Q4State), ("qp", C.POINTER(Q4Params)), ("ecrit", _D), ("wfext", C.c_double), ("wpend", _D),
-                ("dtel_h", _D), ("dtel_q", _D), ("itypts", C.c_int32), ("nelts", C.c_int64)]
+                ("dtel_h", _D), ("dtel_q", _D), ("itypts", C.c_int32), ("nelts", C.c_int64), ("th", _D),
+                ("ehou_h", _D), ("ehou_q", _D), ("ehour", C.c_double)]
 
 
 def q4_params(part) -> Q4Params:
@@ -291,7 +292,7 @@ class ModelOracle:
     def cycle_ecrit(self) -> dict:
         """One output cycle: ECRIT's energies, WFEXT and the controlling element (ITYPTS 1 solid, 3
         shell, 0 none; NELTS its 0-based index in its family, -1 none)."""
-        out = np.zeros(4)
+        out = np.zeros(8)
         self._dtel_h = np.zeros(max(self.m.ne_hex, 1))
         self._dtel_q = np.zeros(max(self.m.ne_q4, 1))
         self.c.ecrit = _pd(out)
@@ -304,7 +305,31 @@ class ModelOracle:
             self.c.dtel_h = None
             self.c.dtel_q = None
         return dict(energies=out[:3].copy(), enint=out[0], encin=out[1], enrot=out[2], wfext=out[3],
-                    itypts=int(self.c.itypts), nelts=int(self.c.nelts))
+                    xmomt=out[4], ymomt=out[5], zmomt=out[6], itypts=int(self.c.itypts), nelts=int(self.c.nelts))
+
+    def track_ehour(self):
+        """Accumulate EHOUR (the hourglass energy of every cycle, group sums in group order) from now on."""
+        self._ehou_h = np.zeros(max(self.m.ne_hex, 1))
+        self._ehou_q = np.zeros(max(self.m.ne_q4, 1))
+        self.c.ehou_h = _pd(self._ehou_h) if self.m.ne_hex else None
+        self.c.ehou_q = _pd(self._ehou_q) if self.m.ne_q4 else None
+
+    @property
+    def ehour(self) -> float:
+        return float(self.c.ehour)
+
+    def cycle_th(self) -> dict:
+        """One output cycle (cycle_ecrit) that also returns every node's D, V, A, VR, AR, X at the
+        output point ("nodes", [nn][18]) and the time and DT2 the time-history record carries."""
+        th = np.zeros((self.m.nn, 18))
+        tt = self.ts.tt
+        self.c.th = _pd(th)
+        try:
+            r = self.cycle_ecrit()
+        finally:
+            self.c.th = None
+        r.update(nodes=th, tt=tt, dt2=self.ts.dt2, ehour=self.ehour)
+        return r
 
     @property
     def wfext(self) -> float:

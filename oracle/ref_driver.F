@@ -264,7 +264,9 @@ C-----MMAIN law-2 glue (mmain.F90:668-745)
      .   1, DPDM, FHEAT, GTH,
      .   1, IGEO)
       CALL SMALLB3(OFFG, OFF, NEL, 4)
-      EHOUR0 = EHOUR
+C     EHOUR starts the group from zero, so that it ends as the group's EHOURT
+      EHOUR0 = ZERO
+      EHOUR = ZERO
       PARTSAV = ZERO
       EANI = ZERO
       CALL SHVIS3(PM, RHO, OFF,

@@ -13,7 +13,7 @@ C  2898-2912, 3013-3062): layer positions/weights from COQINI_WM and the Z0/WF
 C  tables, resultant sums, rupture test, thickness clamp and energy update.
       SUBROUTINE REF_Q4_FORCES(NEL, NNOD, X, V, VR, CONN, OFFG, THK,
      .   THKE, FOR, MOM, GSTR, HOUR, EINT, EPSDG, SIGL, PLAL, EPSDL,
-     .   NSKY, FSKY, IADC, PRM, IPRM, DTIN1, DT2T)
+     .   NSKY, FSKY, IADC, PRM, IPRM, DTIN1, DT2T, EHOU_OUT)
      .   BIND(C, NAME='ref_q4_forces')
       USE ISO_C_BINDING
       USE MATPARAM_DEF_MOD
@@ -40,7 +40,7 @@ C  tables, resultant sums, rupture test, thickness clamp and energy update.
       REAL(C_DOUBLE) :: OFFG(NEL), THK(NEL), THKE(NEL), FOR(NEL,5),
      .   MOM(NEL,3), GSTR(NEL,8), HOUR(NEL,5), EINT(NEL,2), EPSDG(NEL),
      .   SIGL(NEL,5,*), PLAL(NEL,*), EPSDL(NEL,*), FSKY(8,NSKY),
-     .   PRM(24), DT2T
+     .   PRM(24), DT2T, EHOU_OUT
       REAL(C_DOUBLE), VALUE :: DTIN1
 C
       TYPE(MATPARAM_STRUCT_) :: MATP
@@ -341,6 +341,8 @@ C-----MULAWC law-2 glue
         EINT(I,2) = EINT(I,2) + DEGFX(I)*THK0(I)*VOL2
         VISCMX(I) = SQRT(ONE + VISCMX(I)*VISCMX(I)) - VISCMX(I)
       ENDDO
+C     EHOUR starts the group from zero, so that it ends as the group's EHOURT
+      EHOUR = ZERO
       CALL CHVIS3(1, NEL, THK, HOUR, OFF, PX1, PX2, PY1, PY2,
      .   IXC, DT1C, SSP, RHO, STI, VX1, VX2, VX3, VX4, VY1,
      .   VY2, VY3, VY4, VZ1, VZ2, VZ3, VZ4, AREA, THK0, VHX,
@@ -351,6 +353,7 @@ C-----MULAWC law-2 glue
      .   RX1, RX2, RX3, RX4, RY1, RY2, RY3, RY4,
      .   IPARTC, PARTSAV, IHB, 0, ISMS, 0,
      .   SRH1, SRH2, SRH3, 1, 0, A11R, NEL)
+      EHOU_OUT = EHOUR
       IOFC = 0
       CALL CDT3(1, NEL, YM, OFF, DT2T, NELTST, ITYPTST, STI, STIR, OFFG,
      .   DTCSUB, NDT, DT2C, IXC, SSP, VISCMX, PX1, PX2, PY1, PY2,

@@ -183,6 +183,7 @@ class RefModel:
         self.dt2old = 1.0e6 / (1.0 + 1.0 / 10.0)
         self.ncycle = 0
         self.mstop = 0
+        self.ehour = 0.0  # EHOUR: every group's EHOURT, group after group (solids, then shells)
 
     def _hex_forces(self, dt1, dt2t, ne=None):
         """SFORC3 over the groups of the first ne hexes (all by default)."""
@@ -203,12 +204,14 @@ class RefModel:
             for dst, src in zip(arrs, grp):
                 dst[e0:e1] = src
             self.mstop |= ms.value
+            self.ehour = self.ehour + eh.value  # EHOUR = EHOUR + EHOURT (shvis3.F:464)
 
     def _q4_forces(self, dt1, dt2t, ne=None):
         """CFORC3 over the groups of the first ne shells (all by default)."""
         L = lib()
         ne = self.m.ne_q4 if ne is None else ne
         q = self.q
+        eh = C.c_double(0.0)
         for e0 in range(0, ne, self.GROUP):
             e1 = min(ne, e0 + self.GROUP)
             g = {}
@@ -222,9 +225,10 @@ class RefModel:
                             _p(g["gstr"]), _p(g["hourg"]), _p(g["eint"]), _p(g["epsd"]), _p(g["sig"]),
                             _p(g["pla"]), _p(g["epsd_ip"]), C.c_int(self.m.lsky), _p(self.fsky),
                             _p(np.ascontiguousarray(self.qiadc1[e0:e1])), _p(self.qprm), _p(self.qiprm),
-                            C.c_double(dt1), C.byref(dt2t))
+                            C.c_double(dt1), C.byref(dt2t), C.byref(eh))
             for k in q:
                 q[k][..., e0:e1] = g[k]
+            self.ehour = self.ehour + eh.value  # chvis3.F:397
 
     def cycle(self, n: int = 1):
         m = self.m

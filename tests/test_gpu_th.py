@@ -1,0 +1,94 @@
+"""The time history on the GPU (GpuModel.run_time_history through or_model_th_setup /
+or_model_run_th) against the oracle's, written by the same writer:
+
+* the same output times and the header byte for byte;
+* every group record (node D, V, A, VR, AR, X; solid and shell element values) byte for byte: the
+  FP64 state is equal bit for bit under correctly rounded pow/log/exp, so its REAL*4 is too;
+* the global variables within 2 float32 ulps: the GPU sums the energies, momenta, external work
+  and EHOUR in a fixed tree order (EHOUR per tile over the cycles), the oracle node after node and
+  group after group, so their FP64 values differ in the last bits;
+* EHOUR carried through a checkpoint.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import th_oracle as TO
+from openradioss_amd import engine as E
+from openradioss_amd import model as M
+from openradioss_amd import th as TH
+
+pytestmark = pytest.mark.gpu
+
+
+def _groups(m):
+    g = [TH.ThGroup.make("NODE", 1, "nodes", ["DEF", "A", "XYZ", "VR", "AR"],
+                         [0, m.nn // 3, m.nn // 2, m.nn - 1], [1, m.nn // 3 + 1, m.nn // 2 + 1, m.nn])]
+    if m.ne_hex:
+        g.append(TH.ThGroup.make("BRIC", 2, "bricks", ["DEF", "BULK", "VOL", "PLSR"],
+                                 [0, m.ne_hex // 2, m.ne_hex - 1], [1, m.ne_hex // 2 + 1, m.ne_hex]))
+    if m.ne_q4:
+        g.append(TH.ThGroup.make("SHEL", 3, "shells", ["DEF", "STRESS", "THIC"],
+                                 [0, m.ne_q4 // 2, m.ne_q4 - 1], [1, m.ne_q4 // 2 + 1, m.ne_q4]))
+    return TH.ThHeader("case", [(1, "part", 1, 1)], [(1, "law2")], [(1, "prop")], g,
+                       date="Thu Dec  3 12:00:00 2026")
+
+
+CASES = {
+    "mixed": (lambda: M.block_with_plate(n=8, plate=16, seed=11), 60, 4),
+    "tube": (lambda: M.square_tube(width=12.0, length=24.0, h=1.0), 80, 5),
+    "bar": (lambda: M.hex_model_to_model(M.cantilever_bar(6, 5, 12)), 60, 3),
+}
+
+
+def _records(buf):
+    out, p = [], 0
+    while p < len(buf):
+        n = int.from_bytes(buf[p:p + 4], "big")
+        out.append(buf[p:p + 8 + n])
+        p += 8 + n
+    return out
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_time_history_equals_oracle(name):
+    make, ncyc, every = CASES[name]
+    m = make()
+    h = _groups(m)
+    probe = O.ModelOracle(m)
+    probe.cycle(1)
+    dthis = every * probe.ts.dt2   # an output every few cycles
+    ref, recs, this_o = TO.oracle_time_history(m, h, dthis, 1.0, ncyc)
+    with E.GpuModel(m) as g:
+        data, this_g = g.run_time_history(h, dthis, 1.0, max_cycles=ncyc)
+    assert this_g == this_o
+    a, b = TH.read(data), TH.read(ref)
+    assert len(a["times"]) == len(b["times"]) == len(recs) >= ncyc // every
+    ra, rb = _records(data), _records(ref)
+    k = a["header_records"]
+    assert ra[:k] == rb[:k]
+    for ta, tb in zip(a["times"], b["times"]):
+        assert ta["time"] == tb["time"]
+        for ga, gb in zip(ta["groups"], tb["groups"]):
+            assert ga.tobytes() == gb.tobytes()
+        x, y = ta["glob"].astype(np.float64), tb["glob"].astype(np.float64)
+        assert x[5] == y[5] and x[6] == y[6]                  # XMASS, DT2
+        scale = np.maximum(np.abs(y), 1e-30)
+        scale[2:5] = np.abs(y[2:5]).max() + 1e-30          # momenta: relative to the largest
+        assert np.all(np.abs(x - y) <= 2.5e-7 * scale), (name, x, y)
+    assert b["times"][-1]["glob"][11] > 0                      # EHOUR
+
+
+def test_ehour_through_a_checkpoint():
+    m = M.block_with_plate(n=8, plate=16, seed=11)
+    with E.GpuModel(m) as g:
+        g.th_setup([0], [0], [0])
+        g.run(30)
+        ck = g.checkpoint()
+        g.run(20)
+        whole = g.ehour()
+    with E.GpuModel.restore(m, ck) as r:
+        r.th_setup([0], [0], [0])
+        assert r.ehour() == float(ck["ehour"][0])
+        r.run(20)
+        assert abs(r.ehour() - whole) <= 1e-13 * whole
