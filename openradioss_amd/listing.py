@@ -161,5 +161,14 @@ class Listing:
         self.lines.append(line)
         return line
 
+    def terminate(self, ncycle: int, mstop: int = 0) -> None:
+        """The end of the listing (arret.F:762-768): NORMAL TERMINATION and the cycle count, or
+        ERROR TERMINATION for a run stopped by MSTOP (a non-positive Jacobian or time step)."""
+        if mstop:
+            self.lines += ["", "", "", "     ERROR TERMINATION"]
+        else:
+            self.lines += ["", "", "", "     NORMAL TERMINATION      ", "", "", "",
+                           "     TOTAL NUMBER OF CYCLES  :" + fmt_i(ncycle, 8), ""]
+
     def text(self) -> str:
         return "\n".join(self.lines) + "\n"
