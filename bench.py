@@ -336,6 +336,8 @@ def main():
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision form")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched tube sweep")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the small workloads (bar, tube) and their parity block against the reference")
     ap.add_argument("--sweep-full", action="store_true",
                     help="only run the whole 256-tube sweep to 5 ms (FP64 and split) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -439,12 +441,13 @@ def main():
             "kernel_variants": {"hex": rs["variants"][0], "q4": rs["variants"][1]},
             "dtype": "f32 element / f64 node"}
         log(f"split: {eus:.3e} element-updates/s (x{eus / eu64:.2f}), disp err {derr:.2e}, energy err {eerr:.2e}")
-    out["small_models"], out["parity"] = small_leg()
-    for k, r in out["small_models"].items():
-        log(f"{k}: {r['element_updates_per_s']:.3e} element-updates/s, {r['us_per_cycle']:.1f} us per cycle")
-    for k, r in out["parity"].items():
-        log(f"parity {k} at cycle {r['cycles']}: displacement {r['d_rel_err']:.2e}, "
-            f"energy {r.get('energy_rel_err', float('nan')):.2e} against the reference")
+    if not args.no_small:
+        out["small_models"], out["parity"] = small_leg()
+        for k, r in out["small_models"].items():
+            log(f"{k}: {r['element_updates_per_s']:.3e} element-updates/s, {r['us_per_cycle']:.1f} us per cycle")
+        for k, r in out["parity"].items():
+            log(f"parity {k} at cycle {r['cycles']}: displacement {r['d_rel_err']:.2e}, "
+                f"energy {r.get('energy_rel_err', float('nan')):.2e} against the reference")
     if not args.no_sweep:
         out["batch_sweep"] = {}
         for prec in ("fp64", "split"):
