@@ -68,8 +68,9 @@ def gpu_run(model, precision: str, steps: int, warmup: int):
 
 def kernel_name(family: str, real: str, variant: int) -> str:
     """Name of an element-kernel launch variant as the profiles list it (one 256-element tile per
-    workgroup; register budget for 2 waves per SIMD for even variants, 3 for odd ones)."""
-    return f"k_{family}_forces<{real}, {real}, 256, {2 + (variant & 1)}>"
+    workgroup; register budget for 2 waves per SIMD for even variants, 3 for odd ones; the
+    instantiation without the time history's EHOUR, the one the bench cycles run)."""
+    return f"k_{family}_forces<{real}, {real}, 256, {2 + (variant & 1)}, false>"
 
 
 def measured_traffic(kernel: str):

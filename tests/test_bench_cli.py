@@ -20,8 +20,8 @@ def test_more_than_one_gpu_exits_with_a_message():
 
 
 def test_kernel_names_follow_the_launch_variants():
-    assert bench.kernel_name("hex", "double", 0) == "k_hex_forces<double, double, 256, 2>"
-    assert bench.kernel_name("q4", "float", 3) == "k_q4_forces<float, float, 256, 3>"
+    assert bench.kernel_name("hex", "double", 0) == "k_hex_forces<double, double, 256, 2, false>"
+    assert bench.kernel_name("q4", "float", 3) == "k_q4_forces<float, float, 256, 3, false>"
 
 
 def test_traffic_of_the_chosen_hex_variant_is_in_the_committed_profile():
