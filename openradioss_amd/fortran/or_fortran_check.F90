@@ -19,7 +19,7 @@
 !   hexgroup the same for one SFORC3 group (OR_GPU_ELIGIBLE_SOLID, OR_PACK_HEX_PARAMS,
 !            OR_SFORC3_GROUP);
 !   nodal    RESOL's nodal half of a cycle through OR_NODAL_UPDATE;
-!   model    a resident model: create, upload, OR_MODEL_RUN, download.
+!   model    a resident model: create, upload, OR_MODEL_TH_SETUP, OR_MODEL_RUN, OR_MODEL_RUN_TH, download.
 program or_fortran_check
   use, intrinsic :: iso_c_binding
   use or_gpu_mod
@@ -184,7 +184,8 @@ contains
   !        X, V, VR (3,NN); MS, IN (NN); node codes (NN) int32; VIMP (3,NN); hex connectivity
   !        (8,NEH) int32 0-based, VOL0 and RHO (NEH); shell connectivity (4,NEQ) int32 0-based,
   !        THKE (NEQ); ADSKY (NN+1), IADS (8,NEH), IADC (4,NEQ) int64 0-based.
-  ! <out>: int32 status; X, V, VR, D (3,NN); or_time_state_t.
+  ! <out>: int32 status; X, V, VR, D (3,NN); or_time_state_t; or_ecrit_t; the time-history record of
+  !        node 1, hex 1 and shell 1: XMOMT, YMOMT, ZMOMT, EHOUR, 18 node, 13 hex and 12+NPT shell values.
   subroutine run_model()
     integer(c_int64_t) :: nn, neh, neq
     integer(c_int32_t) :: npt, prec, ncyc
@@ -196,8 +197,15 @@ contains
     integer(c_int64_t), allocatable, target :: adsky(:), iadsh(:, :), iadcq(:, :)
     type(or_time_state_t) :: ts
     type(or_ecrit_t) :: ec
+    ! time history of the first node, hex and shell (HIST2's values of the output cycle)
+    integer(c_int32_t), target :: th_first(1) = [0_c_int32_t]
+    real(c_double) :: glob(4), nodeo(18), hexo(13)
+    real(c_double), allocatable :: q4o(:)
     open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', action='read')
     read (u) nn, neh, neq, npt, prec, ncyc, hp, qp
+    allocate (q4o(12 + max(npt, 0_c_int32_t)))
+    hexo = 0.0_c_double
+    q4o = 0.0_c_double
     allocate (xx(3, nn), vv(3, nn), vr(3, nn), d(3, nn), ms(nn), in(nn), code(nn), vimp(3, nn))
     allocate (connh(8, max(neh, 1_c_int64_t)), vol0(max(neh, 1_c_int64_t)), hrho(max(neh, 1_c_int64_t)))
     allocate (connq(4, max(neq, 1_c_int64_t)), thke(max(neq, 1_c_int64_t)))
@@ -222,9 +230,11 @@ contains
                                                                c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
                                                                c_null_ptr, c_null_ptr)
     if (ierr == OR_OK) ierr = or_model_upload_skyline(h, c_loc(adsky), c_loc(iadsh), c_loc(iadcq))
-    ! ncyc cycles, the last one an output cycle (ECRIT's line)
+    if (ierr == OR_OK) ierr = or_model_th_setup(h, 1_c_int64_t, th_first, min(neh, 1_c_int64_t), th_first, &
+                                                min(neq, 1_c_int64_t), th_first)
+    ! ncyc cycles, the last one an output cycle (ECRIT's line and the time-history record)
     if (ierr == OR_OK) ierr = or_model_run(h, int(ncyc - 1, c_int64_t))
-    if (ierr == OR_OK) ierr = or_model_run_ecrit(h, ec)
+    if (ierr == OR_OK) ierr = or_model_run_th(h, ec, glob, nodeo, hexo, q4o)
     if (ierr == OR_OK) ierr = or_model_sync(h)
     if (ierr == OR_OK) ierr = or_model_download_nodes(h, c_loc(xx), c_loc(vv), c_loc(vr), c_loc(d))
     if (ierr == OR_OK) ierr = or_model_get_time(h, ts)
@@ -233,7 +243,7 @@ contains
       if (or_model_destroy(h) /= OR_OK) write (*, '(a)') 'or_model_destroy failed'
     end if
     open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
-    write (u) int(ierr, c_int32_t), xx, vv, vr, d, ts, ec
+    write (u) int(ierr, c_int32_t), xx, vv, vr, d, ts, ec, glob, nodeo, hexo, q4o
     close (u)
   end subroutine run_model
 end program or_fortran_check

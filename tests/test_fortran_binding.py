@@ -320,8 +320,8 @@ def test_fortran_nodal_update_equals_python_call(tmp_path, gpu_lib):
 @needs_exe
 @pytest.mark.gpu
 def test_fortran_resident_model_equals_python_model(tmp_path, gpu_lib):
-    print("case: Fortran resident model (create, upload, 59 cycles and an output cycle, download), mixed block + "
-          "plate", flush=True)
+    print("case: Fortran resident model (create, upload, time-history setup, 59 cycles and an output cycle, "
+          "download), mixed block + plate", flush=True)
     from golden_cases import Q4_CASES, build_q4_case
     m = build_q4_case(Q4_CASES["mixed_c120"])
     hp, qp = E.hex_params(m), E.q4_params(m)
@@ -350,13 +350,23 @@ def test_fortran_resident_model_equals_python_model(tmp_path, gpu_lib):
     n3 = 24 * m.nn
     xf, vf, vrf, df = (np.frombuffer(b[4 + k * n3:4 + (k + 1) * n3], np.float64).reshape(m.nn, 3) for k in range(4))
     tsf = time_state_of(b[4 + 4 * n3:4 + 4 * n3 + 52])
-    ecf = L.OrEcrit.from_buffer_copy(b[4 + 4 * n3 + 52:4 + 4 * n3 + 52 + C.sizeof(L.OrEcrit)])
+    o = 4 + 4 * n3 + 52
+    ecf = L.OrEcrit.from_buffer_copy(b[o:o + C.sizeof(L.OrEcrit)])
+    o += C.sizeof(L.OrEcrit)
+    thf = np.frombuffer(b[o:], np.float64)
     g = E.GpuModel(m)
+    g.th_setup([0], [0], [0])
     g.run(59)
-    ec = g.run_ecrit()
+    ec = g.run_th()
     x, v, vr, d = g.nodes6()
     ts = g.time_state()
     g.close()
+    # the time-history record of the output cycle, as OR_MODEL_RUN_TH returned it to Fortran
+    npt = m.q4.prop.npt
+    assert thf.size == 4 + 18 + 13 + 12 + npt
+    assert np.array_equal(thf[:4], [ec["xmomt"], ec["ymomt"], ec["zmomt"], ec["ehour"]])
+    assert np.array_equal(thf[4:22], ec["nodes"][0]) and np.array_equal(thf[22:35], ec["hex"][0])
+    assert np.array_equal(thf[35:], ec["q4"][0]) and ec["ehour"] > 0
     assert tsf.ncycle == ts.ncycle == 60 and tsf.tt == ts.tt
     assert np.array_equal(xf, x) and np.array_equal(vf, v) and np.array_equal(vrf, vr) and np.array_equal(df, d)
     # the listing record of the output cycle, as OR_MODEL_RUN_ECRIT returned it to Fortran
