@@ -34,6 +34,15 @@ def test_null_arguments_are_rejected():
     assert b"null" in lib.or_last_error()
 
 
+def test_time_history_calls_reject_null_arguments():
+    lib = L.load()
+    assert lib.or_model_th_setup(None, 0, None, 0, None, 0, None) == L.OR_ERR_ARG
+    assert lib.or_model_run_th(None, None, None, None, None, None) == L.OR_ERR_ARG
+    assert lib.or_model_ehour(None, None) == L.OR_ERR_ARG
+    assert lib.or_model_set_ehour(None, 0.0) == L.OR_ERR_ARG
+    assert b"null" in lib.or_last_error()
+
+
 def test_bad_sizes_are_rejected_before_any_device_call():
     lib = L.load()
     h = C.c_void_p()
