@@ -4,7 +4,7 @@ import numpy as np
 from openradioss_amd import model as M
 
 CASES = {
-    # elastic-plastic cantilever bar (configs[0] family, reduced), 10 m/s
+    # elastic-plastic cantilever bar (a reduced 20x20x80 bar), 10 m/s
     "bar_6x5x12_c120": dict(kind="bar", nx=6, ny=5, nz=12, v0=1.0e4, cycles=120),
     # strongly plastic impact (pow branch of M2LAW every cycle), 60 m/s
     "impact_4x4x16_c120": dict(kind="bar", nx=4, ny=4, nz=16, v0=6.0e4, cycles=120),
@@ -48,7 +48,7 @@ def build_case(c):
 
 # ---------------------------------------------------------------- Q4 shells and mixed models
 Q4_CASES = {
-    # axial crush of a small square tube (configs[1] family): Newton return, thickness change
+    # axial crush of a small square tube (a reduced 60x300 tube): Newton return, thickness change
     "tube_8x20_c150": dict(kind="tube", width=8.0, length=20.0, ipla=1, ithk=1, v_top=-1.5e4, cycles=150),
     # strain-rate hardening with filtering and progressive rupture at 1% plastic strain
     "tube_rupture_c150": dict(kind="tube_rupture", width=8.0, length=20.0, v_top=-4.0e4, cycles=150),
@@ -59,7 +59,7 @@ Q4_CASES = {
     # step: CDT3 with IDTMIN(3) = 2 deletes the elements whose time step falls below it
     # (cdt3.F:147-165), and the run continues without them
     "plate_dtmin_c120": dict(kind="plate", seed=5, ipla=0, ithk=0, cycles=120, dtmin=1.4e-7),
-    # hex block with a shell plate tied on its top face (configs[2] family)
+    # hex block with a shell plate tied on its top face (a reduced 160^3 block with its plate)
     "mixed_c120": dict(kind="mixed", seed=7, cycles=120),
 }
 
