@@ -92,3 +92,30 @@ def test_ehour_through_a_checkpoint():
         assert r.ehour() == float(ck["ehour"][0])
         r.run(20)
         assert abs(r.ehour() - whole) <= 1e-13 * whole
+
+
+def test_deck_time_history_equals_oracle():
+    """A deck's /TH/NODE, /TH/BRIC and /TH/SHEL groups (read by the deck reader, header built from
+    its parts and titles) written on the GPU and by the oracle."""
+    import os
+    import test_th_run as TR
+    from openradioss_amd import deck as D
+    here = os.path.dirname(os.path.abspath(__file__))
+    text = open(os.path.join(here, "golden", "deck_brick_shell.rad")).read().replace("/END\n", "") + TR.DECK_TH
+    deck = D.parse(text)
+    m = D.to_model(deck)
+    h = TH.header_from_deck(deck, m, date="Thu Dec  3 12:00:00 2026")
+    probe = O.ModelOracle(m)
+    probe.cycle(1)
+    dthis = 3 * probe.ts.dt2
+    ref, recs, this_o = TO.oracle_time_history(m, h, dthis, 1.0, 40)
+    with E.GpuModel(m) as g:
+        data, this_g = g.run_time_history(h, dthis, 1.0, max_cycles=40)
+    a, b = TH.read(data), TH.read(ref)
+    assert this_g == this_o and len(a["times"]) == len(b["times"]) >= 10
+    ra, rb = _records(data), _records(ref)
+    assert ra[:a["header_records"]] == rb[:b["header_records"]]
+    assert [g["names"] for g in a["groups"]] == [["", "second"], ["", "tip"], [""]]
+    for ta, tb in zip(a["times"], b["times"]):
+        assert ta["time"] == tb["time"]
+        assert all(x.tobytes() == y.tobytes() for x, y in zip(ta["groups"], tb["groups"]))
