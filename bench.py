@@ -130,8 +130,10 @@ def cpu_baseline(model, budget_s: float = 12.0):
 
 def small_leg(cycles: int = 1000) -> tuple[dict, dict]:
     """The small reference workloads (bar 20x20x80, 32,000 hexes; tube 60x300, 72,000 shells), FP64:
-    `cycles` timed cycles from the initial state (after 20 warm-up cycles on a separate instance);
-    at these sizes a cycle is bound by the three dependent kernel launches, not by the kernels' work.
+    `cycles` timed cycles from the initial state (after 20 warm-up cycles and the launch-variant
+    autotuning on a separate instance);
+    at these sizes each SIMD holds at most one or two waves of an element kernel, so a cycle is bound
+    by latency (one element's chain, and the three dependent launches), not by the kernels' work.
     Then each run continues to the length of its reference fixture (bar 1,000 cycles, tube 8,800)
     and is compared with the state the reference's own compiled routines (glibc libm) reached
     there: the parity block (displacement and energy deviations)."""
@@ -149,12 +151,15 @@ def small_leg(cycles: int = 1000) -> tuple[dict, dict]:
         ne = m.ne_hex + m.ne_q4
         with E.GpuModel(m, precision="fp64") as g:
             g.run(20)
+            vh, vq = g.autotune(5)       # every variant gives the same bits
         with E.GpuModel(m, precision="fp64") as g:
+            g.set_variant("hex", vh)
+            g.set_variant("q4", vq)
             t0 = time.perf_counter()
             g.run(cycles)
             t = time.perf_counter() - t0
             out[name] = dict(elements=ne, cycles=cycles, us_per_cycle=1e6 * t / cycles,
-                             element_updates_per_s=ne * cycles / t)
+                             element_updates_per_s=ne * cycles / t, kernel_variants={"hex": vh, "q4": vq})
             g.run(full - cycles)
             ts = g.time_state()
             x, v, vr, d = g.nodes6()
@@ -445,7 +450,8 @@ def main():
     if not args.no_small:
         out["small_models"], out["parity"] = small_leg()
         for k, r in out["small_models"].items():
-            log(f"{k}: {r['element_updates_per_s']:.3e} element-updates/s, {r['us_per_cycle']:.1f} us per cycle")
+            log(f"{k}: {r['element_updates_per_s']:.3e} element-updates/s, {r['us_per_cycle']:.1f} us per cycle "
+                f"(variants {r['kernel_variants']})")
         for k, r in out["parity"].items():
             log(f"parity {k} at cycle {r['cycles']}: displacement {r['d_rel_err']:.2e}, "
                 f"energy {r.get('energy_rel_err', float('nan')):.2e} against the reference")
