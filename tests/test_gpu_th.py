@@ -7,7 +7,8 @@ or_model_run_th) against the oracle's, written by the same writer:
 * the global variables within 2 float32 ulps: the GPU sums the energies, momenta, external work
   and EHOUR in a fixed tree order (EHOUR per tile over the cycles), the oracle node after node and
   group after group, so their FP64 values differ in the last bits;
-* EHOUR carried through a checkpoint.
+* EHOUR carried through a checkpoint;
+* the split form's file: the FP64 run's layout and output cycles, its own element state.
 """
 import numpy as np
 import pytest
@@ -119,3 +120,41 @@ def test_deck_time_history_equals_oracle():
     for ta, tb in zip(a["times"], b["times"]):
         assert ta["time"] == tb["time"]
         assert all(x.tobytes() == y.tobytes() for x, y in zip(ta["groups"], tb["groups"]))
+
+
+def test_split_form_time_history():
+    """The split form (FP32 element arithmetic and state, FP64 nodes): the same header and output
+    cycles as the FP64 run, times within 1e-5, node displacements within 1e-4 of the largest
+    displacement (the split form's tolerance) and velocities within 1e-3 of the largest velocity.
+    Its element values are its own state: with a record every cycle, the last record's solid
+    values (SIG, EINT, RHO, QVIS, PLA, EPSD) equal the split run's downloaded state after that
+    cycle, converted to REAL*4."""
+    m = M.block_with_plate(n=8, plate=16, seed=11)
+    h = _groups(m)
+    probe = O.ModelOracle(m)
+    probe.cycle(1)
+    dthis = 4 * probe.ts.dt2
+    with E.GpuModel(m) as g:
+        d64, _ = g.run_time_history(h, dthis, 1.0, max_cycles=60)
+    with E.GpuModel(m, precision="split") as g:
+        d32, _ = g.run_time_history(h, dthis, 1.0, max_cycles=60)
+    a, b = TH.read(d32), TH.read(d64)
+    k = b["header_records"]
+    assert _records(d32)[:k] == _records(d64)[:k]
+    assert len(a["times"]) == len(b["times"]) >= 15
+    j = [g.kind for g in h.groups].index("NODE")
+    for ta, tb in zip(a["times"], b["times"]):
+        assert abs(float(ta["time"]) - float(tb["time"])) <= 1e-5 * float(tb["time"])
+        for cols, tol in (([0, 1, 2], 1e-4), ([3, 4, 5], 1e-3)):     # D, V
+            x = ta["groups"][j][:, cols].astype(np.float64)
+            y = tb["groups"][j][:, cols].astype(np.float64)
+            assert np.all(np.abs(x - y) <= tol * np.abs(y).max()), (cols, x, y)
+    with E.GpuModel(m, precision="split") as g:
+        every, _ = g.run_time_history(h, 1.0e-30, 1.0, max_cycles=30)
+        hs = g.hex_state()
+    last = TH.read(every)["times"][-1]
+    e = h.groups[[g.kind for g in h.groups].index("BRIC")].index
+    bg = last["groups"][[g.kind for g in h.groups].index("BRIC")]
+    assert np.array_equal(bg[:, 1:7], hs["sig"][:, e].T.astype(np.float32))
+    for col, key in ((7, "eint"), (8, "rho"), (9, "qvis"), (11, "pla"), (13, "epsd")):
+        assert np.array_equal(bg[:, col], hs[key][e].astype(np.float32)), key
